@@ -1,0 +1,151 @@
+/* The MI355X rendering core: C ABI of the HIP device library (libphoton_hip.so, gfx950).
+ *
+ * This is the drop-in boundary for the reference's GPU path. The reference's GpuRenderer / Svgf C++ API
+ * (src/render/gpu_renderer.h:16-75, src/denoise/svgf.h:24-59) is kept as the header-only wrapper
+ * include/photon/gpu_renderer.hpp, whose methods call these entry points one for one:
+ *
+ *   ph_device_count / ph_device_info  <- cuda_available(GpuInfo*)           gpu_renderer.h:27-34
+ *   ph_renderer_create / _destroy     <- GpuRenderer() / ~GpuRenderer()      gpu_renderer.h:38-39
+ *   ph_renderer_upload                <- GpuRenderer::upload                 gpu_renderer.h:44
+ *   ph_renderer_update_lights         <- GpuRenderer::update_lights          gpu_renderer.h:46
+ *   ph_renderer_resize / _reset       <- GpuRenderer::resize / reset         gpu_renderer.h:47-48
+ *   ph_renderer_render                <- GpuRenderer::render                 gpu_renderer.h:51
+ *   ph_renderer_render_denoised       <- GpuRenderer::render_denoised        gpu_renderer.h:53
+ *   ph_renderer_display               <- GpuRenderer::display                gpu_renderer.h:56
+ *   ph_renderer_download_image        <- GpuRenderer::download_image         gpu_renderer.h:58
+ *   ph_renderer_download_display      <- GpuRenderer::download_display       gpu_renderer.h:59
+ *   ph_renderer_width/height/samples/last_frame_ms/display_buffer            gpu_renderer.h:61-65
+ *   ph_svgf_*                         <- Svgf::resize / reset / run / variance_buffer  svgf.h:32-59
+ * New entry points with no reference counterpart: ray-buffer tracing (traversal workload), engine
+ * selection and the autotuner of the wavefront engine, per-kernel statistics, and the serving loop.
+ *
+ * Conventions: every function returns 0 on success and a negative code on failure, with a message in
+ * ph_last_error() for the calling thread (the C++ wrapper turns it into false/error or an exception,
+ * as the reference does). Render, display and download calls block until the device work is done,
+ * as the reference's do. Pointers named *_dev are device memory; all others are host memory.
+ */
+#ifndef PHOTON_HIP_H
+#define PHOTON_HIP_H
+
+#include "photon_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PH_ENGINE_MEGAKERNEL 0 /* one thread per pixel, whole paths (the reference's structure) */
+#define PH_ENGINE_WAVEFRONT 1  /* queue-sorted wavefront with persistent kernels */
+
+typedef struct ph_gpu_info {
+    char name[256];
+    char arch[64];
+    int64_t memory;
+    int32_t compute_units;
+    int32_t wavefront_size;
+    int32_t lds_per_block;
+    int32_t clock_khz;
+    int32_t memory_clock_khz;
+    int32_t memory_bus_width;
+} ph_gpu_info;
+
+/* Launch shape of the wavefront engine (chosen by ph_renderer_autotune or set explicitly). */
+typedef struct ph_wavefront_config {
+    int32_t block;         /* threads per block of the traversal kernels: 64, 128, 256 */
+    int32_t stack_depth;   /* LDS stack entries per lane: 8, 16, 24, 32, 64 (rest spills to scratch) */
+    int32_t path_slots;    /* paths in flight (0: one per pixel, capped by memory) */
+    int32_t blocks_per_cu; /* persistent grid: blocks per compute unit */
+    int32_t sort_materials;/* shade in material-class queues */
+    int32_t use_graph;     /* replay the bounce loop from a captured HIP graph */
+    int32_t graph_iterations; /* bounces per captured graph */
+} ph_wavefront_config;
+
+/* Statistics of the last render call. */
+typedef struct ph_render_stats {
+    double ms_total;          /* wall time of the call */
+    double ms_extend;         /* summed time of closest-hit traversal launches (HIP events) */
+    double ms_shadow;         /* summed time of any-hit traversal launches */
+    double ms_shade;          /* summed time of shading launches */
+    double ms_other;          /* generation, accumulation, bookkeeping */
+    int64_t extend_launches, shadow_launches, shade_launches, iterations;
+    int64_t rays_extend, rays_shadow; /* rays traced */
+    int64_t samples;
+    int64_t node_visits, tri_tests, alpha_tests; /* filled only when counting is enabled */
+} ph_render_stats;
+
+/* The five G-buffer planes the reference denoiser consumes (SvgfInput, svgf.h:24-30), as device
+ * pointers to 4 floats per pixel. */
+typedef struct ph_svgf_input {
+    const float* color_dev;
+    const float* albedo_dev;
+    const float* normal_depth_dev; /* xyz normal, w distance (0 for misses) */
+    const float* position_dev;     /* xyz position, w material id (-1 for misses) */
+    const float* emission_dev;
+} ph_svgf_input;
+
+const char* ph_last_error(void);
+int ph_device_count(void);
+int ph_device_info(int device, ph_gpu_info* out);
+
+/* Device memory helpers for callers without a HIP toolchain of their own (bindings, tests). */
+int ph_device_alloc(int64_t bytes, void** out_dev);
+int ph_device_free(void* dev);
+int ph_copy_to_device(void* dst_dev, const void* src, int64_t bytes);
+int ph_copy_to_host(void* dst, const void* src_dev, int64_t bytes);
+int ph_device_memset(void* dev, int value, int64_t bytes);
+
+typedef struct ph_renderer ph_renderer;
+int ph_renderer_create(ph_renderer** out);
+void ph_renderer_destroy(ph_renderer* r);
+int ph_renderer_upload(ph_renderer* r, const ph_scene_desc* scene);
+int ph_renderer_update_lights(ph_renderer* r, const ph_scene_desc* scene);
+int ph_renderer_resize(ph_renderer* r, int width, int height);
+int ph_renderer_reset(ph_renderer* r);
+int ph_renderer_render(ph_renderer* r, const ph_camera_view* camera, const ph_render_settings* settings, int spp);
+int ph_renderer_render_denoised(ph_renderer* r, const ph_camera_view* camera, const ph_camera_view* previous, const ph_render_settings* settings,
+                                const ph_svgf_settings* svgf);
+int ph_renderer_display(ph_renderer* r, const ph_render_settings* settings, int mode, uint32_t* target_dev);
+int ph_renderer_download_image(ph_renderer* r, float* rgb);
+int ph_renderer_download_display(ph_renderer* r, uint32_t* rgba);
+int ph_renderer_width(const ph_renderer* r);
+int ph_renderer_height(const ph_renderer* r);
+int ph_renderer_samples(const ph_renderer* r);
+float ph_renderer_last_frame_ms(const ph_renderer* r);
+uint32_t* ph_renderer_display_buffer(const ph_renderer* r);
+
+/* Engine selection, tuning and statistics. */
+int ph_renderer_set_engine(ph_renderer* r, int engine);
+int ph_renderer_get_engine(const ph_renderer* r);
+int ph_renderer_set_wavefront_config(ph_renderer* r, const ph_wavefront_config* cfg);
+int ph_renderer_get_wavefront_config(const ph_renderer* r, ph_wavefront_config* cfg);
+/* Times candidate launch shapes on a short render of the given view and keeps the fastest. */
+int ph_renderer_autotune(ph_renderer* r, const ph_camera_view* camera, const ph_render_settings* settings, int spp, ph_wavefront_config* chosen);
+int ph_renderer_set_counting(ph_renderer* r, int enable);
+int ph_renderer_stats(const ph_renderer* r, ph_render_stats* out);
+/* Device pointers of the frame buffers (4 floats per pixel): 0 accum (rgb sum, w sum of luminance^2),
+ * 1 albedo, 2 normal+depth, 3 colour, 4 position+material, 5 emission, 6 denoised; 7 is the heat map
+ * (1 float per pixel: BVH nodes visited by the pixel-centre primary ray). */
+const float* ph_renderer_buffer(const ph_renderer* r, int which);
+
+/* Traversal of a ray buffer against the uploaded scene: closest hit (t, tri, u, v) or any hit
+ * (t, tri). `counts_dev` (optional, 3 uint32 per ray): nodes visited, triangles tested, alpha tests. */
+int ph_trace(ph_renderer* r, const ph_ray* rays_dev, int n, int anyhit, ph_hit* hits_dev, uint32_t* counts_dev);
+/* The same as ph_trace with an explicit launch shape (block 64/128/256, stack depth 8..64), timed with
+ * HIP events on the library's stream; ms may be NULL. */
+int ph_trace_config(ph_renderer* r, const ph_ray* rays_dev, int n, int anyhit, ph_hit* hits_dev, int block, int stack_depth, int persistent,
+                    float* ms);
+
+/* SVGF on caller G-buffers (the reference's Svgf class). */
+typedef struct ph_svgf ph_svgf;
+int ph_svgf_create(ph_svgf** out);
+void ph_svgf_destroy(ph_svgf* f);
+int ph_svgf_resize(ph_svgf* f, int width, int height);
+int ph_svgf_reset(ph_svgf* f);
+int ph_svgf_run(ph_svgf* f, const ph_svgf_input* in, const ph_camera_view* previous_camera, const ph_svgf_settings* settings, float* output_dev);
+const float* ph_svgf_variance_buffer(const ph_svgf* f);
+float ph_svgf_last_ms(const ph_svgf* f);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PHOTON_HIP_H */

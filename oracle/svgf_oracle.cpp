@@ -1,0 +1,278 @@
+// ORACLE — test infrastructure only (see oracle.h).
+//
+// CPU restatement of the reference SVGF denoiser (src/denoise/svgf.cu). The reference has no CPU
+// version, so this is the only executable statement of its arithmetic; every pixel is computed as the
+// CUDA kernels compute it, in the same order:
+//   demodulate / project ....... svgf.cu:19-35
+//   temporal accumulation ...... svgf.cu:37-92   (4-tap bilinear history, material/depth/normal tests)
+//   variance ................... svgf.cu:96-132  (7x7 bilateral estimate while the history is short)
+//   a-trous pass ............... svgf.cu:134-186 (3x3 variance prefilter, 5x5 B-spline, 3 edge stops)
+//   history / remodulate ....... svgf.cu:188-210
+//   pass order ................. svgf.cu:238-255 (history taken after the first a-trous pass)
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "oracle.h"
+#include "oracle_math.hpp"
+
+using namespace orc;
+
+struct or_svgf {
+    int w = 0, h = 0;
+    bool has_history = false;
+    std::vector<V4> plane[8];  // prev_illum, prev_moments, prev_nd, prev_pos, illum, moments, filter0, filter1
+};
+
+namespace {
+
+struct Inputs {
+    const V4 *color, *albedo, *nd, *pos, *emission;
+};
+
+V3 demod(V3 color, V3 emission, V3 albedo) {
+    V3 illum = vmax3(color - emission, v3s(0));
+    return v3(illum.x / fmaxf(albedo.x, 1e-3f), illum.y / fmaxf(albedo.y, 1e-3f), illum.z / fmaxf(albedo.z, 1e-3f));
+}
+
+bool reproject(const ph_camera_view& c, V3 p, int w, int h, float& px, float& py, float& dist) {
+    V3 d = p - v3p(c.position);
+    float z = dot(d, v3p(c.forward));
+    if (z <= 1e-4f) return false;
+    float sx = dot(d, v3p(c.right)) / (z * c.tan_half_fov * c.aspect);
+    float sy = dot(d, v3p(c.up)) / (z * c.tan_half_fov);
+    px = (sx + 1.0f) * 0.5f * float(w) - 0.5f;
+    py = (1.0f - sy) * 0.5f * float(h) - 0.5f;
+    dist = sqrtf(dot(d, d));
+    return true;
+}
+
+float mixf(float a, float b, float t) { return a + (b - a) * t; }
+
+void temporal(or_svgf& f, const Inputs& in, const ph_camera_view& prev_cam, float alpha_c, float alpha_m) {
+    const int w = f.w, h = f.h;
+    const V4 *prev_illum = f.plane[0].data(), *prev_moments = f.plane[1].data(), *prev_nd = f.plane[2].data(), *prev_pos = f.plane[3].data();
+    V4 *out_illum = f.plane[4].data(), *out_moments = f.plane[5].data();
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            int i = y * w + x;
+            V4 pos = in.pos[i];
+            V3 cur = demod(in.color[i].xyz(), in.emission[i].xyz(), in.albedo[i].xyz());
+            float l = lum(cur);
+            V3 hist_c = v3s(0);
+            float hm0 = 0, hm1 = 0;
+            float hist_len = 0, wsum = 0;
+            if (f.has_history && pos.w >= 0) {
+                float px, py, dist;
+                if (reproject(prev_cam, pos.xyz(), w, h, px, py, dist)) {
+                    V3 n = in.nd[i].xyz();
+                    int x0 = int(floorf(px)), y0 = int(floorf(py));
+                    float fx = px - float(x0), fy = py - float(y0);
+                    for (int k = 0; k < 4; k++) {
+                        int xx = x0 + (k & 1), yy = y0 + (k >> 1);
+                        if (xx < 0 || yy < 0 || xx >= w || yy >= h) continue;
+                        int j = yy * w + xx;
+                        V4 pp = prev_pos[j];
+                        V4 pn = prev_nd[j];
+                        if (pp.w != pos.w) continue;
+                        if (fabsf(pn.w - dist) > 0.05f * dist + 0.01f) continue;
+                        if (dot(pn.xyz(), n) < 0.9f) continue;
+                        float bw = ((k & 1) ? fx : 1 - fx) * ((k >> 1) ? fy : 1 - fy);
+                        V4 pi = prev_illum[j];
+                        V4 pm = prev_moments[j];
+                        hist_c = hist_c + pi.xyz() * bw;
+                        hm0 = hm0 + pm.x * bw;
+                        hm1 = hm1 + pm.y * bw;
+                        hist_len += pi.w * bw;
+                        wsum += bw;
+                    }
+                }
+            }
+            V3 c;
+            float m0, m1, len;
+            if (wsum > 1e-3f) {
+                hist_c = hist_c / wsum;
+                float inv = 1.0f / wsum;
+                hm0 = hm0 * inv;
+                hm1 = hm1 * inv;
+                len = fminf(hist_len / wsum + 1.0f, 64.0f);
+                float ac = fmaxf(alpha_c, 1.0f / len), am = fmaxf(alpha_m, 1.0f / len);
+                c = lerp3(hist_c, cur, ac);
+                m0 = mixf(hm0, l, am);
+                m1 = mixf(hm1, l * l, am);
+            } else {
+                len = 1.0f;
+                c = cur;
+                m0 = l;
+                m1 = l * l;
+            }
+            float variance = fmaxf(0.0f, m1 - m0 * m0);
+            out_illum[i] = v4(c, len);
+            out_moments[i] = v4(m0, m1, variance, 0);
+        }
+}
+
+void variance(or_svgf& f, const Inputs& in, float phi_normal) {
+    const int w = f.w, h = f.h;
+    const V4 *illum = f.plane[4].data(), *moments = f.plane[5].data();
+    V4* out = f.plane[6].data();
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            int i = y * w + x;
+            V4 c = illum[i];
+            float len = c.w;
+            float var = moments[i].z;
+            if (len < 4.0f && in.pos[i].w >= 0) {
+                V3 n = in.nd[i].xyz();
+                float z = in.nd[i].w;
+                float wsum = 0, m1 = 0, m2 = 0;
+                V3 acc = v3s(0);
+                for (int dy = -3; dy <= 3; dy++)
+                    for (int dx = -3; dx <= 3; dx++) {
+                        int xx = x + dx, yy = y + dy;
+                        if (xx < 0 || yy < 0 || xx >= w || yy >= h) continue;
+                        int j = yy * w + xx;
+                        if (in.pos[j].w != in.pos[i].w) continue;
+                        V4 nd = in.nd[j];
+                        float wn = powf(fmaxf(0.0f, dot(n, nd.xyz())), phi_normal);
+                        float wz = expf(-fabsf(nd.w - z) / (0.05f * z + 1e-3f));
+                        float wt = wn * wz;
+                        V4 m = moments[j];
+                        m1 += m.x * wt;
+                        m2 += m.y * wt;
+                        acc = acc + illum[j].xyz() * wt;
+                        wsum += wt;
+                    }
+                if (wsum > 0) {
+                    m1 /= wsum;
+                    m2 /= wsum;
+                    var = fmaxf(0.0f, m2 - m1 * m1) * (4.0f / len);
+                    c = v4(acc / wsum, len);
+                }
+            }
+            out[i] = v4(c.xyz(), var);
+        }
+}
+
+void atrous(or_svgf& f, const Inputs& in, int step, float phi_color, float phi_normal, float phi_depth, const V4* src, V4* dst) {
+    const int w = f.w, h = f.h;
+    const float kernel[3] = {3.0f / 8.0f, 1.0f / 4.0f, 1.0f / 16.0f};
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            int i = y * w + x;
+            V4 center = src[i];
+            if (in.pos[i].w < 0) {
+                dst[i] = center;
+                continue;
+            }
+            float var = 0, vw = 0;
+            for (int dy = -1; dy <= 1; dy++)
+                for (int dx = -1; dx <= 1; dx++) {
+                    int xx = std::min(std::max(x + dx, 0), w - 1), yy = std::min(std::max(y + dy, 0), h - 1);
+                    float k = (dx == 0 && dy == 0) ? 0.25f : ((dx == 0 || dy == 0) ? 0.125f : 0.0625f);
+                    var += src[yy * w + xx].w * k;
+                    vw += k;
+                }
+            var /= vw;
+            V4 ndc = in.nd[i];
+            V3 n = ndc.xyz();
+            float z = ndc.w;
+            float zx = in.nd[y * w + std::min(x + 1, w - 1)].w - in.nd[y * w + std::max(x - 1, 0)].w;
+            float zy = in.nd[std::min(y + 1, h - 1) * w + x].w - in.nd[std::max(y - 1, 0) * w + x].w;
+            float grad = 0.5f * fmaxf(fabsf(zx), fabsf(zy));
+            float lum_c = lum(center.xyz());
+            float sigma_l = phi_color * sqrtf(fmaxf(var, 0.0f)) + 1e-4f;
+            V3 sum = v3s(0);
+            float wsum = 0, vsum = 0;
+            for (int dy = -2; dy <= 2; dy++)
+                for (int dx = -2; dx <= 2; dx++) {
+                    int xx = x + dx * step, yy = y + dy * step;
+                    if (xx < 0 || yy < 0 || xx >= w || yy >= h) continue;
+                    int j = yy * w + xx;
+                    if (in.pos[j].w != in.pos[i].w) continue;
+                    V4 s = src[j];
+                    V4 nd = in.nd[j];
+                    float k = kernel[std::abs(dx)] * kernel[std::abs(dy)];
+                    float wl = expf(-fabsf(lum(s.xyz()) - lum_c) / sigma_l);
+                    float wn = powf(fmaxf(0.0f, dot(n, nd.xyz())), phi_normal);
+                    float dist = sqrtf(float(dx * dx + dy * dy)) * float(step);
+                    float wz = expf(-fabsf(nd.w - z) / (phi_depth * grad * dist + 1e-3f * z + 1e-4f));
+                    float wt = k * wl * wn * wz;
+                    sum = sum + s.xyz() * wt;
+                    vsum += wt * wt * s.w;
+                    wsum += wt;
+                }
+            if (wsum > 0)
+                dst[i] = v4(sum / wsum, vsum / (wsum * wsum));
+            else
+                dst[i] = center;
+        }
+}
+
+void history(or_svgf& f, const Inputs& in, const V4* filtered) {
+    const int n = f.w * f.h;
+    for (int i = 0; i < n; i++) {
+        f.plane[0][size_t(i)] = v4(filtered[i].xyz(), f.plane[4][size_t(i)].w);
+        f.plane[1][size_t(i)] = f.plane[5][size_t(i)];
+        f.plane[2][size_t(i)] = in.nd[i];
+        f.plane[3][size_t(i)] = in.pos[i];
+    }
+}
+
+void remodulate(const or_svgf& f, const Inputs& in, const V4* filtered, V4* out) {
+    const int n = f.w * f.h;
+    for (int i = 0; i < n; i++) {
+        if (in.pos[i].w < 0) {
+            out[i] = in.color[i];
+            continue;
+        }
+        V3 a = in.albedo[i].xyz();
+        V3 c = filtered[i].xyz() * vmax3(a, v3s(1e-3f)) + in.emission[i].xyz();
+        out[i] = v4(c, 1);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int or_svgf_create(int width, int height, or_svgf** out) {
+    if (!out || width <= 0 || height <= 0) return -1;
+    or_svgf* f = new or_svgf();
+    f->w = width;
+    f->h = height;
+    for (auto& p : f->plane) p.assign(size_t(width) * size_t(height), V4{});
+    *out = f;
+    return 0;
+}
+
+void or_svgf_destroy(or_svgf* f) { delete f; }
+void or_svgf_reset(or_svgf* f) {
+    if (f) f->has_history = false;
+}
+
+int or_svgf_run(or_svgf* f, const float* color4, const float* albedo4, const float* normal_depth4, const float* position4, const float* emission4,
+                const ph_camera_view* prev_cam, const ph_svgf_settings* st, float* out4) {
+    if (!f || !color4 || !albedo4 || !normal_depth4 || !position4 || !emission4 || !prev_cam || !st || !out4) return -1;
+    Inputs in{reinterpret_cast<const V4*>(color4), reinterpret_cast<const V4*>(albedo4), reinterpret_cast<const V4*>(normal_depth4),
+              reinterpret_cast<const V4*>(position4), reinterpret_cast<const V4*>(emission4)};
+    temporal(*f, in, *prev_cam, st->alpha_color, st->alpha_moments);
+    variance(*f, in, st->phi_normal);
+    int src = 0;
+    for (int it = 0; it < st->atrous_iterations; it++) {
+        atrous(*f, in, 1 << it, st->phi_color, st->phi_normal, st->phi_depth, f->plane[6 + src].data(), f->plane[6 + (1 - src)].data());
+        src = 1 - src;
+        if (it == 0) history(*f, in, f->plane[6 + src].data());
+    }
+    if (st->atrous_iterations == 0) history(*f, in, f->plane[6 + src].data());
+    remodulate(*f, in, f->plane[6 + src].data(), reinterpret_cast<V4*>(out4));
+    f->has_history = true;
+    return 0;
+}
+
+const float* or_svgf_plane(const or_svgf* f, int which) {
+    if (!f || which < 0 || which >= 8) return nullptr;
+    return &f->plane[which][0].x;
+}
+
+}  // extern "C"

@@ -1,0 +1,174 @@
+// BVH traversal over 64-byte dual-box nodes for wave64: one lane per ray.
+//
+// Semantics are those of the reference traversal (src/render/traverse.h:24-147), which fixes the hit
+// that is reported when several triangles tie:
+//   * both child boxes are slab-tested against the tmax current at node entry;
+//   * leaf children are tested in place, left leaf before right leaf, shrinking tmax as they go;
+//   * interior children are re-tested against the shortened tmax, nearest first (left on ties), the
+//     farther one pushed; the stack holds at most 64 entries and a push beyond that is dropped;
+//   * Moller-Trumbore with |det| < 1e-14 rejection and 0 < t < tmax acceptance; alpha-masked
+//     triangles are skipped when the interpolated base-colour alpha is below the cutoff;
+//   * any-hit returns on the first accepted triangle and reports only t and the triangle.
+// The stack lives in LDS: each lane owns a column of `Depth` entries, interleaved across the lanes of
+// the block so that the 64 lanes of a wave touch 64 consecutive dwords (no bank conflicts); entries
+// beyond `Depth` spill to per-lane scratch in the same order, so the visit order never changes.
+#pragma once
+
+#include "dscene.h"
+
+namespace phd {
+
+struct HitRec {
+    float t;
+    int tri;
+    float u, v;
+};
+
+struct TraceCount {
+    uint32_t nodes, tris, alpha;
+};
+
+constexpr int kMaxStack = 64;
+
+template <int Depth>
+struct LaneStack {
+    int* column;  // this lane's first LDS entry
+    int stride;   // lanes per block
+    int spill[kMaxStack - Depth];
+    PH_DEV void store(int sp, int v) {
+        if (sp < Depth)
+            column[sp * stride] = v;
+        else
+            spill[sp - Depth] = v;
+    }
+    PH_DEV int load(int sp) const { return sp < Depth ? column[sp * stride] : spill[sp - Depth]; }
+};
+
+template <>
+struct LaneStack<kMaxStack> {
+    int* column;
+    int stride;
+    PH_DEV void store(int sp, int v) { column[sp * stride] = v; }
+    PH_DEV int load(int sp) const { return column[sp * stride]; }
+};
+
+PH_DEV bool mt_hit(f3 v0, f3 e1, f3 e2, f3 o, f3 d, float tmax, float& t, float& u, float& v) {
+    f3 p = cross(d, e2);
+    float det = dot(e1, p);
+    if (fabsf(det) < 1e-14f) return false;
+    float inv = 1.0f / det;
+    f3 s = o - v0;
+    u = dot(s, p) * inv;
+    if (u < 0.0f || u > 1.0f) return false;
+    f3 q = cross(s, e1);
+    v = dot(d, q) * inv;
+    if (v < 0.0f || u + v > 1.0f) return false;
+    t = dot(e2, q) * inv;
+    return t > 0.0f && t < tmax;
+}
+
+PH_DEV float slab(float4 lo, float4 hi, f3 o, f3 inv, float tmax) {
+    float tx0 = (lo.x - o.x) * inv.x, tx1 = (hi.x - o.x) * inv.x;
+    float ty0 = (lo.y - o.y) * inv.y, ty1 = (hi.y - o.y) * inv.y;
+    float tz0 = (lo.z - o.z) * inv.z, tz1 = (hi.z - o.z) * inv.z;
+    float tnear = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+    float tfar = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
+    return tnear <= tfar ? tnear : kInf;
+}
+
+PH_DEV f3 guarded_reciprocal(f3 d) {
+    auto r = [](float x) { return fabsf(x) > 1e-20f ? 1.0f / x : copysignf(1e20f, x); };
+    return {r(d.x), r(d.y), r(d.z)};
+}
+
+// True when the masked triangle lets the ray through at (u, v) (traverse.h:55-64).
+PH_DEV bool alpha_cut(const DScene& s, int tri, int material, float u, float v) {
+    const float4* mp = s.materials + 5 * material;
+    float alpha = mp[0].w;
+    float4 d = mp[3];
+    float cutoff = d.x;
+    int tex = __float_as_int(d.w);
+    if (tex >= 0 && s.uvs) {
+        const int* idx = s.indices + 3 * tri;
+        f2 uv = uv_at(s.uvs, idx[0]) * (1 - u - v) + uv_at(s.uvs, idx[1]) * u + uv_at(s.uvs, idx[2]) * v;
+        alpha *= tex_bilinear(s, tex, uv).w;
+    }
+    return alpha < cutoff;
+}
+
+template <bool AnyHit, bool Count, class Stack>
+PH_DEV bool traverse(const DScene& s, f3 o, f3 d, float tmax, HitRec& hit, Stack& stack, TraceCount* cnt) {
+    const f3 inv = guarded_reciprocal(d);
+    int sp = 0, node = 0;
+    bool found = false;
+    for (;;) {
+        const float4* np = s.nodes + 4 * node;
+        const float4 lmin = np[0], lmax = np[1], rmin = np[2], rmax = np[3];
+        if (Count) cnt->nodes++;
+        const float tl = slab(lmin, lmax, o, inv, tmax);
+        const float tr = slab(rmin, rmax, o, inv, tmax);
+        int next0 = 0, next1 = 0;
+        float tn0 = 0, tn1 = 0;
+        int nn = 0;
+#pragma unroll
+        for (int side = 0; side < 2; side++) {
+            const float tc = side == 0 ? tl : tr;
+            const int count = __float_as_int(side == 0 ? lmax.w : rmax.w);
+            const int idx = __float_as_int(side == 0 ? lmin.w : rmin.w);
+            if (tc == kInf) continue;
+            if (count > 0) {
+                for (int i = idx; i < idx + count; i++) {
+                    const float4* tp = s.tris + 3 * i;
+                    const float4 a = tp[0], b = tp[1], c = tp[2];
+                    float t, u, v;
+                    if (Count) cnt->tris++;
+                    if (mt_hit(xyz(a), xyz(b), xyz(c), o, d, tmax, t, u, v)) {
+                        if (__float_as_int(b.w) & PH_TRI_ALPHA_TESTED) {
+                            if (Count) cnt->alpha++;
+                            if (alpha_cut(s, i, __float_as_int(a.w), u, v)) continue;
+                        }
+                        hit.t = t;
+                        hit.tri = i;
+                        if (AnyHit) return true;
+                        tmax = t;
+                        hit.u = u;
+                        hit.v = v;
+                        found = true;
+                    }
+                }
+            } else if (idx >= 0) {
+                if (nn == 0) {
+                    next0 = idx;
+                    tn0 = tc;
+                } else {
+                    next1 = idx;
+                    tn1 = tc;
+                }
+                nn++;
+            }
+        }
+        if (nn == 2) {
+            if (tn0 > tn1) {
+                int a = next0;
+                next0 = next1;
+                next1 = a;
+                float b = tn0;
+                tn0 = tn1;
+                tn1 = b;
+            }
+            if (tn1 < tmax && sp < kMaxStack) stack.store(sp++, next1);
+            if (tn0 < tmax) {
+                node = next0;
+                continue;
+            }
+        } else if (nn == 1 && tn0 < tmax) {
+            node = next0;
+            continue;
+        }
+        if (sp == 0) break;
+        node = stack.load(--sp);
+    }
+    return found;
+}
+
+}  // namespace phd

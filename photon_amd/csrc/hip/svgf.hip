@@ -1,0 +1,324 @@
+// SVGF denoiser on the device (Schied et al. 2017), following the reference's arithmetic and pass order
+// (src/denoise/svgf.cu:19-255): temporal accumulation -> variance -> a-trous pass 0 -> history ->
+// a-trous passes 1..4 -> remodulation.
+//
+// MI355X layout: every plane is float4 per pixel (16-byte loads). The variance estimate and the
+// a-trous passes stage their footprint through LDS: a 16x16 output tile reads a (16 + 2r)^2 halo of the
+// G-buffer planes it needs once per block instead of once per tap. The larger a-trous steps (4, 8, 16)
+// gather directly from L2, where the footprint exceeds what a tile can usefully share.
+#include "internal.h"
+#include "svgf_state.h"
+
+using namespace phd;
+
+namespace phk {
+
+namespace {
+
+constexpr int TX = 16, TY = 16;
+
+PH_DEV f3 demod(f3 color, f3 emission, f3 albedo) {
+    f3 illum = vmax3(color - emission, mk3(0));
+    return mk3(illum.x / fmaxf(albedo.x, 1e-3f), illum.y / fmaxf(albedo.y, 1e-3f), illum.z / fmaxf(albedo.z, 1e-3f));
+}
+
+PH_DEV bool reproject(const ph_camera_view& c, f3 p, int w, int h, float& px, float& py, float& dist) {
+    f3 d = p - ld3(c.position);
+    float z = dot(d, ld3(c.forward));
+    if (z <= 1e-4f) return false;
+    float sx = dot(d, ld3(c.right)) / (z * c.tan_half_fov * c.aspect);
+    float sy = dot(d, ld3(c.up)) / (z * c.tan_half_fov);
+    px = (sx + 1.0f) * 0.5f * float(w) - 0.5f;
+    py = (1.0f - sy) * 0.5f * float(h) - 0.5f;
+    dist = __builtin_sqrtf(dot(d, d));
+    return true;
+}
+
+struct Planes {
+    const float4 *color, *albedo, *nd, *pos, *emission;
+};
+
+__global__ __launch_bounds__(TX* TY) void temporal_kernel(Planes in, ph_camera_view prev_cam, int w, int h, int has_history, float alpha_c,
+                                                           float alpha_m, const float4* prev_illum, const float4* prev_moments,
+                                                           const float4* prev_nd, const float4* prev_pos, float4* out_illum,
+                                                           float4* out_moments) {
+    const int x = blockIdx.x * TX + threadIdx.x, y = blockIdx.y * TY + threadIdx.y;
+    if (x >= w || y >= h) return;
+    const int i = y * w + x;
+    const float4 pos = in.pos[i];
+    const f3 cur = demod(xyz(in.color[i]), xyz(in.emission[i]), xyz(in.albedo[i]));
+    const float l = lum(cur);
+    f3 hist_c = mk3(0);
+    float hm0 = 0, hm1 = 0, hist_len = 0, wsum = 0;
+    if (has_history && pos.w >= 0) {
+        float px, py, dist;
+        if (reproject(prev_cam, xyz(pos), w, h, px, py, dist)) {
+            const f3 n = xyz(in.nd[i]);
+            const int x0 = int(floorf(px)), y0 = int(floorf(py));
+            const float fx = px - float(x0), fy = py - float(y0);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int xx = x0 + (k & 1), yy = y0 + (k >> 1);
+                if (xx < 0 || yy < 0 || xx >= w || yy >= h) continue;
+                const int j = yy * w + xx;
+                const float4 pp = prev_pos[j];
+                const float4 pn = prev_nd[j];
+                if (pp.w != pos.w) continue;
+                if (fabsf(pn.w - dist) > 0.05f * dist + 0.01f) continue;
+                if (dot(xyz(pn), n) < 0.9f) continue;
+                const float bw = ((k & 1) ? fx : 1 - fx) * ((k >> 1) ? fy : 1 - fy);
+                const float4 pi = prev_illum[j];
+                const float4 pm = prev_moments[j];
+                hist_c = hist_c + xyz(pi) * bw;
+                hm0 = hm0 + pm.x * bw;
+                hm1 = hm1 + pm.y * bw;
+                hist_len += pi.w * bw;
+                wsum += bw;
+            }
+        }
+    }
+    f3 c;
+    float m0, m1, len;
+    if (wsum > 1e-3f) {
+        hist_c = hist_c / wsum;
+        const float inv = 1.0f / wsum;
+        hm0 = hm0 * inv;
+        hm1 = hm1 * inv;
+        len = fminf(hist_len / wsum + 1.0f, 64.0f);
+        const float ac = fmaxf(alpha_c, 1.0f / len), am = fmaxf(alpha_m, 1.0f / len);
+        c = lerp3(hist_c, cur, ac);
+        m0 = hm0 + (l - hm0) * am;
+        m1 = hm1 + (l * l - hm1) * am;
+    } else {
+        len = 1.0f;
+        c = cur;
+        m0 = l;
+        m1 = l * l;
+    }
+    const float variance = fmaxf(0.0f, m1 - m0 * m0);
+    out_illum[i] = make_float4(c.x, c.y, c.z, len);
+    out_moments[i] = make_float4(m0, m1, variance, 0.0f);
+}
+
+// 7x7 bilateral variance estimate while the history is short (svgf.cu:96-132). The block stages the
+// 22x22 halo of normal+depth and position.w (material) in LDS; moments and illumination are read only
+// for accepted taps of short-history pixels.
+__global__ __launch_bounds__(TX* TY) void variance_kernel(Planes in, int w, int h, const float4* illum, const float4* moments, float phi_normal,
+                                                           float4* out) {
+    constexpr int R = 3, SX = TX + 2 * R, SY = TY + 2 * R;
+    __shared__ float4 s_nd[SY][SX];
+    __shared__ float s_mat[SY][SX];
+    const int bx = blockIdx.x * TX - R, by = blockIdx.y * TY - R;
+    for (int t = threadIdx.y * TX + threadIdx.x; t < SX * SY; t += TX * TY) {
+        const int sy = t / SX, sx = t % SX;
+        const int gx = bx + sx, gy = by + sy;
+        if (gx >= 0 && gy >= 0 && gx < w && gy < h) {
+            const int j = gy * w + gx;
+            s_nd[sy][sx] = in.nd[j];
+            s_mat[sy][sx] = in.pos[j].w;
+        }
+    }
+    __syncthreads();
+    const int x = blockIdx.x * TX + threadIdx.x, y = blockIdx.y * TY + threadIdx.y;
+    if (x >= w || y >= h) return;
+    const int i = y * w + x;
+    float4 c = illum[i];
+    const float len = c.w;
+    float var = moments[i].z;
+    const float mat = s_mat[threadIdx.y + R][threadIdx.x + R];
+    if (len < 4.0f && mat >= 0) {
+        const float4 ndc = s_nd[threadIdx.y + R][threadIdx.x + R];
+        const f3 n = xyz(ndc);
+        const float z = ndc.w;
+        float wsum = 0, m1 = 0, m2 = 0;
+        f3 acc = mk3(0);
+        for (int dy = -R; dy <= R; dy++)
+            for (int dx = -R; dx <= R; dx++) {
+                const int xx = x + dx, yy = y + dy;
+                if (xx < 0 || yy < 0 || xx >= w || yy >= h) continue;
+                const int ly = threadIdx.y + R + dy, lx = threadIdx.x + R + dx;
+                if (s_mat[ly][lx] != mat) continue;
+                const float4 nd = s_nd[ly][lx];
+                const float wn = pow_cr(fmaxf(0.0f, dot(n, xyz(nd))), phi_normal);
+                const float wz = exp_cr(-fabsf(nd.w - z) / (0.05f * z + 1e-3f));
+                const float wt = wn * wz;
+                const int j = yy * w + xx;
+                const float4 m = moments[j];
+                m1 += m.x * wt;
+                m2 += m.y * wt;
+                acc = acc + xyz(illum[j]) * wt;
+                wsum += wt;
+            }
+        if (wsum > 0) {
+            m1 /= wsum;
+            m2 /= wsum;
+            var = fmaxf(0.0f, m2 - m1 * m1) * (4.0f / len);
+            f3 a = acc / wsum;
+            c = make_float4(a.x, a.y, a.z, len);
+        }
+    }
+    out[i] = make_float4(c.x, c.y, c.z, var);
+}
+
+// One a-trous pass (svgf.cu:134-186). For steps 1 and 2 the block stages src, normal+depth and the
+// material id of its (16 + 4*step + 2)^2 footprint in LDS; larger steps read through the caches.
+template <int Step>
+__global__ __launch_bounds__(TX* TY) void atrous_kernel(Planes in, int w, int h, int step_rt, float phi_color, float phi_normal, float phi_depth,
+                                                         const float4* src, float4* dst) {
+    constexpr bool Tiled = Step > 0;
+    constexpr int R = Tiled ? 2 * Step + 1 : 1;  // taps reach 2*step; the 3x3 prefilter and gradient need 1
+    constexpr int SX = Tiled ? TX + 2 * R : 1, SY = Tiled ? TY + 2 * R : 1;
+    __shared__ float4 s_src[SY][SX];
+    __shared__ float4 s_nd[SY][SX];
+    __shared__ float s_mat[SY][SX];
+    const int step = Tiled ? Step : step_rt;
+    if (Tiled) {
+        const int bx = blockIdx.x * TX - R, by = blockIdx.y * TY - R;
+        for (int t = threadIdx.y * TX + threadIdx.x; t < SX * SY; t += TX * TY) {
+            const int sy = t / SX, sx = t % SX;
+            // Clamp to the image: the prefilter and the gradient clamp their coordinates, and the
+            // a-trous taps skip out-of-image positions before reading.
+            const int gx = min(max(blockIdx.x * TX - R + sx, 0), w - 1), gy = min(max(blockIdx.y * TY - R + sy, 0), h - 1);
+            const int j = gy * w + gx;
+            s_src[sy][sx] = src[j];
+            s_nd[sy][sx] = in.nd[j];
+            s_mat[sy][sx] = in.pos[j].w;
+            (void)bx;
+            (void)by;
+        }
+        __syncthreads();
+    }
+    const int x = blockIdx.x * TX + threadIdx.x, y = blockIdx.y * TY + threadIdx.y;
+    if (x >= w || y >= h) return;
+    const int i = y * w + x;
+    const int lx = threadIdx.x + R, ly = threadIdx.y + R;
+    // Loads through LDS (tiled) or global memory; (gx, gy) are image coordinates already in range.
+    auto SRC = [&](int gx, int gy) { return Tiled ? s_src[ly + gy - y][lx + gx - x] : src[gy * w + gx]; };
+    auto ND = [&](int gx, int gy) { return Tiled ? s_nd[ly + gy - y][lx + gx - x] : in.nd[gy * w + gx]; };
+    auto MAT = [&](int gx, int gy) { return Tiled ? s_mat[ly + gy - y][lx + gx - x] : in.pos[gy * w + gx].w; };
+    const float4 center = SRC(x, y);
+    const float mat = MAT(x, y);
+    if (mat < 0) {
+        dst[i] = center;
+        return;
+    }
+    float var = 0, vw = 0;
+#pragma unroll
+    for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+        for (int dx = -1; dx <= 1; dx++) {
+            const int xx = min(max(x + dx, 0), w - 1), yy = min(max(y + dy, 0), h - 1);
+            const float k = (dx == 0 && dy == 0) ? 0.25f : ((dx == 0 || dy == 0) ? 0.125f : 0.0625f);
+            var += SRC(xx, yy).w * k;
+            vw += k;
+        }
+    var /= vw;
+    const float4 ndc = ND(x, y);
+    const f3 n = xyz(ndc);
+    const float z = ndc.w;
+    const float zx = ND(min(x + 1, w - 1), y).w - ND(max(x - 1, 0), y).w;
+    const float zy = ND(x, min(y + 1, h - 1)).w - ND(x, max(y - 1, 0)).w;
+    const float grad = 0.5f * fmaxf(fabsf(zx), fabsf(zy));
+    const float lum_c = lum(xyz(center));
+    const float sigma_l = phi_color * __builtin_sqrtf(fmaxf(var, 0.0f)) + 1e-4f;
+    const float kernel[3] = {3.0f / 8.0f, 1.0f / 4.0f, 1.0f / 16.0f};
+    f3 sum = mk3(0);
+    float wsum = 0, vsum = 0;
+#pragma unroll
+    for (int dy = -2; dy <= 2; dy++)
+#pragma unroll
+        for (int dx = -2; dx <= 2; dx++) {
+            const int xx = x + dx * step, yy = y + dy * step;
+            if (xx < 0 || yy < 0 || xx >= w || yy >= h) continue;
+            if (MAT(xx, yy) != mat) continue;
+            const float4 sv = SRC(xx, yy);
+            const float4 nd = ND(xx, yy);
+            const float k = kernel[dx < 0 ? -dx : dx] * kernel[dy < 0 ? -dy : dy];
+            const float wl = exp_cr(-fabsf(lum(xyz(sv)) - lum_c) / sigma_l);
+            const float wn = pow_cr(fmaxf(0.0f, dot(n, xyz(nd))), phi_normal);
+            const float dist = __builtin_sqrtf(float(dx * dx + dy * dy)) * float(step);
+            const float wz = exp_cr(-fabsf(nd.w - z) / (phi_depth * grad * dist + 1e-3f * z + 1e-4f));
+            const float wt = k * wl * wn * wz;
+            sum = sum + xyz(sv) * wt;
+            vsum += wt * wt * sv.w;
+            wsum += wt;
+        }
+    if (wsum > 0) {
+        const f3 a = sum / wsum;
+        dst[i] = make_float4(a.x, a.y, a.z, vsum / (wsum * wsum));
+    } else {
+        dst[i] = center;
+    }
+}
+
+__global__ void history_kernel(Planes in, int n, const float4* filtered, const float4* illum, const float4* moments, float4* prev_illum,
+                               float4* prev_moments, float4* prev_nd, float4* prev_pos) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 f = filtered[i];
+    prev_illum[i] = make_float4(f.x, f.y, f.z, illum[i].w);
+    prev_moments[i] = moments[i];
+    prev_nd[i] = in.nd[i];
+    prev_pos[i] = in.pos[i];
+}
+
+__global__ void remodulate_kernel(Planes in, int n, const float4* filtered, float4* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (in.pos[i].w < 0) {
+        out[i] = in.color[i];
+        return;
+    }
+    const f3 a = xyz(in.albedo[i]);
+    const f3 c = xyz(filtered[i]) * vmax3(a, mk3(1e-3f)) + xyz(in.emission[i]);
+    out[i] = make_float4(c.x, c.y, c.z, 1.0f);
+}
+
+}  // namespace
+
+void svgf_resize(SvgfState& f, int width, int height) {
+    if (width == f.w && height == f.h && f.plane[0].ptr) return;
+    f.w = width;
+    f.h = height;
+    const size_t n = size_t(width) * size_t(height);
+    for (auto& p : f.plane) {
+        p.allocate(n);
+        PH_CHECK(hipMemset(p.ptr, 0, n * sizeof(float4)));
+    }
+    f.has_history = false;
+}
+
+void svgf_run(SvgfState& f, hipStream_t st, const ph_svgf_input& input, const ph_camera_view& prev_cam, const ph_svgf_settings& s, float4* output) {
+    const int w = f.w, h = f.h, n = w * h;
+    if (n == 0) return;
+    Planes in{reinterpret_cast<const float4*>(input.color_dev), reinterpret_cast<const float4*>(input.albedo_dev),
+              reinterpret_cast<const float4*>(input.normal_depth_dev), reinterpret_cast<const float4*>(input.position_dev),
+              reinterpret_cast<const float4*>(input.emission_dev)};
+    float4 *prev_illum = f.plane[0].ptr, *prev_moments = f.plane[1].ptr, *prev_nd = f.plane[2].ptr, *prev_pos = f.plane[3].ptr;
+    float4 *illum = f.plane[4].ptr, *moments = f.plane[5].ptr, *filter[2] = {f.plane[6].ptr, f.plane[7].ptr};
+    const dim3 tile(TX, TY), grid((w + TX - 1) / TX, (h + TY - 1) / TY);
+    const int lin = 256, lgrid = (n + lin - 1) / lin;
+    hipLaunchKernelGGL(temporal_kernel, grid, tile, 0, st, in, prev_cam, w, h, int(f.has_history), s.alpha_color, s.alpha_moments, prev_illum,
+                       prev_moments, prev_nd, prev_pos, illum, moments);
+    hipLaunchKernelGGL(variance_kernel, grid, tile, 0, st, in, w, h, illum, moments, s.phi_normal, filter[0]);
+    int src = 0;
+    for (int it = 0; it < s.atrous_iterations; it++) {
+        const int step = 1 << it;
+        if (step == 1)
+            hipLaunchKernelGGL(atrous_kernel<1>, grid, tile, 0, st, in, w, h, step, s.phi_color, s.phi_normal, s.phi_depth, filter[src], filter[1 - src]);
+        else if (step == 2)
+            hipLaunchKernelGGL(atrous_kernel<2>, grid, tile, 0, st, in, w, h, step, s.phi_color, s.phi_normal, s.phi_depth, filter[src], filter[1 - src]);
+        else
+            hipLaunchKernelGGL(atrous_kernel<0>, grid, tile, 0, st, in, w, h, step, s.phi_color, s.phi_normal, s.phi_depth, filter[src], filter[1 - src]);
+        src = 1 - src;
+        if (it == 0)
+            hipLaunchKernelGGL(history_kernel, dim3(lgrid), dim3(lin), 0, st, in, n, filter[src], illum, moments, prev_illum, prev_moments, prev_nd, prev_pos);
+    }
+    if (s.atrous_iterations == 0)
+        hipLaunchKernelGGL(history_kernel, dim3(lgrid), dim3(lin), 0, st, in, n, filter[src], illum, moments, prev_illum, prev_moments, prev_nd, prev_pos);
+    hipLaunchKernelGGL(remodulate_kernel, dim3(lgrid), dim3(lin), 0, st, in, n, filter[src], output);
+    PH_CHECK(hipGetLastError());
+    f.has_history = true;
+}
+
+}  // namespace phk

@@ -1,0 +1,354 @@
+// C ABI over the host scene producers (declared in include/photon_host.h).
+#include <cstring>
+#include <exception>
+#include <new>
+#include <string>
+
+#include "photon_host.h"
+#include "scene.hpp"
+
+using namespace phh;
+
+struct phs_scene {
+    Scene scene;
+    // Texture tables handed out by phs_scene_raw.
+    std::vector<int32_t> tex_w, tex_h, tex_repeat, tex_srgb;
+    std::vector<const uint32_t*> tex_ptr;
+};
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(const std::string& msg) {
+    g_error = msg;
+    return -1;
+}
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        return f();
+    } catch (const std::bad_alloc&) {
+        return fail("out of host memory");
+    } catch (const std::exception& e) {
+        return fail(e.what());
+    }
+}
+
+vec3 v3(const float* p) { return vec3(p[0], p[1], p[2]); }
+
+}  // namespace
+
+extern "C" {
+
+const char* phs_last_error(void) { return g_error.c_str(); }
+
+int phs_scene_create(phs_scene** out) {
+    if (!out) return fail("null output pointer");
+    return guarded([&] {
+        *out = new phs_scene();
+        return 0;
+    });
+}
+
+void phs_scene_destroy(phs_scene* s) { delete s; }
+
+int phs_scene_builtin(phs_scene* s, const char* name) {
+    if (!s || !name) return fail("null argument");
+    return guarded([&] { return make_builtin_scene(name, s->scene) ? 0 : fail(std::string("unknown built-in scene ") + name); });
+}
+
+int phs_scene_spheres(phs_scene* s, int count, float box, float alpha_fraction, uint64_t seed) {
+    if (!s || count < 0) return fail("invalid argument");
+    return guarded([&] {
+        make_sphere_soup(s->scene, count, box, alpha_fraction, seed);
+        return 0;
+    });
+}
+
+int phs_scene_field(phs_scene* s, int count, uint64_t seed, int env_width) {
+    if (!s || count < 0 || env_width < 2) return fail("invalid argument");
+    return guarded([&] {
+        make_sphere_field(s->scene, count, seed, env_width);
+        return 0;
+    });
+}
+
+int phs_scene_add_mesh(phs_scene* s, const float* positions, int num_vertices, const int32_t* indices, int num_indices, int material,
+                       const float* normals, const float* uvs) {
+    if (!s || (num_vertices > 0 && !positions) || (num_indices > 0 && !indices) || num_indices % 3) return fail("invalid mesh");
+    for (int i = 0; i < num_indices; i++)
+        if (indices[i] < 0 || indices[i] >= num_vertices) return fail("mesh index out of range");
+    return guarded([&] {
+        std::vector<vec3> pos(static_cast<size_t>(num_vertices)), nrm;
+        std::vector<vec2> uv;
+        for (int i = 0; i < num_vertices; i++) pos[size_t(i)] = v3(positions + 3 * i);
+        if (normals) {
+            nrm.resize(size_t(num_vertices));
+            for (int i = 0; i < num_vertices; i++) nrm[size_t(i)] = v3(normals + 3 * i);
+        }
+        if (uvs) {
+            uv.resize(size_t(num_vertices));
+            for (int i = 0; i < num_vertices; i++) uv[size_t(i)] = vec2(uvs[2 * i], uvs[2 * i + 1]);
+        }
+        std::vector<int> idx(indices, indices + num_indices);
+        s->scene.add_mesh(pos, idx, material, nrm, uv);
+        return 0;
+    });
+}
+
+int phs_scene_add_material(phs_scene* s, const ph_material* m) {
+    if (!s || !m) return fail("null argument");
+    return guarded([&] { return s->scene.add_material(*m); });
+}
+
+int phs_scene_set_material(phs_scene* s, int index, const ph_material* m) {
+    if (!s || !m || index < 0 || index >= int(s->scene.materials.size())) return fail("material index out of range");
+    s->scene.materials[size_t(index)] = *m;
+    return 0;
+}
+
+int phs_scene_get_material(const phs_scene* s, int index, ph_material* out) {
+    if (!s || !out || index < 0 || index >= int(s->scene.materials.size())) return fail("material index out of range");
+    *out = s->scene.materials[size_t(index)];
+    return 0;
+}
+
+int phs_scene_add_texture(phs_scene* s, int width, int height, const uint32_t* texels, int repeat, int srgb) {
+    if (!s || width <= 0 || height <= 0 || !texels) return fail("invalid texture");
+    return guarded([&] {
+        Texture t;
+        t.width = width;
+        t.height = height;
+        t.texels.assign(texels, texels + size_t(width) * size_t(height));
+        t.repeat = repeat != 0;
+        t.srgb = srgb != 0;
+        s->scene.textures.push_back(std::move(t));
+        return int(s->scene.textures.size()) - 1;
+    });
+}
+
+int phs_scene_add_punctual(phs_scene* s, const ph_punctual_light* l) {
+    if (!s || !l) return fail("null argument");
+    return guarded([&] {
+        s->scene.punctual.push_back(*l);
+        return int(s->scene.punctual.size()) - 1;
+    });
+}
+
+int phs_scene_set_env_color(phs_scene* s, const float rgb[3], float intensity) {
+    if (!s || !rgb) return fail("null argument");
+    s->scene.env = Environment();
+    s->scene.env.color = v3(rgb);
+    s->scene.env.intensity = intensity;
+    return 0;
+}
+
+int phs_scene_set_env_image(phs_scene* s, int width, int height, const float* rgb, float intensity, float rotation) {
+    if (!s || width <= 0 || height <= 0 || !rgb) return fail("invalid environment image");
+    return guarded([&] {
+        Environment& e = s->scene.env;
+        e.width = width;
+        e.height = height;
+        e.texels.resize(size_t(width) * size_t(height));
+        for (size_t i = 0; i < e.texels.size(); i++) e.texels[i] = v3(rgb + 3 * i);
+        e.intensity = intensity;
+        e.rotation = rotation;
+        return 0;
+    });
+}
+
+int phs_scene_set_sky(phs_scene* s, const float sun[3], float sun_intensity, int width) {
+    if (!s || !sun || width < 2) return fail("invalid argument");
+    return guarded([&] {
+        make_sky(s->scene.env, v3(sun), sun_intensity, width);
+        return 0;
+    });
+}
+
+int phs_scene_clear_normals(phs_scene* s) {
+    if (!s) return fail("null argument");
+    s->scene.normals.clear();
+    return 0;
+}
+
+int phs_scene_set_camera(phs_scene* s, const float position[3], const float target[3], const float up[3], float vfov, float aperture,
+                         float focus_distance) {
+    if (!s || !position || !target || !up) return fail("null argument");
+    Camera& c = s->scene.camera;
+    c.position = v3(position);
+    c.target = v3(target);
+    c.up = v3(up);
+    c.vfov = vfov;
+    c.aperture = aperture;
+    c.focus_distance = focus_distance;
+    return 0;
+}
+
+int phs_scene_get_camera(const phs_scene* s, float position[3], float target[3], float up[3], float* vfov) {
+    if (!s || !position || !target || !up || !vfov) return fail("null argument");
+    const Camera& c = s->scene.camera;
+    std::memcpy(position, &c.position, 12);
+    std::memcpy(target, &c.target, 12);
+    std::memcpy(up, &c.up, 12);
+    *vfov = c.vfov;
+    return 0;
+}
+
+int phs_scene_camera_view(const phs_scene* s, float aspect, ph_camera_view* out) {
+    if (!s || !out) return fail("null argument");
+    *out = s->scene.camera.view(aspect);
+    return 0;
+}
+
+int phs_scene_build(phs_scene* s, int bins, int max_leaf, float traversal_cost, int threads) {
+    if (!s) return fail("null argument");
+    if (bins > 64) return fail("at most 64 bins");
+    return guarded([&] {
+        BvhOptions o;
+        if (bins > 0) o.bins = bins;
+        if (max_leaf > 0) o.max_leaf_size = max_leaf;
+        if (traversal_cost > 0) o.traversal_cost = traversal_cost;
+        o.threads = threads > 0 ? threads : 0;
+        s->scene.build(o);
+        return 0;
+    });
+}
+
+int phs_scene_rebuild_lights(phs_scene* s) {
+    if (!s || !s->scene.built) return fail("the scene is not built");
+    return guarded([&] {
+        s->scene.rebuild_lights();
+        return 0;
+    });
+}
+
+int phs_scene_desc(const phs_scene* s, ph_scene_desc* out) {
+    if (!s || !out) return fail("null argument");
+    if (!s->scene.built) return fail("the scene is not built");
+    *out = s->scene.desc();
+    return 0;
+}
+
+int phs_scene_num_tris(const phs_scene* s) { return s ? s->scene.num_tris() : fail("null argument"); }
+int phs_scene_num_materials(const phs_scene* s) { return s ? int(s->scene.materials.size()) : fail("null argument"); }
+const int32_t* phs_scene_order(const phs_scene* s) { return s && s->scene.built ? s->scene.bvh.order.data() : nullptr; }
+
+int phs_scene_stats(const phs_scene* s, phs_bvh_stats* out) {
+    if (!s || !out) return fail("null argument");
+    const BvhStats& st = s->scene.bvh.stats;
+    out->nodes = st.nodes;
+    out->leaves = st.leaves;
+    out->max_depth = st.max_depth;
+    out->sah_cost = st.sah_cost;
+    out->build_ms = st.build_ms;
+    return 0;
+}
+
+int64_t phs_scene_device_bytes(const phs_scene* s) {
+    if (!s || !s->scene.built) return fail("the scene is not built");
+    const Scene& c = s->scene;
+    int64_t b = 0;
+    b += int64_t(c.bvh.nodes.size()) * 64 + int64_t(c.tris.size()) * 48 + int64_t(c.bvh_indices.size()) * 4;
+    b += int64_t(c.normals.size()) * 12 + int64_t(c.uvs.size()) * 8 + int64_t(c.tangents.size()) * 16;
+    b += int64_t(c.materials.size()) * 80 + int64_t(c.texture_infos.size()) * 20 + int64_t(c.texel_pool.size()) * 4 + 256 * 4;
+    b += int64_t(c.emissive.size()) * 8 + int64_t(c.tri_emissive.size()) * 4 + int64_t(c.emissive_alias.size()) * 16;
+    b += int64_t(c.punctual.size()) * 48 + int64_t(c.env.texels.size()) * 12;
+    b += int64_t(c.env_marginal.size() + c.env_conditional.size()) * 4;
+    return b;
+}
+
+int phs_scene_raw(phs_scene* s, phs_raw_scene* out) {
+    if (!s || !out) return fail("null argument");
+    const Scene& c = s->scene;
+    *out = phs_raw_scene{};
+    out->positions = c.positions.empty() ? nullptr : &c.positions[0].x;
+    out->num_vertices = int(c.positions.size());
+    out->normals = c.normals.empty() ? nullptr : &c.normals[0].x;
+    out->uvs = c.uvs.empty() ? nullptr : &c.uvs[0].x;
+    out->indices = c.indices.data();
+    out->num_tris = c.num_tris();
+    out->tri_material = c.tri_material.data();
+    out->materials = c.materials.data();
+    out->num_materials = int(c.materials.size());
+    s->tex_w.clear(), s->tex_h.clear(), s->tex_repeat.clear(), s->tex_srgb.clear(), s->tex_ptr.clear();
+    for (const Texture& t : c.textures) {
+        s->tex_w.push_back(t.width);
+        s->tex_h.push_back(t.height);
+        s->tex_repeat.push_back(t.repeat ? 1 : 0);
+        s->tex_srgb.push_back(t.srgb ? 1 : 0);
+        s->tex_ptr.push_back(t.texels.data());
+    }
+    out->num_textures = int(c.textures.size());
+    out->tex_width = s->tex_w.data();
+    out->tex_height = s->tex_h.data();
+    out->tex_texels = s->tex_ptr.data();
+    out->tex_repeat = s->tex_repeat.data();
+    out->tex_srgb = s->tex_srgb.data();
+    out->punctual = c.punctual.empty() ? nullptr : c.punctual.data();
+    out->num_punctual = int(c.punctual.size());
+    out->env_texels = c.env.texels.empty() ? nullptr : &c.env.texels[0].x;
+    out->env_width = c.env.width;
+    out->env_height = c.env.height;
+    std::memcpy(out->env_color, &c.env.color, 12);
+    out->env_intensity = c.env.intensity;
+    out->env_rotation = c.env.rotation;
+    std::memcpy(out->cam_position, &c.camera.position, 12);
+    std::memcpy(out->cam_target, &c.camera.target, 12);
+    std::memcpy(out->cam_up, &c.camera.up, 12);
+    out->cam_vfov = c.camera.vfov;
+    out->cam_aperture = c.camera.aperture;
+    out->cam_focus = c.camera.focus_distance;
+    return 0;
+}
+
+int phs_camera_rays(const ph_camera_view* cam, int width, int height, ph_ray* out) {
+    if (!cam || !out || width <= 0 || height <= 0) return fail("invalid argument");
+    vec3 fwd = v3(cam->forward), right = v3(cam->right), up = v3(cam->up), pos = v3(cam->position);
+    for (int y = 0; y < height; y++)
+        for (int x = 0; x < width; x++) {
+            float px = float(x) + 0.5f, py = float(y) + 0.5f;
+            float sx = (2.0f * px / float(width) - 1.0f) * cam->tan_half_fov * cam->aspect;
+            float sy = (1.0f - 2.0f * py / float(height)) * cam->tan_half_fov;
+            vec3 d = normalize(fwd + right * sx + up * sy);
+            ph_ray& r = out[size_t(y) * size_t(width) + size_t(x)];
+            std::memcpy(r.o, &pos, 12);
+            std::memcpy(r.d, &d, 12);
+            r.tmax = kInf;
+        }
+    return 0;
+}
+
+int phs_sphere_shadow_rays(const phs_scene* s, int count, uint64_t seed, ph_ray* out) {
+    if (!s || !out || count < 0) return fail("invalid argument");
+    const std::vector<vec4>& sp = s->scene.spheres;
+    if (sp.size() < 2) return fail("the scene has fewer than two procedural spheres");
+    Pcg32 rng(seed, 0x5AD0u);
+    const int n = int(sp.size());
+    auto on_sphere = [&](vec4 c, float scale) {
+        // Uniform direction from two draws (z = 1 - 2u, phi = 2 pi v), in double for portability.
+        double z = 1.0 - 2.0 * double(rng.next());
+        double r = std::sqrt(std::max(0.0, 1.0 - z * z));
+        double phi = 2.0 * 3.14159265358979323846 * double(rng.next());
+        vec3 d(float(r * std::cos(phi)), float(r * std::sin(phi)), float(z));
+        return vec3(c.x, c.y, c.z) + d * (c.w * scale);
+    };
+    for (int i = 0; i < count; i++) {
+        int a = std::min(n - 1, int(rng.next() * float(n)));
+        int b = std::min(n - 1, int(rng.next() * float(n)));
+        if (b == a) b = (a + 1) % n;
+        vec3 o = on_sphere(sp[size_t(a)], 1.06f);
+        vec3 t = on_sphere(sp[size_t(b)], 1.0f);
+        vec3 d = t - o;
+        float dist = length(d);
+        vec3 dir = d / dist;
+        ph_ray& r = out[size_t(i)];
+        std::memcpy(r.o, &o, 12);
+        std::memcpy(r.d, &dir, 12);
+        r.tmax = dist * (1.0f - 1e-4f);
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,225 @@
+"""Device path vs the CPU oracle on identical inputs.
+
+Bars (the workload's stated tolerances): primary-hit triangle IDs identical, hit t within 1e-6
+relative, per-pixel radiance within 0.1% relative on at least 99.9% of pixels, SVGF output within 1e-3
+relative. The device path is expected to be bit-identical almost everywhere; the bitwise fraction is
+printed next to each tolerance check.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle_py as O
+from photon_amd import gpu, host
+from photon_amd.types import (HIT_DTYPE, RAY_DTYPE, Material, RenderSettings, SvgfSettings, TONEMAP_LINEAR, DISPLAY_COLOR, DISPLAY_ALBEDO,
+                              DISPLAY_NORMAL, DISPLAY_DEPTH, DISPLAY_VARIANCE, DISPLAY_HEATMAP)
+
+from gpu_util import bitwise_fraction, radiance_agreement, sphere_rays, trace_on_gpu, uploaded
+
+pytestmark = pytest.mark.gpu
+
+
+def test_device_is_visible():
+    info = gpu.device_info(0)
+    print(f"device: {info.name.decode()} {info.arch.decode()} CUs={info.compute_units} mem={info.memory / 2**30:.0f} GiB "
+          f"wave={info.wavefront_size}")
+    assert info.compute_units > 0 and info.wavefront_size == 64
+
+
+@pytest.fixture(scope="module")
+def spheres():
+    return host.Scene.spheres(64, box=100.0, alpha_fraction=0.1, seed=11).build()
+
+
+def test_closest_hit_matches_oracle(spheres):
+    cam_rays, _ = sphere_rays(spheres, 256, 256, 0)
+    r = uploaded(spheres)
+    g = trace_on_gpu(r, cam_rays)
+    o = O.trace(spheres.desc(), cam_rays)
+    hit = o["tri"] >= 0
+    print(f"closest hit: {hit.mean():.3f} of rays hit; ids equal {np.mean(g['tri'] == o['tri']):.6f}")
+    assert hit.mean() > 0.05
+    assert np.array_equal(g["tri"], o["tri"])
+    t_rel = np.abs(g["t"][hit] - o["t"][hit]) / o["t"][hit]
+    assert t_rel.max(initial=0) <= 1e-6
+    assert np.array_equal(g["u"][hit], o["u"][hit]) and np.array_equal(g["v"][hit], o["v"][hit])
+
+
+def test_any_hit_matches_oracle(spheres):
+    _, shadow = sphere_rays(spheres, 8, 8, 65536)
+    r = uploaded(spheres)
+    g = trace_on_gpu(r, shadow, anyhit=True)
+    o = O.trace(spheres.desc(), shadow, anyhit=True)
+    occ = o["tri"] >= 0
+    print(f"any hit: {occ.mean():.3f} occluded")
+    assert np.array_equal(g["tri"] >= 0, occ)
+    assert np.array_equal(g["tri"], o["tri"])
+    assert np.array_equal(g["t"], o["t"])
+
+
+def test_traversal_counters_match_oracle(spheres):
+    cam_rays, _ = sphere_rays(spheres, 128, 128, 0)
+    r = uploaded(spheres)
+    g, gc = trace_on_gpu(r, cam_rays, counts=True)
+    o, oc = O.trace(spheres.desc(), cam_rays, counters=True)
+    assert np.array_equal(gc.astype(np.int64), oc)
+
+
+@pytest.mark.parametrize("block,depth,persistent", [(64, 8, True), (128, 16, True), (128, 24, False), (256, 32, True), (128, 64, False),
+                                                    (64, 16, False)])
+def test_launch_shapes_give_identical_hits(spheres, block, depth, persistent):
+    cam_rays, shadow = sphere_rays(spheres, 128, 128, 16384)
+    r = uploaded(spheres)
+    ref_c = O.trace(spheres.desc(), cam_rays)
+    ref_s = O.trace(spheres.desc(), shadow, anyhit=True)
+    for rays, ref, anyhit in ((cam_rays, ref_c, False), (shadow, ref_s, True)):
+        rd = gpu.DeviceArray.from_numpy(rays)
+        hd = gpu.DeviceArray(len(rays) * 16)
+        r.trace_config(rd.ptr, len(rays), hd.ptr, anyhit=anyhit, block=block, stack_depth=depth, persistent=persistent)
+        assert hd.numpy(HIT_DTYPE).tobytes() == ref.tobytes()
+
+
+def test_deep_stack_spills_in_order():
+    # A thin, deep BVH (leaves of one triangle along a line) forces stacks past the LDS depth.
+    s = host.Scene()
+    n = 4096
+    xs = np.arange(n, dtype=np.float32) * 0.01
+    pos = np.zeros((n * 3, 3), np.float32)
+    pos[0::3, 0] = xs
+    pos[1::3, 0] = xs + 0.005
+    pos[1::3, 1] = 0.0
+    pos[2::3, 0] = xs
+    pos[2::3, 1] = 0.005
+    pos[:, 2] = (np.arange(n * 3) // 3 % 7).astype(np.float32) * 0.001
+    m = s.add_material(Material.default())
+    s.add_mesh(pos, np.arange(n * 3, dtype=np.int32), m)
+    s.build(max_leaf=1)
+    rng = np.random.default_rng(3)
+    rays = np.zeros(8192, RAY_DTYPE)
+    rays["o"] = np.stack([rng.uniform(-1, 42, 8192), rng.uniform(-0.01, 0.01, 8192), np.full(8192, -1.0)], 1)
+    d = np.stack([rng.normal(0, 0.02, 8192), rng.normal(0, 0.02, 8192), np.ones(8192)], 1)
+    rays["d"] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    rays["tmax"] = 1e30
+    r = uploaded(s)
+    ref = O.trace(s.desc(), rays)
+    for depth in (8, 16, 32):
+        rd = gpu.DeviceArray.from_numpy(rays)
+        hd = gpu.DeviceArray(len(rays) * 16)
+        r.trace_config(rd.ptr, len(rays), hd.ptr, block=128, stack_depth=depth, persistent=False)
+        assert hd.numpy(HIT_DTYPE).tobytes() == ref.tobytes(), depth
+
+
+@pytest.mark.parametrize("name,w,h,spp,depth", [("cornell-c1", 128, 128, 16, 8), ("materials", 96, 54, 16, 8), ("glass", 96, 54, 16, 8),
+                                                ("cornell", 64, 64, 16, 8)])
+def test_megakernel_matches_oracle(name, w, h, spp, depth):
+    scene = host.Scene.builtin(name).build()
+    rs = RenderSettings.make(w, h, max_depth=depth, rr_depth=3, seed=1)
+    cam = scene.camera_view(w / h)
+    r = uploaded(scene, w, h, engine=gpu.ENGINE_MEGAKERNEL)
+    r.render(cam, rs, spp)
+    img = r.download_image()
+    ref = O.render(scene.desc(), cam, rs, spp)
+    agree, bits = radiance_agreement(img, ref), bitwise_fraction(img, ref)
+    print(f"{name}: {100 * agree:.3f}% within 0.1%, {100 * bits:.3f}% bitwise, means {img.mean():.5f}/{ref.mean():.5f}")
+    assert agree >= 0.999
+    assert r.samples == spp
+
+
+def test_gbuffer_matches_oracle():
+    scene = host.Scene.builtin("cornell-c1").build()
+    w, h = 96, 96
+    rs = RenderSettings.make(w, h, seed=3)
+    cam = scene.camera_view(1.0)
+    r = uploaded(scene, w, h)
+    r.render_denoised(cam, cam, rs, SvgfSettings.make())
+    planes = [gpu.copy_device_to_numpy(r.buffer_ptr(b), w * h * 16, np.float32, (h, w, 4))
+              for b in (gpu.BUF_COLOR, gpu.BUF_ALBEDO, gpu.BUF_NORMAL_DEPTH, gpu.BUF_POSITION, gpu.BUF_EMISSION)]
+    ref = O.render_gbuffer(scene.desc(), cam, rs, 0)
+    for name, a, b in zip(("color", "albedo", "nd", "position", "emission"), planes, ref):
+        frac = radiance_agreement(a, b)
+        print(f"{name}: {100 * frac:.3f}% within 0.1%, {100 * bitwise_fraction(a, b):.3f}% bitwise")
+        assert frac >= 0.999, name
+
+
+def _svgf_frames(scene, w, h, frames):
+    rs = RenderSettings.make(w, h, seed=7)
+    cam0 = scene.camera(); out = []
+    prev = None
+    for f in range(frames):
+        ang = np.deg2rad(0.5 * f)
+        p = np.array(cam0["position"], np.float64) - np.array(cam0["target"], np.float64)
+        c, s_ = np.cos(ang), np.sin(ang)
+        p = np.array([c * p[0] + s_ * p[2], p[1], -s_ * p[0] + c * p[2]]) + np.array(cam0["target"])
+        scene.set_camera(p, cam0["target"], cam0["up"], cam0["vfov"])
+        cam = scene.camera_view(w / h)
+        g = O.render_gbuffer(scene.desc(), cam, rs, f)
+        out.append((cam, prev if prev is not None else cam, g))
+        prev = cam
+    scene.set_camera(cam0["position"], cam0["target"], cam0["up"], cam0["vfov"])
+    return out
+
+
+def test_svgf_matches_oracle_on_identical_gbuffers():
+    scene = host.Scene.builtin("cornell-c1").build()
+    w, h = 120, 90
+    frames = _svgf_frames(scene, w, h, 6)
+    sv = SvgfSettings.make()
+    ora = O.SvgfOracle(w, h)
+    f = gpu.Svgf()
+    f.resize(w, h)
+    out = gpu.DeviceArray(w * h * 16)
+    for k, (cam, prev, g) in enumerate(frames):
+        dev = [gpu.DeviceArray.from_numpy(p) for p in g]
+        f.run(*[d.ptr for d in dev], prev, sv, out.ptr)
+        a = out.numpy(np.float32, (h, w, 4))
+        b = ora.run(*g, prev, sv)
+        d = np.abs(a - b)
+        ok = np.all((d <= 1e-3 * np.abs(b)) | (d <= 1e-6), axis=-1)
+        print(f"frame {k}: {100 * ok.mean():.3f}% within 1e-3, max rel {np.max(d / np.maximum(np.abs(b), 1e-6)):.2e}")
+        assert ok.mean() >= 0.999
+
+
+def test_display_modes_and_heatmap():
+    scene = host.Scene.builtin("cornell-c1").build()
+    w, h = 64, 48
+    rs = RenderSettings.make(w, h, seed=1, tonemap=TONEMAP_LINEAR)
+    cam = scene.camera_view(w / h)
+    r = uploaded(scene, w, h)
+    r.render(cam, rs, 4)
+    r.display(rs, DISPLAY_COLOR)
+    px = r.download_display()
+    import ctypes as C
+    ref_sum = O.render_sum(scene.desc(), cam, rs, 4).reshape(-1)
+    ref_px = np.zeros(w * h, np.uint32)
+    O.oracle().or_display_color(ref_sum.ctypes.data_as(C.POINTER(C.c_float)), w * h, 4, 1.0, TONEMAP_LINEAR, ref_px.ctypes.data)
+    print(f"display: {np.mean(px.reshape(-1) == ref_px):.5f} of packed pixels equal")
+    assert np.mean(px.reshape(-1) == ref_px) >= 0.999
+    for mode in (DISPLAY_ALBEDO, DISPLAY_NORMAL, DISPLAY_DEPTH, DISPLAY_VARIANCE, DISPLAY_HEATMAP):
+        r.display(rs, mode)
+        assert r.download_display().shape == (h, w)
+    heat = gpu.copy_device_to_numpy(r.buffer_ptr(gpu.BUF_HEAT), w * h * 4, np.float32)
+    visits = np.zeros(w * h, np.float32)
+    O.oracle().or_heatmap(C.byref(scene.desc()), C.byref(cam), w, h, visits.ctypes.data_as(C.POINTER(C.c_float)))
+    assert visits.max() > 1
+    assert np.array_equal(heat, visits)
+
+
+def test_furnace_converges_and_white_is_white():
+    scene = host.Scene.builtin("furnace")
+    scene.clear_normals()
+    scene.build()
+    w = h = 64
+    rs = RenderSettings.make(w, h, max_depth=64, rr_depth=64, nee=0)
+    cam = scene.camera_view(1.0)
+    r = uploaded(scene, w, h)
+    r.render(cam, rs, 32)
+    img = r.download_image()
+    lum = 0.2126 * img[..., 0] + 0.7152 * img[..., 1] + 0.0722 * img[..., 2]
+    assert np.abs(lum - 1).max() < 2e-3
+    assert r.samples == 32
+    r.reset()
+    assert r.samples == 0
+    rs.tonemap = TONEMAP_LINEAR
+    r.render(cam, rs, 4)
+    r.display(rs, DISPLAY_COLOR)
+    px = r.download_display()
+    assert px[h // 2, w // 2] == 0xFFFFFFFF
