@@ -69,7 +69,9 @@ typedef struct ph_render_stats {
     int64_t extend_launches, shadow_launches, shade_launches, iterations;
     int64_t rays_extend, rays_shadow; /* rays traced */
     int64_t samples;
-    int64_t node_visits, tri_tests, alpha_tests; /* filled only when counting is enabled */
+    /* Filled only when counting is enabled: closest-hit (extension) and any-hit (shadow) traversal work. */
+    int64_t node_visits, tri_tests, alpha_tests;
+    int64_t shadow_node_visits, shadow_tri_tests, shadow_alpha_tests;
 } ph_render_stats;
 
 /* The five G-buffer planes the reference denoiser consumes (SvgfInput, svgf.h:24-30), as device

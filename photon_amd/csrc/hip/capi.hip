@@ -82,7 +82,7 @@ struct ph_renderer {
     ph_camera_view last_camera{};
     SvgfState svgf;
 
-    int engine = PH_ENGINE_MEGAKERNEL;
+    int engine = PH_ENGINE_WAVEFRONT;
     std::unique_ptr<WavefrontEngine> wavefront;
     ph_render_stats stats{};
     bool counting = false;

@@ -45,7 +45,6 @@ enum : int {
 };
 
 constexpr int kClasses = 4;
-constexpr int kClassLobes[kClasses] = {1, 2, 4, 7};
 
 struct JobParams {
     DScene scene;
@@ -83,6 +82,21 @@ struct TraceTotals {
 };
 
 PH_DEV int lane_id() { return __lane_id(); }
+
+PH_DEV unsigned long long wave_sum(unsigned long long v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// Adds this wave's traversal counters to the global totals (one atomic per counter per wave).
+PH_DEV void flush_counts(const TraceCount& c, TraceTotals* totals) {
+    const unsigned long long n = wave_sum(c.nodes), t = wave_sum(c.tris), a = wave_sum(c.alpha);
+    if (lane_id() == 0) {
+        atomicAdd(&totals->nodes, n);
+        atomicAdd(&totals->tris, t);
+        atomicAdd(&totals->alpha, a);
+    }
+}
 
 // Appends `value` for every lane with `pred` using one atomic per wave. All lanes of the wave must call.
 PH_DEV void wave_append(int* queue, int* count, bool pred, int value) {
@@ -142,8 +156,15 @@ __global__ void init_kernel(const JobParams* jpp, State st, int* counters, int s
     wave_append(st.q[0], counters + C_EXT0, has, slot);
 }
 
-__global__ void bookkeep_kernel(int* counters, int par) {
+// Starts iteration `par`: folds the ray counts (this iteration's extension queue, the previous
+// iteration's shadow queue) into the 64-bit totals, then clears the per-iteration counters.
+__global__ void bookkeep_kernel(int* counters, int par, unsigned long long* rays) {
     const int t = threadIdx.x;
+    if (t == 0) {
+        rays[0] += (unsigned long long)counters[C_EXT0 + par];
+        rays[1] += (unsigned long long)counters[C_SHADOW];
+    }
+    __syncthreads();
     if (t == 0) counters[C_EXT0 + (1 - par)] = 0;
     if (t >= C_SHADOW && t < C_WORK) counters[t] = 0;
     if (t >= C_CUR && t < C_CUR + 8) counters[t] = 0;
@@ -153,7 +174,7 @@ template <int Block, int Depth, bool Count>
 __global__ __launch_bounds__(Block) void extend_kernel(const JobParams* jpp, State st, int* counters, int par, TraceTotals* totals) {
     __shared__ int lds[Depth * Block];
     LaneStack<Depth> stack{lds + threadIdx.x, Block};
-    const DScene& s = jpp->scene;
+    const DScene s = jpp->scene;
     const int* mat_class = jpp->mat_class;
     const int n = counters[C_EXT0 + par];
     const int* queue = st.q[par];
@@ -173,21 +194,16 @@ __global__ __launch_bounds__(Block) void extend_kernel(const JobParams* jpp, Sta
             if (hit.tri >= 0) cls = mat_class[__float_as_int(s.tris[3 * hit.tri].w)];
         }
         wave_append(st.q_miss, counters + C_MISS, active && cls < 0, slot);
-#pragma unroll
         for (int c = 0; c < kClasses; c++) wave_append(st.q_class[c], counters + C_CLASS + c, active && cls == c, slot);
     }
-    if (Count) {
-        atomicAdd(&totals->nodes, (unsigned long long)cnt.nodes);
-        atomicAdd(&totals->tris, (unsigned long long)cnt.tris);
-        atomicAdd(&totals->alpha, (unsigned long long)cnt.alpha);
-    }
+    if (Count) flush_counts(cnt, totals);
 }
 
 template <int Block, int Depth, bool Count>
 __global__ __launch_bounds__(Block) void shadow_kernel(const JobParams* jpp, State st, int* counters, TraceTotals* totals) {
     __shared__ int lds[Depth * Block];
     LaneStack<Depth> stack{lds + threadIdx.x, Block};
-    const DScene& s = jpp->scene;
+    const DScene s = jpp->scene;
     const int n = counters[C_SHADOW];
     TraceCount cnt{0, 0, 0};
     for (;;) {
@@ -208,11 +224,7 @@ __global__ __launch_bounds__(Block) void shadow_kernel(const JobParams* jpp, Sta
             }
         }
     }
-    if (Count) {
-        atomicAdd(&totals->nodes, (unsigned long long)cnt.nodes);
-        atomicAdd(&totals->tris, (unsigned long long)cnt.tris);
-        atomicAdd(&totals->alpha, (unsigned long long)cnt.alpha);
-    }
+    if (Count) flush_counts(cnt, totals);
 }
 
 PH_DEV f3 add_clamped(f3 L, f3 c, int depth, float clamp) { return L + clamp_contribution(c, depth, clamp); }
@@ -482,7 +494,9 @@ struct WavefrontEngine::Impl {
     DevBuf<float4> ro, rd, beta, L, hit, so, sd, sc;
     DevBuf<JobParams> params;
     DevBuf<TraceTotals> totals;
+    DevBuf<unsigned long long> ray_totals;  // [0] extension rays traced, [1] shadow rays traced
     int* host_counters = nullptr;
+    int cus = 0;
     std::vector<hipEvent_t> events;
     // Captured bounce loop and the key it was captured for.
     hipGraphExec_t graph_exec = nullptr;
@@ -493,7 +507,9 @@ struct WavefrontEngine::Impl {
         PH_CHECK(hipHostMalloc(reinterpret_cast<void**>(&host_counters), C_COUNT * sizeof(int), hipHostMallocDefault));
         counters.allocate(C_COUNT);
         params.allocate(1);
-        totals.allocate(1);
+        totals.allocate(2);  // [0] extension rays, [1] shadow rays
+        ray_totals.allocate(2);
+        cus = device_cus();
     }
     ~Impl() {
         drop_graph();
@@ -540,13 +556,12 @@ struct WavefrontEngine::Impl {
 
     // One bounce (iteration parity `par`). Event pairs bracket the traversal launches when timing.
     void iteration(hipStream_t st, const ShapeFns& fns, int par, bool count, hipEvent_t* ev) {
-        const int cus = device_cus();
         const int tgrid = cus * std::max(1, cfg.blocks_per_cu);
         const int sgrid = cus * 4;
         State s = state();
         const JobParams* jp = params.ptr;
         int* cnt = counters.ptr;
-        hipLaunchKernelGGL(bookkeep_kernel, dim3(1), dim3(64), 0, st, cnt, par);
+        hipLaunchKernelGGL(bookkeep_kernel, dim3(1), dim3(64), 0, st, cnt, par, ray_totals.ptr);
         if (ev) PH_CHECK(hipEventRecord(ev[0], st));
         fns.extend(st, tgrid, jp, s, cnt, par, totals.ptr, count);
         if (ev) PH_CHECK(hipEventRecord(ev[1], st));
@@ -556,7 +571,7 @@ struct WavefrontEngine::Impl {
         hipLaunchKernelGGL(shade_kernel<4>, dim3(sgrid), dim3(256), 0, st, jp, s, cnt, par, 2);
         hipLaunchKernelGGL(shade_kernel<7>, dim3(sgrid), dim3(256), 0, st, jp, s, cnt, par, 3);
         if (ev) PH_CHECK(hipEventRecord(ev[2], st));
-        fns.shadow(st, tgrid, jp, s, cnt, par, totals.ptr, count);
+        fns.shadow(st, tgrid, jp, s, cnt, par, totals.ptr + 1, count);
         if (ev) PH_CHECK(hipEventRecord(ev[3], st));
         hipLaunchKernelGGL(regen_kernel, dim3(sgrid), dim3(256), 0, st, jp, s, cnt, par);
         PH_CHECK(hipGetLastError());
@@ -595,7 +610,7 @@ void WavefrontEngine::render(hipStream_t st, const WavefrontJob& job, bool count
     std::vector<float4> mats(size_t(nm) * 5);
     PH_CHECK(hipMemcpyAsync(mats.data(), job.scene.materials, mats.size() * sizeof(float4), hipMemcpyDeviceToHost, st));
     PH_CHECK(hipStreamSynchronize(st));
-    std::vector<int> cls(size_t(nm));
+    std::vector<int> cls(static_cast<size_t>(nm));
     for (int i = 0; i < nm; i++) cls[size_t(i)] = material_class(&mats[size_t(i) * 5]);
     m.mat_class.upload(cls.data(), cls.size(), st);
     JobParams jp{};
@@ -614,7 +629,8 @@ void WavefrontEngine::render(hipStream_t st, const WavefrontJob& job, bool count
     jp.mat_class = m.mat_class.ptr;
     PH_CHECK(hipMemcpyAsync(m.params.ptr, &jp, sizeof(jp), hipMemcpyHostToDevice, st));
     PH_CHECK(hipMemsetAsync(m.counters.ptr, 0, C_COUNT * sizeof(int), st));
-    PH_CHECK(hipMemsetAsync(m.totals.ptr, 0, sizeof(TraceTotals), st));
+    PH_CHECK(hipMemsetAsync(m.totals.ptr, 0, 2 * sizeof(TraceTotals), st));
+    PH_CHECK(hipMemsetAsync(m.ray_totals.ptr, 0, 2 * sizeof(unsigned long long), st));
     int* cnt = m.counters.ptr;
     {
         // Work items 0..slots-1 go to the slots directly; the rest are handed out by regeneration.
@@ -670,21 +686,27 @@ void WavefrontEngine::render(hipStream_t st, const WavefrontJob& job, bool count
         stats.shadow_launches += G;
         // G is even, so the next iteration reads queue 0.
         if (m.host_counters[C_EXT0] == 0) break;
-        if (iterations > int64_t(job.spp) * int64_t(std::max(1, job.opt.max_depth)) * 64 + 1024)
-            throw std::runtime_error("wavefront loop did not drain");
+        // A sample takes at most max_depth iterations and a slot runs ceil(pixels / slots) pixels.
+        const int64_t bound = int64_t(job.spp) * int64_t(std::max(1, job.opt.max_depth)) * int64_t((pixels + slots - 1) / slots) + 2 * G;
+        if (iterations > bound) throw std::runtime_error("wavefront loop did not drain");
     }
-    TraceTotals tot{};
-    if (counting) {
-        PH_CHECK(hipMemcpy(&tot, m.totals.ptr, sizeof(tot), hipMemcpyDeviceToHost));
-    }
+    TraceTotals tot[2] = {};
+    if (counting) PH_CHECK(hipMemcpy(tot, m.totals.ptr, sizeof(tot), hipMemcpyDeviceToHost));
+    unsigned long long rays[2] = {0, 0};
+    PH_CHECK(hipMemcpy(rays, m.ray_totals.ptr, sizeof(rays), hipMemcpyDeviceToHost));
+    stats.rays_extend = int64_t(rays[0]);
+    stats.rays_shadow = int64_t(rays[1]) + m.host_counters[C_SHADOW];  // the last iteration's queue
     stats.ms_extend = ms_extend;
     stats.ms_shadow = ms_shadow;
     stats.iterations = iterations;
     stats.shade_launches = iterations * kClasses;
     stats.samples = int64_t(pixels) * job.spp;
-    stats.node_visits = int64_t(tot.nodes);
-    stats.tri_tests = int64_t(tot.tris);
-    stats.alpha_tests = int64_t(tot.alpha);
+    stats.node_visits = int64_t(tot[0].nodes);
+    stats.tri_tests = int64_t(tot[0].tris);
+    stats.alpha_tests = int64_t(tot[0].alpha);
+    stats.shadow_node_visits = int64_t(tot[1].nodes);
+    stats.shadow_tri_tests = int64_t(tot[1].tris);
+    stats.shadow_alpha_tests = int64_t(tot[1].alpha);
     stats.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 

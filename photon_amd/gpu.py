@@ -35,7 +35,8 @@ class RenderStats(C.Structure):
                 ("ms_other", C.c_double), ("extend_launches", C.c_int64), ("shadow_launches", C.c_int64),
                 ("shade_launches", C.c_int64), ("iterations", C.c_int64), ("rays_extend", C.c_int64),
                 ("rays_shadow", C.c_int64), ("samples", C.c_int64), ("node_visits", C.c_int64), ("tri_tests", C.c_int64),
-                ("alpha_tests", C.c_int64)]
+                ("alpha_tests", C.c_int64), ("shadow_node_visits", C.c_int64), ("shadow_tri_tests", C.c_int64),
+                ("shadow_alpha_tests", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
