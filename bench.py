@@ -22,8 +22,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 METRIC = "Msamples/s at 1080p 64spp; traversal Mrays/s; request p50/p99 latency"
 
-# Algorithmic bytes of one traversal: ray in 28, hit out 16, 64 per node visit,
-# 48 per triangle test, 80 + 12 + 24 + 16 per alpha test (material, indices, uvs, 4 texels).
+# Algorithmic bytes of one traversal: ray in 28, hit out 16, 64 per node visit, 48 per triangle test,
+# 80 + 12 + 24 + 16 per alpha test (material, indices, uvs, 4 texels).
 RAY_IO, NODE_B, TRI_B, ALPHA_B = 28 + 16, 64, 48, 80 + 12 + 24 + 16
 
 
@@ -137,6 +137,105 @@ def bench_traversal(gpu, host):
     return res
 
 
+RESOLUTIONS = ((256, 256), (512, 512), (1280, 720), (1920, 1080))
+SPPS = (4, 16, 64, 256)
+DEPTHS = (4, 8, 16)
+
+
+def draw_jobs(n, seed):
+    """Jobs of the serving workload: resolution, spp, depth and scene drawn uniformly, seeded per job."""
+    import numpy as np
+    jobs = []
+    for j in range(n):
+        g = np.random.default_rng([seed, j])
+        w, h = RESOLUTIONS[g.integers(len(RESOLUTIONS))]
+        jobs.append(dict(scene=int(g.integers(3)), w=w, h=h, spp=SPPS[g.integers(len(SPPS))], depth=DEPTHS[g.integers(len(DEPTHS))],
+                         seed=int(g.integers(1 << 31))))
+    return jobs
+
+
+def bench_serving(args, gpu, host, field):
+    """Open-loop Poisson arrivals at 80% of the saturated throughput over three resident scenes."""
+    import numpy as np
+    t = time.time()
+    scenes = [host.Scene.field(10, seed=7, env_width=2048).build(),
+              host.Scene.spheres(256, box=100.0, alpha_fraction=0.1, seed=1).build(), field]
+    log(f"serving scenes: {[s.num_tris for s in scenes]} triangles, setup {time.time() - t:.1f} s")
+    srv = gpu.Server(path_slots=args.serve_slots, iterations_per_step=args.serve_iterations)
+    ids = [srv.add_scene(s) for s in scenes]
+
+    def submit(job):
+        return srv.submit(ids[job["scene"]], job["w"], job["h"], job["spp"], scenes[job["scene"]].camera_view(job["w"] / job["h"]),
+                          max_depth=job["depth"], rr_depth=3, seed=job["seed"])
+
+    def drain(live, lat, t_arrival):
+        for k in list(live):
+            st = srv.status(live[k])
+            if st.state == gpu.JOB_DONE:
+                lat[k] = st.done_ms - t_arrival[k]
+                srv.release(live[k])
+                del live[k]
+
+    # Saturated throughput: a burst of jobs, measured until the backlog falls to a quarter.
+    burst = draw_jobs(args.serve_burst, seed=99)
+    t0 = srv.now_ms()
+    live = {k: submit(j) for k, j in enumerate(burst)}
+    ids_burst = dict(live)
+    lat = {}
+    while srv.pending > args.serve_burst // 4:
+        srv.step()
+        drain(live, lat, {k: t0 for k in ids_burst})
+    t1 = srv.now_ms()
+    done_samples = sum(srv.status(j).samples_done if k in live else srv.status(j).samples for k, j in ids_burst.items())
+    peak = done_samples / ((t1 - t0) * 1e-3)
+    srv.run_until_idle()
+    drain(live, lat, {k: t0 for k in ids_burst})
+    log(f"serving peak: {peak / 1e6:.1f} Msamples/s over {(t1 - t0) / 1e3:.1f} s")
+
+    # Open loop at 80% of peak: Poisson arrivals with the workload's mean job size.
+    mean_px = np.mean([w * h for w, h in RESOLUTIONS])
+    mean_samples = mean_px * np.mean(SPPS)
+    rate = args.serve_load * peak / mean_samples  # jobs per second
+    jobs = draw_jobs(args.serve_jobs, seed=2024)
+    gaps = np.random.default_rng(7).exponential(1.0 / rate, len(jobs))
+    start = srv.now_ms() + 1.0
+    arrival = start + np.cumsum(gaps) * 1e3
+    live, lat, t_arr = {}, {}, {}
+    nxt = 0
+    while nxt < len(jobs) or srv.pending:
+        now = srv.now_ms()
+        while nxt < len(jobs) and arrival[nxt] <= now:
+            live[nxt] = submit(jobs[nxt])
+            t_arr[nxt] = arrival[nxt]
+            nxt += 1
+        if srv.pending:
+            srv.step()
+            drain(live, lat, t_arr)
+        elif nxt < len(jobs):
+            time.sleep(max(0.0, (arrival[nxt] - srv.now_ms()) * 1e-3))
+    end = srv.now_ms()
+    srv.close()
+    L = np.array([lat[k] for k in range(len(jobs))])
+    samples = np.array([j["w"] * j["h"] * j["spp"] for j in jobs], np.float64)
+    ideal = samples / peak * 1e3  # ms at the saturated rate
+    slow = L / ideal
+    by_size = {}
+    for lo, hi, name in ((0, 2e6, "<2M samples"), (2e6, 5e7, "2M-50M"), (5e7, 1e12, ">50M")):
+        sel = (samples >= lo) & (samples < hi)
+        if sel.any():
+            by_size[name] = dict(jobs=int(sel.sum()), p50_ms=round(float(np.percentile(L[sel], 50)), 2),
+                                 p99_ms=round(float(np.percentile(L[sel], 99)), 2))
+    res = dict(jobs=len(jobs), load=args.serve_load, arrival_rate_jobs_s=round(rate, 3), peak_msamples_s=round(peak / 1e6, 2),
+               msamples_s=round(samples.sum() / ((end - start) * 1e-3) / 1e6, 2), duration_s=round((end - start) * 1e-3, 2),
+               latency_ms={"p50": round(float(np.percentile(L, 50)), 2), "p99": round(float(np.percentile(L, 99)), 2)},
+               slowdown={"p50": round(float(np.percentile(slow, 50)), 2), "p99": round(float(np.percentile(slow, 99)), 2)},
+               by_job_size=by_size, path_slots=args.serve_slots, iterations_per_step=args.serve_iterations)
+    res["latency_p99_over_p50"] = round(res["latency_ms"]["p99"] / res["latency_ms"]["p50"], 2)
+    res["slowdown_p99_over_p50"] = round(res["slowdown"]["p99"] / res["slowdown"]["p50"], 2)
+    log(f"serving: {res}")
+    return res
+
+
 def cpu_baseline(scene, cam, rs):
     """The reference CPU renderer (or the oracle) on full 1080p at 1 spp with all host threads."""
     from oracle import oracle_py as O
@@ -169,10 +268,15 @@ def main():
     ap.add_argument("--megakernel-divisor", type=int, default=4, help="time the plain kernel on spp/divisor samples")
     ap.add_argument("--no-traversal", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=200, help="jobs of the open-loop serving run (the full stream is 10,000)")
+    ap.add_argument("--serve-burst", type=int, default=48)
+    ap.add_argument("--serve-load", type=float, default=0.8)
+    ap.add_argument("--serve-slots", type=int, default=1 << 21)
+    ap.add_argument("--serve-iterations", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is designed but not built (see DESIGN.md)",
-                          "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this build runs on one GPU: --gpus must be 1", "n_gpus": args.gpus}))
         sys.exit(2)
     from photon_amd import gpu, host
     from photon_amd.types import RenderSettings
@@ -181,6 +285,7 @@ def main():
     log(f"device {info.name.decode()} {info.arch.decode()} CUs={info.compute_units}")
     scene, cam, rs, rnd = bench_render(args, gpu, host, RenderSettings)
     trav = None if args.no_traversal else bench_traversal(gpu, host)
+    serve = None if args.no_serve else bench_serving(args, gpu, host, scene)
     cpu = None if args.no_cpu else cpu_baseline(scene, cam, rs)
     line = {
         "metric": METRIC,
@@ -208,6 +313,7 @@ def main():
         "megakernel_msamples_s": round(rnd.get("megakernel_msamples_s", 0.0), 3) or None,
         "speedup_vs_megakernel": round(rnd.get("speedup_vs_megakernel", 0.0), 3) or None,
         "traversal": trav,
+        "serving": serve,
     }
     print(json.dumps(line), flush=True)
 

@@ -59,7 +59,9 @@ typedef struct ph_wavefront_config {
     int32_t graph_iterations; /* bounces per captured graph */
 } ph_wavefront_config;
 
-/* Statistics of the last render call. */
+/* Statistics of the last render call. After render_denoised, ms_shade is the one-sample G-buffer trace
+ * and ms_other the SVGF passes. Per-kernel times are measured with HIP events when the bounce loop is
+ * launched directly; a graph replay reports only ms_total. */
 typedef struct ph_render_stats {
     double ms_total;          /* wall time of the call */
     double ms_extend;         /* summed time of closest-hit traversal launches (HIP events) */
@@ -145,6 +147,61 @@ int ph_svgf_reset(ph_svgf* f);
 int ph_svgf_run(ph_svgf* f, const ph_svgf_input* in, const ph_camera_view* previous_camera, const ph_svgf_settings* settings, float* output_dev);
 const float* ph_svgf_variance_buffer(const ph_svgf* f);
 float ph_svgf_last_ms(const ph_svgf* f);
+
+/* Serving: several scenes stay resident; render jobs of mixed resolution, sample count and depth are
+ * admitted by HBM footprint while others run, and their paths share the same wavefronts (continuous
+ * batching). Each pixel is rendered by one path slot in sample order, so a job's image is identical
+ * whether it runs alone or among others. No counterpart in the reference. */
+typedef struct ph_server ph_server;
+
+typedef struct ph_server_config {
+    int64_t memory_budget;       /* bytes of job frame buffers admitted at once (0: 16 GiB) */
+    int32_t path_slots;          /* paths in flight shared by all jobs (0: 2^21) */
+    int32_t iterations_per_step; /* bounces per ph_server_step, even (0: 8) */
+    int32_t max_jobs;            /* jobs admitted at once, at most 1024 (0: 1024) */
+    int64_t pass_samples;        /* pixel samples per pass of a job (0: 4 x path_slots) */
+} ph_server_config;
+
+typedef struct ph_job_desc {
+    int32_t scene; /* id returned by ph_server_add_scene */
+    int32_t width, height, spp;
+    ph_camera_view camera;
+    ph_path_options path;
+    uint32_t seed, frame0;
+} ph_job_desc;
+
+#define PH_JOB_QUEUED 0
+#define PH_JOB_RUNNING 1
+#define PH_JOB_DONE 2
+#define PH_JOB_RELEASED 3
+
+typedef struct ph_job_status {
+    int32_t state;
+    int32_t pixels;
+    int64_t samples_done, samples;       /* pixel samples finished (as of the last step) / requested */
+    double submit_ms, admit_ms, done_ms; /* on the server's clock (ph_server_now_ms) */
+} ph_job_status;
+
+int ph_server_create(const ph_server_config* cfg, ph_server** out);
+void ph_server_destroy(ph_server* s);
+int ph_server_set_wavefront_config(ph_server* s, const ph_wavefront_config* cfg);
+/* Uploads a scene and keeps it resident; returns its id. */
+int ph_server_add_scene(ph_server* s, const ph_scene_desc* scene);
+/* Queues a job (admitted by a later step); returns its id. */
+int ph_server_submit(ph_server* s, const ph_job_desc* job);
+/* Admits queued jobs that fit, advances all paths in flight by iterations_per_step bounces (blocking),
+ * queues the next pass of every job whose pass finished, and returns the number of jobs finished since
+ * the server started. A job runs as passes of at most pass_samples pixel samples; passes are served
+ * in the order they were queued, so running jobs share the path slots round-robin. */
+int ph_server_step(ph_server* s);
+int ph_server_status(const ph_server* s, int job, ph_job_status* out);
+/* Average radiance of a finished job (3 floats per pixel); frees its frame buffer. */
+int ph_server_result(ph_server* s, int job, float* rgb);
+/* Frees a finished job's frame buffer without reading it. */
+int ph_server_release(ph_server* s, int job);
+/* Jobs queued or running. */
+int ph_server_pending(const ph_server* s);
+double ph_server_now_ms(const ph_server* s);
 
 #ifdef __cplusplus
 }

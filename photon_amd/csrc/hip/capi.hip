@@ -7,6 +7,7 @@
 #include <cstring>
 #include <memory>
 
+#include "device_scene.h"
 #include "internal.h"
 #include "svgf_state.h"
 #include "wavefront.h"
@@ -15,26 +16,14 @@ using namespace phd;
 using namespace phk;
 
 namespace {
-
 thread_local std::string g_error;
+int fail(const std::string& msg) { return phk::set_error(msg); }
+}  // namespace
 
-int fail(const std::string& msg) {
+int phk::set_error(const std::string& msg) {
     g_error = msg;
     return -1;
 }
-
-template <class F>
-int guarded(F&& f) {
-    try {
-        return f();
-    } catch (const std::bad_alloc&) {
-        return fail("out of host memory");
-    } catch (const std::exception& e) {
-        return fail(e.what());
-    }
-}
-
-}  // namespace
 
 struct ph_svgf {
     SvgfState state;
@@ -55,18 +44,19 @@ struct ph_svgf {
 
 struct ph_renderer {
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_mid = nullptr;
 
-    // Scene arrays in HBM.
-    DevBuf<float4> nodes, tris, materials;
-    DevBuf<int> indices, tri_emissive;
-    DevBuf<float> normals, uvs, tangents, srgb_lut, env_texels, env_marginal, env_conditional;
-    DevBuf<ph_texture_info> textures;
-    DevBuf<uint32_t> texels;
-    DevBuf<ph_emissive_tri> emissive;
-    DevBuf<ph_alias_entry> alias;
-    DevBuf<ph_punctual_light> punctual;
+    // The scene in HBM, its device view and its material -> shading-queue table.
+    DeviceScene scene;
     DScene view{};
+    DevBuf<int> mat_class;
+
+    void refresh_view() {
+        view = scene.view();
+        std::vector<int> cls = material_classes(view, stream);
+        mat_class.upload(cls.data(), cls.size(), stream);
+        PH_CHECK(hipStreamSynchronize(stream));
+    }
     float scene_extent = 1;
     bool uploaded = false;
 
@@ -91,6 +81,7 @@ struct ph_renderer {
         PH_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         PH_CHECK(hipEventCreate(&ev0));
         PH_CHECK(hipEventCreate(&ev1));
+        PH_CHECK(hipEventCreate(&ev_mid));
         work_counter.allocate(1);
         wavefront = std::make_unique<WavefrontEngine>();
     }
@@ -99,6 +90,7 @@ struct ph_renderer {
         if (stream) (void)hipStreamSynchronize(stream);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
+        if (ev_mid) (void)hipEventDestroy(ev_mid);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -114,52 +106,6 @@ struct ph_renderer {
 };
 
 namespace {
-
-void validate(const ph_scene_desc* d) {
-    if (!d) throw std::invalid_argument("null scene descriptor");
-    if (!d->nodes || d->num_nodes <= 0) throw std::invalid_argument("the scene is not built (no BVH nodes)");
-    if (d->num_tris < 0 || (d->num_tris > 0 && (!d->tris || !d->indices))) throw std::invalid_argument("missing triangle arrays");
-    if (!d->materials || d->num_materials <= 0) throw std::invalid_argument("the scene has no materials");
-    if (!d->srgb_lut) throw std::invalid_argument("missing sRGB table");
-    if (d->num_emissive > 0 && (!d->emissive || !d->emissive_alias || !d->tri_emissive)) throw std::invalid_argument("missing light tables");
-    if (d->env_texels && (d->env_width <= 0 || d->env_height <= 0 || !d->env_marginal || !d->env_conditional))
-        throw std::invalid_argument("incomplete environment image");
-    if (d->num_textures > 0 && (!d->textures || !d->texels)) throw std::invalid_argument("missing texture pool");
-}
-
-void upload_lights(ph_renderer& m, const ph_scene_desc& d) {
-    hipStream_t st = m.stream;
-    m.tris.upload(d.tris, size_t(d.num_tris) * 3, st);
-    m.materials.upload(d.materials, size_t(d.num_materials) * 5, st);
-    m.emissive.upload(d.emissive, size_t(d.num_emissive), st);
-    m.tri_emissive.upload(d.tri_emissive, d.tri_emissive ? size_t(d.num_tris) : 0, st);
-    m.alias.upload(d.emissive_alias, size_t(d.num_emissive), st);
-    const size_t env_n = d.env_texels ? size_t(d.env_width) * size_t(d.env_height) : 0;
-    m.env_texels.upload(d.env_texels, env_n * 3, st);
-    m.env_marginal.upload(d.env_marginal, env_n ? size_t(d.env_height) + 1 : 0, st);
-    m.env_conditional.upload(d.env_conditional, env_n ? size_t(d.env_height) * size_t(d.env_width + 1) : 0, st);
-    DScene& v = m.view;
-    v.tris = or_null(m.tris);
-    v.materials = or_null(m.materials);
-    v.emissive = or_null(m.emissive);
-    v.tri_emissive = or_null(m.tri_emissive);
-    v.alias = or_null(m.alias);
-    v.num_emissive = d.num_emissive;
-    v.emissive_power = d.emissive_power;
-    v.env_texels = or_null(m.env_texels);
-    v.env_marginal = or_null(m.env_marginal);
-    v.env_conditional = or_null(m.env_conditional);
-    v.env_w = d.env_width;
-    v.env_h = d.env_height;
-    v.env_intensity = d.env_intensity;
-    v.env_rotation = d.env_rotation;
-    v.env_integral = d.env_marginal_integral;
-    std::memcpy(v.env_color, d.env_color, 12);
-    v.p_env = d.p_env;
-    v.p_emissive = d.p_emissive;
-    v.p_punctual = d.p_punctual;
-    v.num_materials = d.num_materials;
-}
 
 PathOpts opts_of(const ph_render_settings& rs) { return PathOpts{rs.path.max_depth, rs.path.rr_depth, rs.path.clamp, rs.path.nee}; }
 
@@ -250,37 +196,12 @@ void ph_renderer_destroy(ph_renderer* r) { delete r; }
 int ph_renderer_upload(ph_renderer* r, const ph_scene_desc* d) {
     if (!r) return fail("null renderer");
     return guarded([&] {
-        validate(d);
+        DeviceScene::validate(d);
         ph_renderer& m = *r;
-        hipStream_t st = m.stream;
         m.uploaded = false;
-        m.nodes.upload(d->nodes, size_t(d->num_nodes) * 4, st);
-        m.indices.upload(d->indices, size_t(d->num_tris) * 3, st);
-        m.normals.upload(d->normals, d->normals ? size_t(d->num_vertices) * 3 : 0, st);
-        m.uvs.upload(d->uvs, d->uvs ? size_t(d->num_vertices) * 2 : 0, st);
-        m.tangents.upload(d->tangents, d->tangents ? size_t(d->num_vertices) * 4 : 0, st);
-        m.textures.upload(d->textures, size_t(d->num_textures), st);
-        m.texels.upload(d->texels, size_t(d->num_texels), st);
-        m.srgb_lut.upload(d->srgb_lut, 256, st);
-        m.punctual.upload(d->punctual, size_t(d->num_punctual), st);
-        upload_lights(m, *d);
-        DScene& v = m.view;
-        v.nodes = or_null(m.nodes);
-        v.indices = or_null(m.indices);
-        v.normals = or_null(m.normals);
-        v.uvs = or_null(m.uvs);
-        v.tangents = or_null(m.tangents);
-        v.textures = or_null(m.textures);
-        v.texels = or_null(m.texels);
-        v.srgb_lut = or_null(m.srgb_lut);
-        v.punctual = or_null(m.punctual);
-        v.num_tris = d->num_tris;
-        v.num_nodes = d->num_nodes;
-        v.num_punctual = d->num_punctual;
-        v.num_textures = d->num_textures;
-        float dx = d->bounds_max[0] - d->bounds_min[0], dy = d->bounds_max[1] - d->bounds_min[1], dz = d->bounds_max[2] - d->bounds_min[2];
-        m.scene_extent = fmaxf(sqrtf(dx * dx + dy * dy + dz * dz), 1e-3f);
-        PH_CHECK(hipStreamSynchronize(st));
+        m.scene.upload(*d, m.stream);
+        m.refresh_view();
+        m.scene_extent = m.scene.extent();
         m.uploaded = true;
         m.reset();
         return 0;
@@ -291,9 +212,10 @@ int ph_renderer_update_lights(ph_renderer* r, const ph_scene_desc* d) {
     if (!r) return fail("null renderer");
     if (!r->uploaded) return ph_renderer_upload(r, d);
     return guarded([&] {
-        validate(d);
-        upload_lights(*r, *d);
+        DeviceScene::validate(d);
+        r->scene.upload_lights(*d, r->stream);
         PH_CHECK(hipStreamSynchronize(r->stream));
+        r->refresh_view();
         r->reset();
         return 0;
     });
@@ -359,6 +281,7 @@ int ph_renderer_render(ph_renderer* r, const ph_camera_view* cam, const ph_rende
             job.accum = m.accum.ptr;
             job.albedo = m.albedo.ptr;
             job.nd = m.nd.ptr;
+            job.mat_class = m.mat_class.ptr;
             m.wavefront->render(m.stream, job, m.counting, m.stats);
             m.last_ms = float(m.stats.ms_total);
         }
@@ -384,11 +307,20 @@ int ph_renderer_render_denoised(ph_renderer* r, const ph_camera_view* cam, const
         PH_CHECK(hipEventRecord(m.ev0, m.stream));
         launch_sample(m.stream, m.view, *cam, m.width, m.height, m.frame, rs->seed, opts_of(*rs), m.color.ptr, m.albedo.ptr, m.nd.ptr, m.position.ptr,
                       m.emission.ptr);
+        PH_CHECK(hipEventRecord(m.ev_mid, m.stream));
         ph_svgf_input in{&m.color.ptr->x, &m.albedo.ptr->x, &m.nd.ptr->x, &m.position.ptr->x, &m.emission.ptr->x};
         svgf_run(m.svgf, m.stream, in, *prev, *sv, m.denoised_buf.ptr);
         PH_CHECK(hipEventRecord(m.ev1, m.stream));
         PH_CHECK(hipEventSynchronize(m.ev1));
         PH_CHECK(hipEventElapsedTime(&m.last_ms, m.ev0, m.ev1));
+        float trace_ms = 0, denoise_ms = 0;
+        PH_CHECK(hipEventElapsedTime(&trace_ms, m.ev0, m.ev_mid));
+        PH_CHECK(hipEventElapsedTime(&denoise_ms, m.ev_mid, m.ev1));
+        m.stats = ph_render_stats{};
+        m.stats.ms_total = m.last_ms;
+        m.stats.ms_shade = trace_ms;
+        m.stats.ms_other = denoise_ms;
+        m.stats.samples = int64_t(m.width) * m.height;
         m.frame++;
         m.samples = 1;
         m.last_camera = *cam;
@@ -502,6 +434,7 @@ int ph_renderer_autotune(ph_renderer* r, const ph_camera_view* cam, const ph_ren
         job.accum = m.color.ptr;
         job.albedo = m.position.ptr;
         job.nd = m.emission.ptr;
+        job.mat_class = m.mat_class.ptr;
         ph_wavefront_config best = m.wavefront->autotune(m.stream, job);
         if (chosen) *chosen = best;
         return 0;

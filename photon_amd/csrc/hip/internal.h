@@ -23,6 +23,24 @@ struct HipError : std::runtime_error {
         if (e_ != hipSuccess) throw ::phk::HipError(std::string("HIP error: ") + hipGetErrorString(e_) + " (" #x ")"); \
     } while (0)
 
+// The calling thread's last error message (ph_last_error) and the entry-point wrapper that turns
+// exceptions into a negative status. A failed HIP call also leaves HIP's last-error state set; it is
+// cleared so that a later call does not report it again.
+int set_error(const std::string& msg);
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        return f();
+    } catch (const std::bad_alloc&) {
+        (void)hipGetLastError();
+        return set_error("out of host memory");
+    } catch (const std::exception& e) {
+        (void)hipGetLastError();
+        return set_error(e.what());
+    }
+}
+
 // An owning device allocation.
 template <class T>
 struct DevBuf {

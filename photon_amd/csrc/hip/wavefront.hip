@@ -1,27 +1,33 @@
 // Wavefront path tracing engine.
 //
 // Paths in flight live in structure-of-arrays state in HBM, one "slot" per path. A slot owns one pixel
-// at a time and runs that pixel's samples one after another, so each pixel's accumulation is the same
-// sequence of float additions as in the one-thread-per-pixel kernel (bit-identical images); when a
-// pixel's samples are done the slot takes the next pixel from a global work counter (regeneration).
+// of one job at a time and runs that pixel's samples one after another, so each pixel's accumulation
+// is the same sequence of float additions as in the one-thread-per-pixel kernel (bit-identical images,
+// whatever else shares the wavefront); when a pixel's samples are done the slot takes the next work
+// item (regeneration). A work item is one pixel of a pass: a range of sample indices of one job.
+// Passes are queued in order (chunks), and a job's next pass is queued only once its previous pass
+// has finished, so the samples of a pixel still run in order while large jobs yield to new ones.
 //
-// One bounce of every path in flight is one iteration of six stages:
-//   bookkeep  reset the per-iteration queue counters and work cursors (one wave)
+// One bounce of every path in flight is one iteration of these stages:
+//   bookkeep  fold ray counts, clear the per-iteration queue counters and cursors (one wave)
+//   wake      idle slots take newly admitted work (serving only)
 //   extend    closest-hit traversal of the extension queue; hits are binned by material class
 //   miss      environment contribution of escaped paths
 //   shade x4  surface interaction, NEE light sample, BSDF sample, Russian roulette; one launch per
 //             material class (metal / opaque / glass / mixed) so a wave runs one lobe structure
 //   shadow    any-hit traversal of the NEE queue; unoccluded contributions are added to L
-//   regen     terminated paths: add L to the pixel, start the next sample or the next pixel
-// The traversal, shading and regeneration kernels are persistent: a fixed grid whose waves pull
-// batches of 64 queue entries from a cursor until the queue is drained, so uneven rays never leave a
-// wave idle. Queue appends are wave-aggregated (one atomic per wave). The bounce loop can be captured
-// once into a HIP graph of G iterations and replayed until the queues are empty.
+//   regen     finished paths: add L to the pixel, start the next sample or the next work item
+// Traversal, shading and regeneration kernels are persistent: a fixed grid whose waves pull batches of
+// 64 queue entries from a cursor until the queue is drained, so uneven rays never leave a wave idle.
+// Queue appends are wave-aggregated (one atomic per wave). The bounce loop can be captured once into a
+// HIP graph of G iterations and replayed; kernels read their sizes from device memory, so the graph
+// stays valid while jobs come and go.
+#include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <tuple>
-#include <vector>
 
 #include "wavefront.h"
 
@@ -32,35 +38,41 @@ namespace phk {
 namespace {
 
 enum : int {
-    C_EXT0 = 0,
+    C_EXT0 = 0,  // extension queue counts, by iteration parity
     C_EXT1 = 1,
     C_SHADOW = 2,
     C_TERM = 3,
     C_MISS = 4,
     C_CLASS = 5,  // 4 counters: metal, opaque, glass, mixed
-    C_WORK = 9,   // next work item (pixel) to hand out
-    C_CUR = 10,   // cursors: 10 extend, 11 miss, 12..15 shade, 16 shadow, 17 regen
-    C_RETIRED = 18,
+    C_IDLE0 = 9,  // idle-slot queue counts, by iteration parity
+    C_IDLE1 = 10,
+    C_WORK = 11,    // next work item to hand out
+    C_TOTAL = 12,   // work items admitted so far
+    C_CHUNKS = 13,  // entries of the chunk table
+    C_CUR = 16,     // cursors: +0 extend, +1 miss, +2..5 shade, +6 shadow, +7 regen, +8 wake
     C_COUNT = 32
 };
 
 constexpr int kClasses = 4;
+constexpr int kMaxChunks = 1 << 16;
 
 struct JobParams {
     DScene scene;
     ph_camera_view camera;
-    int width, height, spp, total_work;
+    int width, height, spp, pixels;
     uint32_t frame0, seed;
     PathOpts opt;
     float4* accum;
     float4* albedo;
     float4* nd;
-    const int* mat_class;  // per material: index into the class queues
+    const int* mat_class;
 };
 
 struct State {
+    int* job;
     int* pixel;
     int* sample;
+    int* sample_end;  // end of the slot's pass (serving)
     uint64_t* rng;
     float4* ro;  // origin, prev_pdf
     float4* rd;  // direction, flags (depth | prev_delta << 16)
@@ -71,10 +83,20 @@ struct State {
     float4* sd;  // shadow direction
     float4* sc;  // shadow contribution
     int* q[2];
+    int* q_idle[2];
     int* q_shadow;
     int* q_term;
     int* q_miss;
     int* q_class[kClasses];
+};
+
+// Work items in queue order: chunk c holds items [chunk_first[c], ...), the pixels of one pass,
+// described by chunk_pass[c] = (table entry, first sample, end sample).
+struct Work {
+    const JobParams* jobs;
+    const int* chunk_first;
+    const int4* chunk_pass;
+    int* job_done;  // pixel passes finished, per table entry
 };
 
 struct TraceTotals {
@@ -110,6 +132,18 @@ PH_DEV void wave_append(int* queue, int* count, bool pred, int value) {
     if (pred) queue[base + __popcll(mask & ((1ull << lane) - 1ull))] = value;
 }
 
+// Gives every lane with `pred` the next work item index (one atomic per wave). All lanes must call.
+PH_DEV int wave_take(int* counter, bool pred) {
+    const uint64_t mask = __ballot(pred);
+    if (mask == 0) return 0;
+    const int lane = lane_id();
+    const int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+    int first = 0;
+    if (lane == leader) first = atomicAdd(counter, __popcll(mask));
+    first = __shfl(first, leader);
+    return first + __popcll(mask & ((1ull << lane) - 1ull));
+}
+
 // Persistent work distribution: a wave takes 64 consecutive queue entries per round.
 PH_DEV int wave_fetch(int* cursor) {
     int base = 0;
@@ -117,8 +151,8 @@ PH_DEV int wave_fetch(int* cursor) {
     return __shfl(base, 0);
 }
 
-// Work item -> pixel: 8-row bands, 8-column tiles inside a band (the last tile and band may be
-// narrower), row-major inside a tile, so the 64 lanes of a wave start on an 8x8 pixel block.
+// Work item -> pixel of a w x h image: 8-row bands, 8-column tiles inside a band (the last tile and band
+// may be narrower), row-major inside a tile, so a wave's 64 lanes start on one 8x8 pixel block.
 PH_DEV int work_pixel(int w, int width, int height) {
     const int band = w / (8 * width);
     const int bh = min(8, height - band * 8);
@@ -128,6 +162,26 @@ PH_DEV int work_pixel(int w, int width, int height) {
     const int off = in_band - tc * 8 * bh;
     const int x = tc * 8 + off % tw, y = band * 8 + off / tw;
     return y * width + x;
+}
+
+// Table entry owning work item `w` (the last chunk that starts at or before it).
+template <bool Multi>
+PH_DEV int chunk_of(const Work& work, int chunks, int w) {
+    if (!Multi) return 0;
+    int lo = 0, hi = chunks;
+    while (lo + 1 < hi) {
+        const int mid = (lo + hi) / 2;
+        if (work.chunk_first[mid] <= w)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+template <bool Multi>
+PH_DEV const JobParams& job_of(const Work& work, const State& st, int slot) {
+    return Multi ? work.jobs[st.job[slot]] : work.jobs[0];
 }
 
 PH_DEV void start_sample(const JobParams& jp, const State& st, int slot, int pixel, int k) {
@@ -146,14 +200,34 @@ PH_DEV void start_sample(const JobParams& jp, const State& st, int slot, int pix
     st.L[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
+// Starts work item `w` on `slot`: finds its job and pixel, records them, seeds the first sample.
+template <bool Multi>
+PH_DEV void start_work(const Work& work, const State& st, int chunks, int slot, int w) {
+    const int c = chunk_of<Multi>(work, chunks, w);
+    const int4 pass = Multi ? work.chunk_pass[c] : make_int4(0, 0, 0, 0);
+    const JobParams& jp = work.jobs[pass.x];
+    const int local = w - (Multi ? work.chunk_first[c] : 0);
+    if (Multi) {
+        st.job[slot] = pass.x;
+        st.sample_end[slot] = pass.z;
+    }
+    start_sample(jp, st, slot, work_pixel(local, jp.width, jp.height), pass.y);
+}
+
 PH_DEV Pcg slot_rng(const State& st, int slot, int pixel) { return Pcg{st.rng[slot], (uint64_t(uint32_t(pixel)) << 1u) | 1u}; }
 
-__global__ void init_kernel(const JobParams* jpp, State st, int* counters, int slots) {
-    const JobParams& jp = *jpp;
+// Single job: slot s starts work item s.
+__global__ void init_kernel(Work work, State st, int* counters, int slots) {
     const int slot = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool has = slot < slots && slot < jp.total_work;
-    if (has) start_sample(jp, st, slot, work_pixel(slot, jp.width, jp.height), 0);
+    const bool has = slot < slots && slot < counters[C_TOTAL];
+    if (has) start_work<false>(work, st, 1, slot, slot);
     wave_append(st.q[0], counters + C_EXT0, has, slot);
+}
+
+// Serving: every slot starts idle.
+__global__ void init_idle_kernel(State st, int* counters, int slots) {
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    wave_append(st.q_idle[0], counters + C_IDLE0, slot < slots, slot);
 }
 
 // Starts iteration `par`: folds the ray counts (this iteration's extension queue, the previous
@@ -166,16 +240,35 @@ __global__ void bookkeep_kernel(int* counters, int par, unsigned long long* rays
     }
     __syncthreads();
     if (t == 0) counters[C_EXT0 + (1 - par)] = 0;
-    if (t >= C_SHADOW && t < C_WORK) counters[t] = 0;
-    if (t >= C_CUR && t < C_CUR + 8) counters[t] = 0;
+    if (t == 1) counters[C_IDLE0 + (1 - par)] = 0;
+    if (t >= C_SHADOW && t < C_CLASS + kClasses) counters[t] = 0;
+    if (t >= C_CUR && t < C_CUR + 9) counters[t] = 0;
 }
 
-template <int Block, int Depth, bool Count>
-__global__ __launch_bounds__(Block) void extend_kernel(const JobParams* jpp, State st, int* counters, int par, TraceTotals* totals) {
+// Idle slots take newly admitted work items; the rest stay idle.
+__global__ __launch_bounds__(256) void wake_kernel(Work work, State st, int* counters, int par) {
+    const int n = counters[C_IDLE0 + par];
+    const int total = counters[C_TOTAL], chunks = counters[C_CHUNKS];
+    for (;;) {
+        const int base = wave_fetch(counters + C_CUR + 8);
+        if (base >= n) break;
+        const int i = base + lane_id();
+        const bool active = i < n;
+        const int slot = active ? st.q_idle[par][i] : 0;
+        // Stop taking items once they run out (every lane past `total` stays idle).
+        const bool want = active && counters[C_WORK] < total;
+        const int w = wave_take(counters + C_WORK, want);
+        const bool got = want && w < total;
+        if (got) start_work<true>(work, st, chunks, slot, w);
+        wave_append(st.q[par], counters + C_EXT0 + par, got, slot);
+        wave_append(st.q_idle[1 - par], counters + C_IDLE0 + (1 - par), active && !got, slot);
+    }
+}
+
+template <int Block, int Depth, bool Count, bool Multi>
+__global__ __launch_bounds__(Block) void extend_kernel(Work work, State st, int* counters, int par, TraceTotals* totals) {
     __shared__ int lds[Depth * Block];
     LaneStack<Depth> stack{lds + threadIdx.x, Block};
-    const DScene s = jpp->scene;
-    const int* mat_class = jpp->mat_class;
     const int n = counters[C_EXT0 + par];
     const int* queue = st.q[par];
     TraceCount cnt{0, 0, 0};
@@ -187,11 +280,13 @@ __global__ __launch_bounds__(Block) void extend_kernel(const JobParams* jpp, Sta
         int slot = 0, cls = -1;
         if (active) {
             slot = queue[i];
+            const JobParams& jp = job_of<Multi>(work, st, slot);
+            const DScene& s = jp.scene;
             const float4 o4 = st.ro[slot], d4 = st.rd[slot];
             HitRec hit{kInf, -1, 0.0f, 0.0f};
             traverse<false, Count>(s, xyz(o4), xyz(d4), kInf, hit, stack, &cnt);
             st.hit[slot] = make_float4(hit.t, __int_as_float(hit.tri), hit.u, hit.v);
-            if (hit.tri >= 0) cls = mat_class[__float_as_int(s.tris[3 * hit.tri].w)];
+            if (hit.tri >= 0) cls = jp.mat_class[__float_as_int(s.tris[3 * hit.tri].w)];
         }
         wave_append(st.q_miss, counters + C_MISS, active && cls < 0, slot);
         for (int c = 0; c < kClasses; c++) wave_append(st.q_class[c], counters + C_CLASS + c, active && cls == c, slot);
@@ -199,11 +294,10 @@ __global__ __launch_bounds__(Block) void extend_kernel(const JobParams* jpp, Sta
     if (Count) flush_counts(cnt, totals);
 }
 
-template <int Block, int Depth, bool Count>
-__global__ __launch_bounds__(Block) void shadow_kernel(const JobParams* jpp, State st, int* counters, TraceTotals* totals) {
+template <int Block, int Depth, bool Count, bool Multi>
+__global__ __launch_bounds__(Block) void shadow_kernel(Work work, State st, int* counters, TraceTotals* totals) {
     __shared__ int lds[Depth * Block];
     LaneStack<Depth> stack{lds + threadIdx.x, Block};
-    const DScene s = jpp->scene;
     const int n = counters[C_SHADOW];
     TraceCount cnt{0, 0, 0};
     for (;;) {
@@ -212,6 +306,7 @@ __global__ __launch_bounds__(Block) void shadow_kernel(const JobParams* jpp, Sta
         const int i = base + lane_id();
         if (i < n) {
             const int slot = st.q_shadow[i];
+            const DScene& s = job_of<Multi>(work, st, slot).scene;
             const float4 o4 = st.so[slot], d4 = st.sd[slot];
             HitRec hit{kInf, -1, 0.0f, 0.0f};
             if (!traverse<true, Count>(s, xyz(o4), xyz(d4), o4.w, hit, stack, &cnt)) {
@@ -227,11 +322,8 @@ __global__ __launch_bounds__(Block) void shadow_kernel(const JobParams* jpp, Sta
     if (Count) flush_counts(cnt, totals);
 }
 
-PH_DEV f3 add_clamped(f3 L, f3 c, int depth, float clamp) { return L + clamp_contribution(c, depth, clamp); }
-
-__global__ __launch_bounds__(256) void miss_kernel(const JobParams* jpp, State st, int* counters) {
-    const JobParams& jp = *jpp;
-    const DScene& s = jp.scene;
+template <bool Multi>
+__global__ __launch_bounds__(256) void miss_kernel(Work work, State st, int* counters) {
     const int n = counters[C_MISS];
     for (;;) {
         const int base = wave_fetch(counters + C_CUR + 1);
@@ -241,18 +333,20 @@ __global__ __launch_bounds__(256) void miss_kernel(const JobParams* jpp, State s
         int slot = 0;
         if (active) {
             slot = st.q_miss[i];
+            const JobParams& jp = job_of<Multi>(work, st, slot);
+            const DScene& s = jp.scene;
             const float4 o4 = st.ro[slot], d4 = st.rd[slot];
             const f3 rd = xyz(d4);
             const int flags = __float_as_int(d4.w);
             const int depth = flags & 0xFFFF;
             const bool prev_delta = (flags >> 16) & 1;
             const float4 b4 = st.beta[slot];
-            float4 L4 = st.L[slot];
+            const float4 L4 = st.L[slot];
             const f3 le = sky_radiance(s, rd);
             const float w = (prev_delta || s.p_env <= 0) ? 1.0f : mis_power(o4.w, sky_pdf(s, rd));
-            const f3 L = add_clamped(xyz(L4), xyz(b4) * le * w, depth, jp.opt.clamp);
+            const f3 L = xyz(L4) + clamp_contribution(xyz(b4) * le * w, depth, jp.opt.clamp);
             st.L[slot] = make_float4(L.x, L.y, L.z, 0.0f);
-            if (depth == 0 && st.sample[slot] == 0) {
+            if (depth == 0 && st.sample[slot] == 0 && jp.albedo) {
                 const int p = st.pixel[slot];
                 jp.albedo[p] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
                 jp.nd[p] = make_float4(-rd.x, -rd.y, -rd.z, 0.0f);
@@ -263,14 +357,10 @@ __global__ __launch_bounds__(256) void miss_kernel(const JobParams* jpp, State s
 }
 
 // One path vertex (integrator.h:178-260) for the slots of one material class.
-template <int Lobes>
-__global__ __launch_bounds__(256) void shade_kernel(const JobParams* jpp, State st, int* counters, int par, int cls) {
-    const JobParams& jp = *jpp;
-    const DScene& s = jp.scene;
-    const PathOpts opt = jp.opt;
+template <int Lobes, bool Multi>
+__global__ __launch_bounds__(256) void shade_kernel(Work work, State st, int* counters, int par, int cls) {
     const int n = counters[C_CLASS + cls];
     const int* queue = st.q_class[cls];
-    const bool has_lights = s.p_env + s.p_emissive + s.p_punctual > 0;
     for (;;) {
         const int base = wave_fetch(counters + C_CUR + 2 + cls);
         if (base >= n) break;
@@ -280,6 +370,10 @@ __global__ __launch_bounds__(256) void shade_kernel(const JobParams* jpp, State 
         bool cont = false, shadow = false;
         if (active) {
             slot = queue[i];
+            const JobParams& jp = job_of<Multi>(work, st, slot);
+            const DScene& s = jp.scene;
+            const PathOpts opt = jp.opt;
+            const bool has_lights = s.p_env + s.p_emissive + s.p_punctual > 0;
             const int pixel = st.pixel[slot];
             const float4 o4 = st.ro[slot], d4 = st.rd[slot], h4 = st.hit[slot];
             const f3 rd = xyz(d4);
@@ -295,10 +389,10 @@ __global__ __launch_bounds__(256) void shade_kernel(const JobParams* jpp, State 
             if (!black(it.emission)) {
                 float w = 1.0f;
                 if (!prev_delta) w = mis_power(prev_pdf, emitter_pdf(s, tri, rd, h4.x));
-                L = add_clamped(L, beta * it.emission * w, depth, opt.clamp);
+                L = L + clamp_contribution(beta * it.emission * w, depth, opt.clamp);
             }
             const Mix bsdf = make_mix(it.params);
-            if (depth == 0 && st.sample[slot] == 0) {
+            if (depth == 0 && st.sample[slot] == 0 && jp.albedo) {
                 const f3 a = mix_albedo(bsdf);
                 jp.albedo[pixel] = make_float4(a.x, a.y, a.z, 1.0f);
                 jp.nd[pixel] = make_float4(it.ns.x, it.ns.y, it.ns.z, h4.x);
@@ -367,19 +461,21 @@ __global__ __launch_bounds__(256) void shade_kernel(const JobParams* jpp, State 
 }
 
 // Finished paths: add the sample to its pixel (render_kernel's accumulation order), then start the
-// pixel's next sample or take the next pixel.
-__global__ __launch_bounds__(256) void regen_kernel(const JobParams* jpp, State st, int* counters, int par) {
-    const JobParams& jp = *jpp;
+// pixel's next sample, or take the next work item, or go idle.
+template <bool Multi>
+__global__ __launch_bounds__(256) void regen_kernel(Work work, State st, int* counters, int par) {
     const int n = counters[C_TERM];
+    const int total = counters[C_TOTAL], chunks = counters[C_CHUNKS];
     for (;;) {
         const int base = wave_fetch(counters + C_CUR + 7);
         if (base >= n) break;
         const int i = base + lane_id();
         const bool active = i < n;
         int slot = 0, pixel = -1, k = 0;
-        bool need_pixel = false;
+        bool need_work = false;
         if (active) {
             slot = st.q_term[i];
+            const JobParams& jp = job_of<Multi>(work, st, slot);
             pixel = st.pixel[slot];
             k = st.sample[slot];
             f3 L = xyz(st.L[slot]);
@@ -392,32 +488,15 @@ __global__ __launch_bounds__(256) void regen_kernel(const JobParams* jpp, State 
             a.w += l * l;
             jp.accum[pixel] = a;
             k++;
-            need_pixel = k >= jp.spp;
+            need_work = k >= (Multi ? st.sample_end[slot] : jp.spp);
+            if (need_work) atomicAdd(work.job_done + (Multi ? st.job[slot] : 0), 1);
+            else start_sample(jp, st, slot, pixel, k);
         }
-        // Lanes whose pixel is finished take new work items with one atomic per wave.
-        const uint64_t mask = __ballot(need_pixel);
-        int work = 0;
-        if (mask) {
-            const int lane = lane_id();
-            const int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
-            int first = 0;
-            if (lane == leader) first = atomicAdd(counters + C_WORK, __popcll(mask));
-            first = __shfl(first, leader);
-            work = first + __popcll(mask & ((1ull << lane) - 1ull));
-        }
-        bool again = false;
-        if (active) {
-            if (need_pixel) {
-                if (work < jp.total_work) {
-                    start_sample(jp, st, slot, work_pixel(work, jp.width, jp.height), 0);
-                    again = true;
-                }
-            } else {
-                start_sample(jp, st, slot, pixel, k);
-                again = true;
-            }
-        }
-        wave_append(st.q[1 - par], counters + C_EXT0 + (1 - par), again, slot);
+        const int w = wave_take(counters + C_WORK, need_work);
+        const bool got = need_work && w < total;
+        if (got) start_work<Multi>(work, st, chunks, slot, w);
+        wave_append(st.q[1 - par], counters + C_EXT0 + (1 - par), active && (!need_work || got), slot);
+        wave_append(st.q_idle[1 - par], counters + C_IDLE0 + (1 - par), need_work && !got, slot);
     }
 }
 
@@ -428,33 +507,34 @@ int device_cus() {
     return cus;
 }
 
-using TraceLaunch = void (*)(hipStream_t, int, const JobParams*, State, int*, int, TraceTotals*, bool);
+using TraceLaunch = void (*)(hipStream_t, int, Work, State, int*, int, TraceTotals*, bool);
 
-template <int Block, int Depth>
-void launch_extend(hipStream_t st, int grid, const JobParams* jp, State s, int* counters, int par, TraceTotals* tot, bool count) {
+template <int Block, int Depth, bool Multi>
+void launch_extend(hipStream_t st, int grid, Work w, State s, int* counters, int par, TraceTotals* tot, bool count) {
     if (count)
-        hipLaunchKernelGGL((extend_kernel<Block, Depth, true>), dim3(grid), dim3(Block), 0, st, jp, s, counters, par, tot);
+        hipLaunchKernelGGL((extend_kernel<Block, Depth, true, Multi>), dim3(grid), dim3(Block), 0, st, w, s, counters, par, tot);
     else
-        hipLaunchKernelGGL((extend_kernel<Block, Depth, false>), dim3(grid), dim3(Block), 0, st, jp, s, counters, par, tot);
+        hipLaunchKernelGGL((extend_kernel<Block, Depth, false, Multi>), dim3(grid), dim3(Block), 0, st, w, s, counters, par, tot);
 }
 
-template <int Block, int Depth>
-void launch_shadow(hipStream_t st, int grid, const JobParams* jp, State s, int* counters, int, TraceTotals* tot, bool count) {
+template <int Block, int Depth, bool Multi>
+void launch_shadow(hipStream_t st, int grid, Work w, State s, int* counters, int, TraceTotals* tot, bool count) {
     if (count)
-        hipLaunchKernelGGL((shadow_kernel<Block, Depth, true>), dim3(grid), dim3(Block), 0, st, jp, s, counters, tot);
+        hipLaunchKernelGGL((shadow_kernel<Block, Depth, true, Multi>), dim3(grid), dim3(Block), 0, st, w, s, counters, tot);
     else
-        hipLaunchKernelGGL((shadow_kernel<Block, Depth, false>), dim3(grid), dim3(Block), 0, st, jp, s, counters, tot);
+        hipLaunchKernelGGL((shadow_kernel<Block, Depth, false, Multi>), dim3(grid), dim3(Block), 0, st, w, s, counters, tot);
 }
 
 struct ShapeFns {
     TraceLaunch extend, shadow;
 };
 
+template <bool Multi>
 bool shape_fns(int block, int depth, ShapeFns& f) {
-#define PH_WF_SHAPE(B, D)                                       \
-    if (block == B && depth == D) {                             \
-        f = ShapeFns{&launch_extend<B, D>, &launch_shadow<B, D>}; \
-        return true;                                            \
+#define PH_WF_SHAPE(B, D)                                                       \
+    if (block == B && depth == D) {                                             \
+        f = ShapeFns{&launch_extend<B, D, Multi>, &launch_shadow<B, D, Multi>}; \
+        return true;                                                            \
     }
     PH_WF_SHAPE(64, 8)
     PH_WF_SHAPE(64, 16)
@@ -470,11 +550,10 @@ bool shape_fns(int block, int depth, ShapeFns& f) {
     return false;
 }
 
-// Material -> class queue index: 0 metal only, 1 opaque only, 2 glass only, 3 anything else.
-// A textured metallic factor can take any value below the constant, so it counts as mixed.
 int material_class(const float4* m5) {
     const float metallic = m5[1].w, transmission = m5[2].z;
     const int mr_tex = __builtin_bit_cast(int, m5[4].x);
+    // A textured metallic factor can take any value below the constant: treat it as mixed.
     if (mr_tex >= 0) return 3;
     const float m = fminf(fmaxf(metallic, 0.0f), 1.0f), t = fminf(fmaxf(transmission, 0.0f), 1.0f);
     const bool metal = m > 0, glass = (1 - m) * t > 0, opaque = (1 - m) * (1 - t) > 0;
@@ -486,35 +565,68 @@ int material_class(const float4* m5) {
 
 }  // namespace
 
+std::vector<int> material_classes(const DScene& scene, hipStream_t st) {
+    const int nm = scene.num_materials;
+    std::vector<float4> mats(size_t(nm) * 5);
+    PH_CHECK(hipMemcpyAsync(mats.data(), scene.materials, mats.size() * sizeof(float4), hipMemcpyDeviceToHost, st));
+    PH_CHECK(hipStreamSynchronize(st));
+    std::vector<int> cls(static_cast<size_t>(nm));
+    for (int i = 0; i < nm; i++) cls[size_t(i)] = material_class(&mats[size_t(i) * 5]);
+    return cls;
+}
+
 struct WavefrontEngine::Impl {
     ph_wavefront_config cfg{128, 16, 0, 8, 1, 1, 16};
     int slots = 0;
-    DevBuf<int> pixel, sample, counters, q0, q1, q_shadow, q_term, q_miss, q_cls[kClasses], mat_class;
+    int cus = 0;
+    DevBuf<int> job, pixel, sample, sample_end, counters, q0, q1, qi0, qi1, q_shadow, q_term, q_miss, q_cls[kClasses], mat_class;
+    DevBuf<int> chunk_first, job_done;
+    DevBuf<int4> chunk_pass;
     DevBuf<uint64_t> rng;
     DevBuf<float4> ro, rd, beta, L, hit, so, sd, sc;
-    DevBuf<JobParams> params;
-    DevBuf<TraceTotals> totals;
-    DevBuf<unsigned long long> ray_totals;  // [0] extension rays traced, [1] shadow rays traced
+    DevBuf<JobParams> jobs;
+    DevBuf<TraceTotals> totals;             // [0] extension rays, [1] shadow rays
+    DevBuf<unsigned long long> ray_totals;  // rays traced: [0] extension, [1] shadow
     int* host_counters = nullptr;
-    int cus = 0;
+    int* host_done = nullptr;
+    int* host_sizes = nullptr;
     std::vector<hipEvent_t> events;
-    // Captured bounce loop and the key it was captured for.
     hipGraphExec_t graph_exec = nullptr;
-    std::tuple<int, int, int, int, int, int, bool, bool, int> graph_key{};
+    std::tuple<int, int, int, int, int, bool> graph_key{};
     std::map<std::tuple<const void*, int, int, int>, ph_wavefront_config> tuned;
+    // Serving bookkeeping (host side).
+    bool serving = false;
+    std::vector<bool> entry_used;
+    std::vector<int> entry_pixels;
+    struct Chunk {
+        int first;
+        int4 pass;
+    };
+    std::deque<Chunk> chunks;  // queue order
+    bool chunks_dirty = false;
+    int64_t total_work = 0;
 
     Impl() {
         PH_CHECK(hipHostMalloc(reinterpret_cast<void**>(&host_counters), C_COUNT * sizeof(int), hipHostMallocDefault));
+        PH_CHECK(hipHostMalloc(reinterpret_cast<void**>(&host_done), kMaxJobs * sizeof(int), hipHostMallocDefault));
+        PH_CHECK(hipHostMalloc(reinterpret_cast<void**>(&host_sizes), 2 * sizeof(int), hipHostMallocDefault));
         counters.allocate(C_COUNT);
-        params.allocate(1);
-        totals.allocate(2);  // [0] extension rays, [1] shadow rays
+        jobs.allocate(kMaxJobs);
+        chunk_first.allocate(kMaxChunks);
+        chunk_pass.allocate(kMaxChunks);
+        job_done.allocate(kMaxJobs);
+        totals.allocate(2);
         ray_totals.allocate(2);
         cus = device_cus();
+        entry_used.assign(kMaxJobs, false);
+        entry_pixels.assign(kMaxJobs, 0);
     }
     ~Impl() {
         drop_graph();
         for (hipEvent_t e : events) (void)hipEventDestroy(e);
         if (host_counters) (void)hipHostFree(host_counters);
+        if (host_done) (void)hipHostFree(host_done);
+        if (host_sizes) (void)hipHostFree(host_sizes);
     }
 
     void drop_graph() {
@@ -525,7 +637,7 @@ struct WavefrontEngine::Impl {
     void ensure_slots(int n) {
         if (n == slots) return;
         drop_graph();
-        for (DevBuf<int>* b : {&pixel, &sample, &q0, &q1, &q_shadow, &q_term, &q_miss}) b->allocate(size_t(n));
+        for (DevBuf<int>* b : {&job, &pixel, &sample, &sample_end, &q0, &q1, &qi0, &qi1, &q_shadow, &q_term, &q_miss}) b->allocate(size_t(n));
         for (auto& q : q_cls) q.allocate(size_t(n));
         rng.allocate(size_t(n));
         for (DevBuf<float4>* b : {&ro, &rd, &beta, &L, &hit, &so, &sd, &sc}) b->allocate(size_t(n));
@@ -534,8 +646,10 @@ struct WavefrontEngine::Impl {
 
     State state() {
         State s;
+        s.job = job.ptr;
         s.pixel = pixel.ptr;
         s.sample = sample.ptr;
+        s.sample_end = sample_end.ptr;
         s.rng = rng.ptr;
         s.ro = ro.ptr;
         s.rd = rd.ptr;
@@ -547,6 +661,8 @@ struct WavefrontEngine::Impl {
         s.sc = sc.ptr;
         s.q[0] = q0.ptr;
         s.q[1] = q1.ptr;
+        s.q_idle[0] = qi0.ptr;
+        s.q_idle[1] = qi1.ptr;
         s.q_shadow = q_shadow.ptr;
         s.q_term = q_term.ptr;
         s.q_miss = q_miss.ptr;
@@ -554,27 +670,118 @@ struct WavefrontEngine::Impl {
         return s;
     }
 
+    Work work() const { return Work{jobs.ptr, chunk_first.ptr, chunk_pass.ptr, job_done.ptr}; }
+
+    ShapeFns shapes(bool multi) const {
+        ShapeFns f{};
+        const bool ok = multi ? shape_fns<true>(cfg.block, cfg.stack_depth, f) : shape_fns<false>(cfg.block, cfg.stack_depth, f);
+        if (!ok) throw std::invalid_argument("unsupported wavefront launch shape");
+        return f;
+    }
+
     // One bounce (iteration parity `par`). Event pairs bracket the traversal launches when timing.
+    template <bool Multi>
     void iteration(hipStream_t st, const ShapeFns& fns, int par, bool count, hipEvent_t* ev) {
         const int tgrid = cus * std::max(1, cfg.blocks_per_cu);
         const int sgrid = cus * 4;
-        State s = state();
-        const JobParams* jp = params.ptr;
+        const State s = state();
+        const Work w = work();
         int* cnt = counters.ptr;
         hipLaunchKernelGGL(bookkeep_kernel, dim3(1), dim3(64), 0, st, cnt, par, ray_totals.ptr);
+        if (Multi) hipLaunchKernelGGL(wake_kernel, dim3(sgrid), dim3(256), 0, st, w, s, cnt, par);
         if (ev) PH_CHECK(hipEventRecord(ev[0], st));
-        fns.extend(st, tgrid, jp, s, cnt, par, totals.ptr, count);
+        fns.extend(st, tgrid, w, s, cnt, par, totals.ptr, count);
         if (ev) PH_CHECK(hipEventRecord(ev[1], st));
-        hipLaunchKernelGGL(miss_kernel, dim3(sgrid), dim3(256), 0, st, jp, s, cnt);
-        hipLaunchKernelGGL(shade_kernel<1>, dim3(sgrid), dim3(256), 0, st, jp, s, cnt, par, 0);
-        hipLaunchKernelGGL(shade_kernel<2>, dim3(sgrid), dim3(256), 0, st, jp, s, cnt, par, 1);
-        hipLaunchKernelGGL(shade_kernel<4>, dim3(sgrid), dim3(256), 0, st, jp, s, cnt, par, 2);
-        hipLaunchKernelGGL(shade_kernel<7>, dim3(sgrid), dim3(256), 0, st, jp, s, cnt, par, 3);
+        hipLaunchKernelGGL(miss_kernel<Multi>, dim3(sgrid), dim3(256), 0, st, w, s, cnt);
+        hipLaunchKernelGGL((shade_kernel<1, Multi>), dim3(sgrid), dim3(256), 0, st, w, s, cnt, par, 0);
+        hipLaunchKernelGGL((shade_kernel<2, Multi>), dim3(sgrid), dim3(256), 0, st, w, s, cnt, par, 1);
+        hipLaunchKernelGGL((shade_kernel<4, Multi>), dim3(sgrid), dim3(256), 0, st, w, s, cnt, par, 2);
+        hipLaunchKernelGGL((shade_kernel<7, Multi>), dim3(sgrid), dim3(256), 0, st, w, s, cnt, par, 3);
         if (ev) PH_CHECK(hipEventRecord(ev[2], st));
-        fns.shadow(st, tgrid, jp, s, cnt, par, totals.ptr + 1, count);
+        fns.shadow(st, tgrid, w, s, cnt, par, totals.ptr + 1, count);
         if (ev) PH_CHECK(hipEventRecord(ev[3], st));
-        hipLaunchKernelGGL(regen_kernel, dim3(sgrid), dim3(256), 0, st, jp, s, cnt, par);
+        hipLaunchKernelGGL(regen_kernel<Multi>, dim3(sgrid), dim3(256), 0, st, w, s, cnt, par);
         PH_CHECK(hipGetLastError());
+    }
+
+    void ensure_events(int n) {
+        for (size_t i = events.size(); i < size_t(n); i++) {
+            hipEvent_t e;
+            PH_CHECK(hipEventCreate(&e));
+            events.push_back(e);
+        }
+    }
+
+    // Runs G bounces (G even) from a graph or directly; returns (ms_extend, ms_shadow) when timed.
+    std::pair<double, double> run_batch(hipStream_t st, int G, bool multi, bool counting) {
+        const ShapeFns fns = shapes(multi);
+        const bool graph = cfg.use_graph != 0 && !counting;
+        if (graph) {
+            auto key = std::make_tuple(cfg.block, cfg.stack_depth, cfg.blocks_per_cu, G, slots, multi);
+            if (!graph_exec || key != graph_key) {
+                drop_graph();
+                hipGraph_t g = nullptr;
+                PH_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                for (int it = 0; it < G; it++) {
+                    if (multi)
+                        iteration<true>(st, fns, it & 1, false, nullptr);
+                    else
+                        iteration<false>(st, fns, it & 1, false, nullptr);
+                }
+                PH_CHECK(hipStreamEndCapture(st, &g));
+                PH_CHECK(hipGraphInstantiate(&graph_exec, g, nullptr, nullptr, 0));
+                (void)hipGraphDestroy(g);
+                graph_key = key;
+            }
+            PH_CHECK(hipGraphLaunch(graph_exec, st));
+            PH_CHECK(hipMemcpyAsync(host_counters, counters.ptr, C_COUNT * sizeof(int), hipMemcpyDeviceToHost, st));
+            PH_CHECK(hipStreamSynchronize(st));
+            return {0.0, 0.0};
+        }
+        // Launched directly: HIP events bracket every traversal launch (graph-recorded events cannot
+        // be timed, so a graph replay reports only its total).
+        ensure_events(4 * G);
+        for (int it = 0; it < G; it++) {
+            if (multi)
+                iteration<true>(st, fns, it & 1, counting, &events[size_t(4 * it)]);
+            else
+                iteration<false>(st, fns, it & 1, counting, &events[size_t(4 * it)]);
+        }
+        PH_CHECK(hipMemcpyAsync(host_counters, counters.ptr, C_COUNT * sizeof(int), hipMemcpyDeviceToHost, st));
+        PH_CHECK(hipStreamSynchronize(st));
+        double me = 0, ms = 0;
+        for (int it = 0; it < G; it++) {
+            float a = 0, b = 0;
+            PH_CHECK(hipEventElapsedTime(&a, events[size_t(4 * it)], events[size_t(4 * it + 1)]));
+            PH_CHECK(hipEventElapsedTime(&b, events[size_t(4 * it + 2)], events[size_t(4 * it + 3)]));
+            me += a;
+            ms += b;
+        }
+        return {me, ms};
+    }
+
+    void reset_counters(hipStream_t st) {
+        PH_CHECK(hipMemsetAsync(counters.ptr, 0, C_COUNT * sizeof(int), st));
+        PH_CHECK(hipMemsetAsync(totals.ptr, 0, 2 * sizeof(TraceTotals), st));
+        PH_CHECK(hipMemsetAsync(ray_totals.ptr, 0, 2 * sizeof(unsigned long long), st));
+    }
+
+    JobParams params_of(const WavefrontJob& job, const int* cls) const {
+        JobParams jp{};
+        jp.scene = job.scene;
+        jp.camera = job.camera;
+        jp.width = job.width;
+        jp.height = job.height;
+        jp.spp = job.spp;
+        jp.pixels = job.width * job.height;
+        jp.frame0 = job.frame0;
+        jp.seed = job.seed;
+        jp.opt = job.opt;
+        jp.accum = job.accum;
+        jp.albedo = job.albedo;
+        jp.nd = job.nd;
+        jp.mat_class = cls;
+        return jp;
     }
 };
 
@@ -583,9 +790,9 @@ WavefrontEngine::~WavefrontEngine() = default;
 
 void WavefrontEngine::set_config(const ph_wavefront_config& c) {
     ShapeFns f;
-    if (!shape_fns(c.block, c.stack_depth, f)) throw std::invalid_argument("unsupported block/stack depth for the wavefront engine");
+    if (!shape_fns<false>(c.block, c.stack_depth, f)) throw std::invalid_argument("unsupported block/stack depth for the wavefront engine");
     if (c.path_slots < 0 || c.blocks_per_cu <= 0 || c.blocks_per_cu > 64) throw std::invalid_argument("invalid wavefront configuration");
-    if (c.use_graph && (c.graph_iterations < 2 || c.graph_iterations > 256 || c.graph_iterations % 2))
+    if (c.graph_iterations < 2 || c.graph_iterations > 256 || c.graph_iterations % 2)
         throw std::invalid_argument("graph_iterations must be even and in [2, 256]");
     impl_->cfg = c;
     impl_->drop_graph();
@@ -599,95 +806,36 @@ void WavefrontEngine::render(hipStream_t st, const WavefrontJob& job, bool count
     const auto t0 = std::chrono::steady_clock::now();
     const int pixels = job.width * job.height;
     if (pixels <= 0 || job.spp <= 0) return;
-    ShapeFns fns;
-    if (!shape_fns(m.cfg.block, m.cfg.stack_depth, fns)) throw std::invalid_argument("unsupported wavefront launch shape");
-    // Paths in flight: one per pixel up to a cap of 4M slots (~730 MB of state).
-    int slots = m.cfg.path_slots > 0 ? std::min(m.cfg.path_slots, pixels) : std::min(pixels, 1 << 22);
+    if (m.serving) throw std::runtime_error("the engine is serving; a progressive render needs its own engine");
+    // Paths in flight: one per pixel up to a cap of 4M slots (~900 MB of state).
+    const int slots = m.cfg.path_slots > 0 ? std::min(m.cfg.path_slots, pixels) : std::min(pixels, 1 << 22);
     m.ensure_slots(slots);
+    if (!job.mat_class) {
+        std::vector<int> cls = material_classes(job.scene, st);
+        m.mat_class.upload(cls.data(), cls.size(), st);
+    }
+    const JobParams jp = m.params_of(job, job.mat_class ? job.mat_class : m.mat_class.ptr);
+    PH_CHECK(hipMemcpyAsync(m.jobs.ptr, &jp, sizeof(jp), hipMemcpyHostToDevice, st));
+    m.reset_counters(st);
+    // Work items 0..slots-1 go to the slots directly; regeneration hands out the rest.
+    const int init[2] = {std::min(slots, pixels), pixels};
+    PH_CHECK(hipMemcpyAsync(m.counters.ptr + C_WORK, init, sizeof(init), hipMemcpyHostToDevice, st));
+    PH_CHECK(hipMemsetAsync(m.job_done.ptr, 0, sizeof(int), st));
+    hipLaunchKernelGGL(init_kernel, dim3((slots + 255) / 256), dim3(256), 0, st, m.work(), m.state(), m.counters.ptr, slots);
+    PH_CHECK(hipGetLastError());
 
-    // Per-material class table and job parameters (device copies, read by every kernel).
-    const int nm = job.scene.num_materials;
-    std::vector<float4> mats(size_t(nm) * 5);
-    PH_CHECK(hipMemcpyAsync(mats.data(), job.scene.materials, mats.size() * sizeof(float4), hipMemcpyDeviceToHost, st));
-    PH_CHECK(hipStreamSynchronize(st));
-    std::vector<int> cls(static_cast<size_t>(nm));
-    for (int i = 0; i < nm; i++) cls[size_t(i)] = material_class(&mats[size_t(i) * 5]);
-    m.mat_class.upload(cls.data(), cls.size(), st);
-    JobParams jp{};
-    jp.scene = job.scene;
-    jp.camera = job.camera;
-    jp.width = job.width;
-    jp.height = job.height;
-    jp.spp = job.spp;
-    jp.total_work = pixels;
-    jp.frame0 = job.frame0;
-    jp.seed = job.seed;
-    jp.opt = job.opt;
-    jp.accum = job.accum;
-    jp.albedo = job.albedo;
-    jp.nd = job.nd;
-    jp.mat_class = m.mat_class.ptr;
-    PH_CHECK(hipMemcpyAsync(m.params.ptr, &jp, sizeof(jp), hipMemcpyHostToDevice, st));
-    PH_CHECK(hipMemsetAsync(m.counters.ptr, 0, C_COUNT * sizeof(int), st));
-    PH_CHECK(hipMemsetAsync(m.totals.ptr, 0, 2 * sizeof(TraceTotals), st));
-    PH_CHECK(hipMemsetAsync(m.ray_totals.ptr, 0, 2 * sizeof(unsigned long long), st));
-    int* cnt = m.counters.ptr;
-    {
-        // Work items 0..slots-1 go to the slots directly; the rest are handed out by regeneration.
-        int first = std::min(slots, pixels);
-        PH_CHECK(hipMemcpyAsync(cnt + C_WORK, &first, sizeof(int), hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(init_kernel, dim3((slots + 255) / 256), dim3(256), 0, st, m.params.ptr, m.state(), cnt, slots);
-        PH_CHECK(hipGetLastError());
-    }
-
-    const int G = std::max(2, m.cfg.graph_iterations);
-    const bool timing = true;
-    if (m.events.size() < size_t(4 * G)) {
-        for (size_t i = m.events.size(); i < size_t(4 * G); i++) {
-            hipEvent_t e;
-            PH_CHECK(hipEventCreate(&e));
-            m.events.push_back(e);
-        }
-    }
-    const bool graph = m.cfg.use_graph != 0 && !counting;
-    if (graph) {
-        auto key = std::make_tuple(m.cfg.block, m.cfg.stack_depth, m.cfg.blocks_per_cu, G, slots, m.cfg.sort_materials, counting, timing,
-                                   int(reinterpret_cast<uintptr_t>(st) & 0x7fffffff));
-        if (!m.graph_exec || key != m.graph_key) {
-            m.drop_graph();
-            hipGraph_t g = nullptr;
-            PH_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            for (int it = 0; it < G; it++) m.iteration(st, fns, it & 1, false, timing ? &m.events[size_t(4 * it)] : nullptr);
-            PH_CHECK(hipStreamEndCapture(st, &g));
-            PH_CHECK(hipGraphInstantiate(&m.graph_exec, g, nullptr, nullptr, 0));
-            (void)hipGraphDestroy(g);
-            m.graph_key = key;
-        }
-    }
+    const int G = m.cfg.graph_iterations;
+    // A sample takes at most max_depth bounces and a slot runs ceil(pixels / slots) pixels.
+    const int64_t bound = int64_t(job.spp) * int64_t(std::max(1, job.opt.max_depth)) * int64_t((pixels + slots - 1) / slots) + 2 * G;
     double ms_extend = 0, ms_shadow = 0;
     int64_t iterations = 0;
     for (;;) {
-        if (graph) {
-            PH_CHECK(hipGraphLaunch(m.graph_exec, st));
-        } else {
-            for (int it = 0; it < G; it++) m.iteration(st, fns, it & 1, counting, timing ? &m.events[size_t(4 * it)] : nullptr);
-        }
-        PH_CHECK(hipMemcpyAsync(m.host_counters, cnt, C_COUNT * sizeof(int), hipMemcpyDeviceToHost, st));
-        PH_CHECK(hipStreamSynchronize(st));
-        for (int it = 0; it < G; it++) {
-            float a = 0, b = 0;
-            PH_CHECK(hipEventElapsedTime(&a, m.events[size_t(4 * it)], m.events[size_t(4 * it + 1)]));
-            PH_CHECK(hipEventElapsedTime(&b, m.events[size_t(4 * it + 2)], m.events[size_t(4 * it + 3)]));
-            ms_extend += a;
-            ms_shadow += b;
-        }
+        auto t = m.run_batch(st, G, false, counting);
+        ms_extend += t.first;
+        ms_shadow += t.second;
         iterations += G;
-        stats.extend_launches += G;
-        stats.shadow_launches += G;
         // G is even, so the next iteration reads queue 0.
         if (m.host_counters[C_EXT0] == 0) break;
-        // A sample takes at most max_depth iterations and a slot runs ceil(pixels / slots) pixels.
-        const int64_t bound = int64_t(job.spp) * int64_t(std::max(1, job.opt.max_depth)) * int64_t((pixels + slots - 1) / slots) + 2 * G;
         if (iterations > bound) throw std::runtime_error("wavefront loop did not drain");
     }
     TraceTotals tot[2] = {};
@@ -699,6 +847,8 @@ void WavefrontEngine::render(hipStream_t st, const WavefrontJob& job, bool count
     stats.ms_extend = ms_extend;
     stats.ms_shadow = ms_shadow;
     stats.iterations = iterations;
+    stats.extend_launches = iterations;
+    stats.shadow_launches = iterations;
     stats.shade_launches = iterations * kClasses;
     stats.samples = int64_t(pixels) * job.spp;
     stats.node_visits = int64_t(tot[0].nodes);
@@ -718,18 +868,19 @@ ph_wavefront_config WavefrontEngine::autotune(hipStream_t st, const WavefrontJob
         set_config(found->second);
         return found->second;
     }
-    const ph_wavefront_config start = m.cfg;
+    const size_t accum_bytes = size_t(job.width) * size_t(job.height) * sizeof(float4);
     auto time_of = [&](const ph_wavefront_config& c) {
         set_config(c);
         ph_render_stats s{};
-        render(st, job, false, s);  // warm (graph capture, allocation)
-        PH_CHECK(hipMemsetAsync(job.accum, 0, size_t(job.width) * size_t(job.height) * sizeof(float4), st));
+        render(st, job, false, s);  // warm: allocation, graph capture
+        PH_CHECK(hipMemsetAsync(job.accum, 0, accum_bytes, st));
+        const auto t0 = std::chrono::steady_clock::now();
         render(st, job, false, s);
-        return s.ms_total;
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     };
-    ph_wavefront_config best = start;
+    ph_wavefront_config best = m.cfg;
     double best_ms = time_of(best);
-    // Coordinate descent: traversal shape, then persistent grid size, then paths in flight.
+    // Coordinate descent: traversal shape, persistent grid size, paths in flight, graph replay.
     for (int block : {64, 128, 256})
         for (int depth : {8, 16, 24}) {
             ph_wavefront_config c = best;
@@ -746,17 +897,121 @@ ph_wavefront_config WavefrontEngine::autotune(hipStream_t st, const WavefrontJob
     }
     const int pixels = job.width * job.height;
     for (int slots : {pixels / 4, pixels / 2, 0}) {
-        if (slots < 0 || (slots > 0 && slots < 65536)) continue;
+        if (slots > 0 && slots < 65536) continue;
         ph_wavefront_config c = best;
         c.path_slots = slots;
         double t = time_of(c);
         if (t < best_ms) best_ms = t, best = c;
     }
+    for (int g : {0, 1}) {
+        ph_wavefront_config c = best;
+        c.use_graph = g;
+        double t = time_of(c);
+        if (t < best_ms) best_ms = t, best = c;
+    }
     set_config(best);
     m.tuned[key] = best;
-    PH_CHECK(hipMemsetAsync(job.accum, 0, size_t(job.width) * size_t(job.height) * sizeof(float4), st));
+    PH_CHECK(hipMemsetAsync(job.accum, 0, accum_bytes, st));
     PH_CHECK(hipStreamSynchronize(st));
     return best;
 }
+
+void WavefrontEngine::serve_begin(hipStream_t st, int slots) {
+    Impl& m = *impl_;
+    if (slots <= 0) throw std::invalid_argument("the serving engine needs at least one path slot");
+    m.ensure_slots(slots);
+    m.serving = true;
+    m.entry_used.assign(kMaxJobs, false);
+    m.entry_pixels.assign(kMaxJobs, 0);
+    m.chunks.clear();
+    m.chunks_dirty = false;
+    m.total_work = 0;
+    m.reset_counters(st);
+    PH_CHECK(hipMemsetAsync(m.job_done.ptr, 0, kMaxJobs * sizeof(int), st));
+    hipLaunchKernelGGL(init_idle_kernel, dim3((slots + 255) / 256), dim3(256), 0, st, m.state(), m.counters.ptr, slots);
+    PH_CHECK(hipGetLastError());
+    PH_CHECK(hipMemcpyAsync(m.host_counters, m.counters.ptr, C_COUNT * sizeof(int), hipMemcpyDeviceToHost, st));
+    PH_CHECK(hipStreamSynchronize(st));
+    std::memset(m.host_done, 0, kMaxJobs * sizeof(int));
+}
+
+int WavefrontEngine::serve_admit(hipStream_t st, const WavefrontJob& job) {
+    Impl& m = *impl_;
+    if (!m.serving) throw std::runtime_error("serve_begin was not called");
+    if (job.width <= 0 || job.height <= 0 || job.spp <= 0) throw std::invalid_argument("empty job");
+    if (!job.mat_class) throw std::invalid_argument("serving jobs need a per-scene material class table");
+    int entry = -1;
+    for (int e = 0; e < kMaxJobs; e++)
+        if (!m.entry_used[size_t(e)]) {
+            entry = e;
+            break;
+        }
+    if (entry < 0) return -1;
+    const JobParams jp = m.params_of(job, job.mat_class);
+    PH_CHECK(hipMemcpyAsync(m.jobs.ptr + entry, &jp, sizeof(jp), hipMemcpyHostToDevice, st));
+    PH_CHECK(hipMemsetAsync(m.job_done.ptr + entry, 0, sizeof(int), st));
+    m.host_done[entry] = 0;
+    m.entry_used[size_t(entry)] = true;
+    m.entry_pixels[size_t(entry)] = job.width * job.height;
+    return entry;
+}
+
+bool WavefrontEngine::serve_enqueue(int entry, int k0, int k1) {
+    Impl& m = *impl_;
+    if (entry < 0 || entry >= kMaxJobs || !m.entry_used[size_t(entry)]) throw std::invalid_argument("not an admitted job");
+    if (k0 < 0 || k1 <= k0) throw std::invalid_argument("empty pass");
+    const int pixels = m.entry_pixels[size_t(entry)];
+    // Drop chunks whose items have all been handed out (their pixels keep their pass through the slot).
+    const int handed = m.host_counters[C_WORK];
+    while (m.chunks.size() > 1 && m.chunks[1].first <= handed) {
+        m.chunks.pop_front();
+        m.chunks_dirty = true;
+    }
+    if (int(m.chunks.size()) >= kMaxChunks) return false;
+    // Lanes that looked for work when none was left still advanced the counter past the queued total;
+    // the new items start after the counter so that none of them is skipped.
+    const int64_t first = std::max<int64_t>(m.total_work, handed);
+    if (first + pixels > int64_t(INT32_MAX) - (int64_t(m.slots) + 64)) throw std::runtime_error("work index space exhausted; restart the server");
+    m.chunks.push_back(Impl::Chunk{int(first), make_int4(entry, k0, k1, 0)});
+    m.total_work = first + pixels;
+    m.chunks_dirty = true;
+    return true;
+}
+
+void WavefrontEngine::serve_run(hipStream_t st, int iterations) {
+    Impl& m = *impl_;
+    if (!m.serving) throw std::runtime_error("serve_begin was not called");
+    if (m.chunks_dirty) {
+        std::vector<int> starts;
+        std::vector<int4> passes;
+        for (const auto& c : m.chunks) {
+            starts.push_back(c.first);
+            passes.push_back(c.pass);
+        }
+        if (!starts.empty()) {
+            PH_CHECK(hipMemcpyAsync(m.chunk_first.ptr, starts.data(), starts.size() * sizeof(int), hipMemcpyHostToDevice, st));
+            PH_CHECK(hipMemcpyAsync(m.chunk_pass.ptr, passes.data(), passes.size() * sizeof(int4), hipMemcpyHostToDevice, st));
+        }
+        // Staged in pinned memory: the copy runs after this call returns.
+        m.host_sizes[0] = int(m.total_work);
+        m.host_sizes[1] = int(m.chunks.size());
+        PH_CHECK(hipMemcpyAsync(m.counters.ptr + C_TOTAL, m.host_sizes, 2 * sizeof(int), hipMemcpyHostToDevice, st));
+        m.chunks_dirty = false;
+    }
+    const int G = std::max(2, iterations + (iterations & 1));
+    m.run_batch(st, G, true, false);
+    PH_CHECK(hipMemcpyAsync(m.host_done, m.job_done.ptr, kMaxJobs * sizeof(int), hipMemcpyDeviceToHost, st));
+    PH_CHECK(hipStreamSynchronize(st));
+}
+
+int WavefrontEngine::serve_pixels_done(int entry) const { return impl_->host_done[entry]; }
+
+void WavefrontEngine::serve_release(int entry) {
+    if (entry >= 0 && entry < kMaxJobs) impl_->entry_used[size_t(entry)] = false;
+}
+
+int WavefrontEngine::serve_in_flight() const { return impl_->host_counters[C_EXT0]; }
+
+int64_t WavefrontEngine::serve_work_remaining() const { return impl_->total_work - impl_->host_counters[C_WORK]; }
 
 }  // namespace phk

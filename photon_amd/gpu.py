@@ -10,7 +10,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from .types import (CameraView, RenderSettings, SceneDesc, SvgfSettings, DISPLAY_COLOR)
+from .types import (CameraView, PathOptions, RenderSettings, SceneDesc, SvgfSettings, DISPLAY_COLOR)
 
 ENGINE_MEGAKERNEL, ENGINE_WAVEFRONT = 0, 1
 BUF_ACCUM, BUF_ALBEDO, BUF_NORMAL_DEPTH, BUF_COLOR, BUF_POSITION, BUF_EMISSION, BUF_DENOISED, BUF_HEAT = range(8)
@@ -40,6 +40,24 @@ class RenderStats(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("memory_budget", C.c_int64), ("path_slots", C.c_int32), ("iterations_per_step", C.c_int32),
+                ("max_jobs", C.c_int32), ("pass_samples", C.c_int64)]
+
+
+class JobDesc(C.Structure):
+    _fields_ = [("scene", C.c_int32), ("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32),
+                ("camera", CameraView), ("path", PathOptions), ("seed", C.c_uint32), ("frame0", C.c_uint32)]
+
+
+class JobStatus(C.Structure):
+    _fields_ = [("state", C.c_int32), ("pixels", C.c_int32), ("samples_done", C.c_int64), ("samples", C.c_int64),
+                ("submit_ms", C.c_double), ("admit_ms", C.c_double), ("done_ms", C.c_double)]
+
+
+JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_RELEASED = range(4)
 
 
 class SvgfInput(C.Structure):
@@ -103,6 +121,19 @@ def lib():
         L.ph_copy_to_device.argtypes = [P, P, C.c_int64]
         L.ph_copy_to_host.argtypes = [P, P, C.c_int64]
         L.ph_device_memset.argtypes = [P, C.c_int, C.c_int64]
+        L.ph_server_create.argtypes = [C.POINTER(ServerConfig), C.POINTER(P)]
+        L.ph_server_destroy.argtypes = [P]
+        L.ph_server_destroy.restype = None
+        L.ph_server_set_wavefront_config.argtypes = [P, C.POINTER(WavefrontConfig)]
+        L.ph_server_add_scene.argtypes = [P, C.POINTER(SceneDesc)]
+        L.ph_server_submit.argtypes = [P, C.POINTER(JobDesc)]
+        L.ph_server_step.argtypes = [P]
+        L.ph_server_status.argtypes = [P, C.c_int, C.POINTER(JobStatus)]
+        L.ph_server_result.argtypes = [P, C.c_int, P]
+        L.ph_server_release.argtypes = [P, C.c_int]
+        L.ph_server_pending.argtypes = [P]
+        L.ph_server_now_ms.argtypes = [P]
+        L.ph_server_now_ms.restype = C.c_double
         _sig_done = True
     return L
 
@@ -322,3 +353,65 @@ class Svgf:
 
     def last_ms(self):
         return lib().ph_svgf_last_ms(self._h)
+
+
+class Server:
+    """The serving loop (ph_server_*): resident scenes, jobs admitted by HBM footprint, continuous batching."""
+
+    def __init__(self, memory_budget=0, path_slots=0, iterations_per_step=0, max_jobs=0, pass_samples=0):
+        self._h = C.c_void_p()
+        cfg = ServerConfig(memory_budget, path_slots, iterations_per_step, max_jobs, pass_samples)
+        _check(lib().ph_server_create(C.byref(cfg), C.byref(self._h)))
+        self._sizes = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ph_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_wavefront_config(self, cfg):
+        _check(lib().ph_server_set_wavefront_config(self._h, C.byref(cfg)))
+
+    def add_scene(self, scene):
+        d = scene.desc() if hasattr(scene, "desc") else scene
+        return _check(lib().ph_server_add_scene(self._h, C.byref(d)))
+
+    def submit(self, scene, width, height, spp, camera, max_depth=8, rr_depth=3, clamp=0.0, nee=1, seed=0, frame0=0):
+        job = JobDesc(scene, width, height, spp, camera, PathOptions(max_depth, rr_depth, clamp, nee), seed, frame0)
+        jid = _check(lib().ph_server_submit(self._h, C.byref(job)))
+        self._sizes[jid] = (height, width)
+        return jid
+
+    def step(self):
+        return _check(lib().ph_server_step(self._h))
+
+    def status(self, job):
+        st = JobStatus()
+        _check(lib().ph_server_status(self._h, job, C.byref(st)))
+        return st
+
+    def result(self, job):
+        h, w = self._sizes[job]
+        out = np.zeros((h, w, 3), np.float32)
+        _check(lib().ph_server_result(self._h, job, out.ctypes.data))
+        return out
+
+    def release(self, job):
+        _check(lib().ph_server_release(self._h, job))
+
+    @property
+    def pending(self):
+        return lib().ph_server_pending(self._h)
+
+    def now_ms(self):
+        return lib().ph_server_now_ms(self._h)
+
+    def run_until_idle(self, max_steps=1 << 20):
+        for _ in range(max_steps):
+            if self.pending == 0:
+                return
+            self.step()
+        raise RuntimeError("server did not drain")

@@ -1,0 +1,106 @@
+"""The serving loop: jobs over resident scenes, admitted by frame-buffer footprint, batched together.
+
+A pixel's samples are taken in order by one path slot, so a job's image does not depend on what else
+shares the wavefronts: it must be bit-identical to the same job rendered alone, and to the single-job
+renderer (which follows the reference's GpuRenderer::render), whatever the arrival order, slot count
+and admission budget.
+"""
+import numpy as np
+import pytest
+
+from photon_amd import gpu, host
+from photon_amd.types import RenderSettings
+
+from gpu_util import uploaded
+
+pytestmark = pytest.mark.gpu
+
+# (scene name, width, height, spp, max_depth, seed): mixed resolutions, sample counts and depths.
+JOBS = [("cornell-c1", 64, 64, 8, 8, 1), ("materials", 96, 54, 4, 6, 2), ("glass", 40, 30, 12, 10, 3),
+        ("cornell-c1", 33, 17, 5, 3, 4), ("furnace", 48, 48, 2, 8, 5), ("materials", 20, 90, 6, 12, 6)]
+
+
+@pytest.fixture(scope="module")
+def scenes():
+    return {n: host.Scene.builtin(n).build() for n in sorted({j[0] for j in JOBS})}
+
+
+def _alone(scene, w, h, spp, depth, seed):
+    r = uploaded(scene, w, h, engine=gpu.ENGINE_WAVEFRONT)
+    r.render(scene.camera_view(w / h), RenderSettings.make(w, h, max_depth=depth, rr_depth=3, seed=seed), spp)
+    img = r.download_image()
+    r.close()
+    return img
+
+
+def _serve(scenes, jobs, order, **cfg):
+    srv = gpu.Server(**cfg)
+    ids = {n: srv.add_scene(s) for n, s in scenes.items()}
+    jid = {}
+    for k in order:
+        n, w, h, spp, depth, seed = jobs[k]
+        jid[k] = srv.submit(ids[n], w, h, spp, scenes[n].camera_view(w / h), max_depth=depth, seed=seed)
+    srv.run_until_idle()
+    out = {}
+    for k in order:
+        st = srv.status(jid[k])
+        assert st.state == gpu.JOB_DONE and st.samples_done == st.samples == st.pixels * jobs[k][3]
+        assert st.submit_ms <= st.admit_ms <= st.done_ms
+        out[k] = srv.result(jid[k])
+    srv.close()
+    return out
+
+
+def test_each_job_alone_equals_the_renderer(scenes):
+    for k, (n, w, h, spp, depth, seed) in enumerate(JOBS):
+        ref = _alone(scenes[n], w, h, spp, depth, seed)
+        got = _serve(scenes, JOBS, [k], path_slots=1 << 14)[k]
+        assert got.tobytes() == ref.tobytes(), f"job {k} ({n} {w}x{h})"
+
+
+@pytest.mark.parametrize("cfg", [dict(path_slots=1 << 14), dict(path_slots=3000, iterations_per_step=2),
+                                 dict(path_slots=1 << 16, max_jobs=2), dict(path_slots=5000, memory_budget=64 * 64 * 16),
+                                 dict(path_slots=4096, pass_samples=3000), dict(path_slots=2000, pass_samples=1)])
+def test_jobs_in_a_mix_equal_jobs_alone(scenes, cfg):
+    alone = {k: _serve(scenes, JOBS, [k], path_slots=1 << 14)[k] for k in range(len(JOBS))}
+    order = [3, 0, 5, 1, 4, 2]
+    mixed = _serve(scenes, JOBS, order, **cfg)
+    for k in order:
+        assert mixed[k].tobytes() == alone[k].tobytes(), f"job {k} differs in the mix ({cfg})"
+
+
+def test_jobs_submitted_while_others_run(scenes):
+    """Admission between steps: new work items start after those already handed out."""
+    srv = gpu.Server(path_slots=2048, iterations_per_step=2)
+    ids = {n: srv.add_scene(s) for n, s in scenes.items()}
+    jid = {}
+    pending = list(range(len(JOBS)))
+    steps = 0
+    while pending or srv.pending:
+        if pending and steps % 3 == 0:
+            k = pending.pop(0)
+            n, w, h, spp, depth, seed = JOBS[k]
+            jid[k] = srv.submit(ids[n], w, h, spp, scenes[n].camera_view(w / h), max_depth=depth, seed=seed)
+        srv.step()
+        steps += 1
+        assert steps < 100000
+    for k, j in jid.items():
+        n, w, h, spp, depth, seed = JOBS[k]
+        ref = _alone(scenes[n], w, h, spp, depth, seed)
+        assert srv.result(j).tobytes() == ref.tobytes(), f"job {k}"
+    srv.close()
+
+
+def test_errors():
+    srv = gpu.Server(path_slots=1024)
+    with pytest.raises(RuntimeError):
+        srv.submit(0, 8, 8, 1, host.Scene.builtin("cornell").build().camera_view(1.0))
+    sid = srv.add_scene(host.Scene.builtin("cornell").build())
+    j = srv.submit(sid, 8, 8, 1, host.Scene.builtin("cornell").build().camera_view(1.0))
+    with pytest.raises(RuntimeError, match="not done"):
+        srv.result(j)
+    srv.run_until_idle()
+    srv.result(j)
+    with pytest.raises(RuntimeError, match="already"):
+        srv.result(j)
+    srv.close()
