@@ -25,6 +25,13 @@ METRIC = "Msamples/s at 1080p 64spp; traversal Mrays/s; request p50/p99 latency"
 # Algorithmic bytes of one traversal: ray in 28, hit out 16, 64 per node visit, 48 per triangle test,
 # 80 + 12 + 24 + 16 per alpha test (material, indices, uvs, 4 texels).
 RAY_IO, NODE_B, TRI_B, ALPHA_B = 28 + 16, 64, 48, 80 + 12 + 24 + 16
+# Algorithmic bytes of one SVGF frame per pixel with the 16-byte planes the denoiser reads: each pass
+# reads its distinct inputs once and writes its outputs once. temporal: 5 current planes + 4 history
+# planes in, illumination + moments out (11 x 16); variance: illumination, moments, normal, position in,
+# 1 out (5 x 16); five a-trous passes: filtered, normal, position in, 1 out (4 x 16 each); history:
+# filtered, illumination, moments, normal, position in, 4 out (9 x 16); remodulate: filtered, colour,
+# albedo, emission, position in, 1 out (6 x 16). Total 816 bytes.
+SVGF_BYTES_PER_PIXEL = 11 * 16 + 5 * 16 + 5 * 4 * 16 + 9 * 16 + 6 * 16
 
 
 def log(msg):
@@ -134,6 +141,50 @@ def bench_traversal(gpu, host):
         res[name] = dict(mrays_s=len(rays) / (best[0] * 1e-3) / 1e6, ms=best[0], block=best[1], stack_depth=best[2])
         log(f"traversal {name}: {res[name]}")
     r.close()
+    return res
+
+
+def bench_svgf(args, gpu, host, field):
+    """Real-time sequence: 1 spp G-buffer trace + SVGF per frame at 1600x900, camera orbiting 0.5 deg/frame."""
+    import math
+    from photon_amd.types import RenderSettings, SvgfSettings
+    W, H = 1600, 900
+    cam = field.camera()
+    target = cam["target"]
+    base = [cam["position"][i] - target[i] for i in range(3)]
+    rs = RenderSettings.make(W, H, max_depth=16, rr_depth=3, seed=1)
+    sv = SvgfSettings.make()
+    r = gpu.GpuRenderer()
+    ok, err = r.upload(field)
+    assert ok, err
+    r.resize(W, H)
+
+    def view(frame):
+        a = math.radians(0.5 * frame)
+        c, s = math.cos(a), math.sin(a)
+        p = (target[0] + c * base[0] + s * base[2], target[1] + base[1], target[2] - s * base[0] + c * base[2])
+        field.set_camera(p, target, cam["up"], cam["vfov"])
+        return field.camera_view(W / H)
+
+    prev = view(0)
+    trace_ms, svgf_ms, wall = [], [], []
+    for f in range(args.svgf_frames):
+        cur = view(f)
+        t0 = time.perf_counter()
+        r.render_denoised(cur, prev, rs, sv)
+        wall.append((time.perf_counter() - t0) * 1e3)
+        st = r.stats()
+        trace_ms.append(st.ms_shade)
+        svgf_ms.append(st.ms_other)
+        prev = cur
+    field.set_camera(cam["position"], target, cam["up"], cam["vfov"])
+    r.close()
+    k = slice(1, None)  # the first frame also allocates
+    res = dict(frames=args.svgf_frames, width=W, height=H, svgf_ms=round(sum(svgf_ms[k]) / len(svgf_ms[k]), 4),
+               gbuffer_trace_ms=round(sum(trace_ms[k]) / len(trace_ms[k]), 3), frame_ms=round(sum(wall[k]) / len(wall[k]), 3),
+               svgf_bytes_per_pixel=SVGF_BYTES_PER_PIXEL)
+    res["svgf_achieved_gbs"] = round(W * H * SVGF_BYTES_PER_PIXEL / (res["svgf_ms"] * 1e-3) / 1e9, 1)
+    log(f"svgf: {res}")
     return res
 
 
@@ -269,6 +320,8 @@ def main():
     ap.add_argument("--no-traversal", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-svgf", action="store_true")
+    ap.add_argument("--svgf-frames", type=int, default=64)
     ap.add_argument("--serve-jobs", type=int, default=200, help="jobs of the open-loop serving run (the full stream is 10,000)")
     ap.add_argument("--serve-burst", type=int, default=48)
     ap.add_argument("--serve-load", type=float, default=0.8)
@@ -285,6 +338,7 @@ def main():
     log(f"device {info.name.decode()} {info.arch.decode()} CUs={info.compute_units}")
     scene, cam, rs, rnd = bench_render(args, gpu, host, RenderSettings)
     trav = None if args.no_traversal else bench_traversal(gpu, host)
+    svgf = None if args.no_svgf else bench_svgf(args, gpu, host, scene)
     serve = None if args.no_serve else bench_serving(args, gpu, host, scene)
     cpu = None if args.no_cpu else cpu_baseline(scene, cam, rs)
     line = {
@@ -313,6 +367,7 @@ def main():
         "megakernel_msamples_s": round(rnd.get("megakernel_msamples_s", 0.0), 3) or None,
         "speedup_vs_megakernel": round(rnd.get("speedup_vs_megakernel", 0.0), 3) or None,
         "traversal": trav,
+        "svgf": svgf,
         "serving": serve,
     }
     print(json.dumps(line), flush=True)

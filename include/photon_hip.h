@@ -57,6 +57,7 @@ typedef struct ph_wavefront_config {
     int32_t sort_materials;/* shade in material-class queues */
     int32_t use_graph;     /* replay the bounce loop from a captured HIP graph */
     int32_t graph_iterations; /* bounces per captured graph */
+    int32_t refill_lanes;  /* a traversal wave takes new rays once this many of its 64 lanes are free */
 } ph_wavefront_config;
 
 /* Statistics of the last render call. After render_denoised, ms_shade is the one-sample G-buffer trace
@@ -157,7 +158,7 @@ typedef struct ph_server ph_server;
 typedef struct ph_server_config {
     int64_t memory_budget;       /* bytes of job frame buffers admitted at once (0: 16 GiB) */
     int32_t path_slots;          /* paths in flight shared by all jobs (0: 2^21) */
-    int32_t iterations_per_step; /* bounces per ph_server_step, even (0: 8) */
+    int32_t iterations_per_step; /* bounces per ph_server_step (0: 8) */
     int32_t max_jobs;            /* jobs admitted at once, at most 1024 (0: 1024) */
     int64_t pass_samples;        /* pixel samples per pass of a job (0: 4 x path_slots) */
 } ph_server_config;

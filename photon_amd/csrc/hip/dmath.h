@@ -15,9 +15,44 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #define PH_DEV __device__ __forceinline__
 
 namespace phd {
+
+// A load through a pointer known to address device (global) memory. Pointers read from memory are
+// generic to the compiler, which would otherwise issue flat loads for them. Structs are read word by
+// word.
+template <class T>
+PH_DEV T ldg(const T* p) {
+    if constexpr (std::is_arithmetic_v<T>) {
+        return *(const T __attribute__((address_space(1)))*)p;
+    } else {
+        static_assert(sizeof(T) % 4 == 0, "ldg reads whole words");
+        T v;
+        const uint32_t __attribute__((address_space(1)))* src = (const uint32_t __attribute__((address_space(1)))*)p;
+        uint32_t* dst = reinterpret_cast<uint32_t*>(&v);
+#pragma unroll
+        for (int i = 0; i < int(sizeof(T) / 4); i++) dst[i] = src[i];
+        return v;
+    }
+}
+
+PH_DEV float4 ldg(const float4* p) {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    const v4 v = *(const v4 __attribute__((address_space(1)))*)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
+// A pointer whose value is the same in every lane, moved to scalar registers.
+template <class T>
+PH_DEV const T* uniform_ptr(const T* p) {
+    const uint64_t v = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(v)), hi = __builtin_amdgcn_readfirstlane(uint32_t(v >> 32));
+    return reinterpret_cast<const T*>((uint64_t(hi) << 32) | lo);
+}
+
 
 constexpr float kPi = 3.14159265358979323846f;
 constexpr float kTwoPi = 6.28318530717958647692f;

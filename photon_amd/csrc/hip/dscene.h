@@ -51,7 +51,7 @@ struct Mat {
 
 PH_DEV Mat load_material(const DScene& s, int m) {
     const float4* p = s.materials + 5 * m;
-    float4 a = p[0], b = p[1], c = p[2], d = p[3], e = p[4];
+    float4 a = ldg(p), b = ldg(p + 1), c = ldg(p + 2), d = ldg(p + 3), e = ldg(p + 4);
     Mat r;
     r.base_color = a;
     r.emission = mk3(b.x, b.y, b.z);
@@ -77,21 +77,22 @@ struct Tri {
 
 PH_DEV Tri load_tri(const DScene& s, int i) {
     const float4* p = s.tris + 3 * i;
-    float4 a = p[0], b = p[1], c = p[2];
+    float4 a = ldg(p), b = ldg(p + 1), c = ldg(p + 2);
     return {xyz(a), xyz(b), xyz(c), __float_as_int(a.w), __float_as_int(b.w)};
 }
 
-PH_DEV f2 uv_at(const float* uvs, int i) { return {uvs[2 * i], uvs[2 * i + 1]}; }
-PH_DEV f3 n_at(const float* n, int i) { return {n[3 * i], n[3 * i + 1], n[3 * i + 2]}; }
+PH_DEV f2 uv_at(const float* uvs, int i) { return {ldg(uvs + 2 * i), ldg(uvs + 2 * i + 1)}; }
+PH_DEV f3 n_at(const float* n, int i) { return {ldg(n + 3 * i), ldg(n + 3 * i + 1), ldg(n + 3 * i + 2)}; }
 PH_DEV f4 t_at(const float* t, int i) {
-    float4 v = reinterpret_cast<const float4*>(t)[i];
+    float4 v = ldg(reinterpret_cast<const float4*>(t) + i);
     return {v.x, v.y, v.z, v.w};
 }
+PH_DEV int index_at(const int* idx, int i) { return ldg(idx + i); }
 
 // ---- textures (texture.h:9-44) ------------------------------------------------------------------------
 PH_DEV f4 texel(uint32_t p, bool srgb, const float* lut) {
     float r = float(p & 0xFF), g = float((p >> 8) & 0xFF), b = float((p >> 16) & 0xFF), a = float(p >> 24);
-    if (srgb) return {lut[p & 0xFF], lut[(p >> 8) & 0xFF], lut[(p >> 16) & 0xFF], a * (1.0f / 255.0f)};
+    if (srgb) return {ldg(lut + (p & 0xFF)), ldg(lut + ((p >> 8) & 0xFF)), ldg(lut + ((p >> 16) & 0xFF)), a * (1.0f / 255.0f)};
     return f4{r, g, b, a} * (1.0f / 255.0f);
 }
 
@@ -104,7 +105,7 @@ PH_DEV int wrap_index(int i, int n, bool repeat) {
 }
 
 PH_DEV f4 tex_bilinear(const DScene& s, int tex, f2 uv) {
-    const ph_texture_info t = s.textures[tex];
+    const ph_texture_info t = ldg(s.textures + tex);
     float x = uv.x * float(t.width) - 0.5f, y = uv.y * float(t.height) - 0.5f;
     float fx = floorf(x), fy = floorf(y);
     if (!(fabsf(fx) < 1e8f) || !(fabsf(fy) < 1e8f)) return {0, 0, 0, 1};
@@ -115,10 +116,10 @@ PH_DEV f4 tex_bilinear(const DScene& s, int tex, f2 uv) {
     int ya = wrap_index(y0, t.height, rep), yb = wrap_index(y0 + 1, t.height, rep);
     const uint32_t* base = s.texels + t.offset;
     bool srgb = t.srgb != 0;
-    f4 a = texel(base[ya * t.width + xa], srgb, s.srgb_lut);
-    f4 b = texel(base[ya * t.width + xb], srgb, s.srgb_lut);
-    f4 c = texel(base[yb * t.width + xa], srgb, s.srgb_lut);
-    f4 d = texel(base[yb * t.width + xb], srgb, s.srgb_lut);
+    f4 a = texel(ldg(base + ya * t.width + xa), srgb, s.srgb_lut);
+    f4 b = texel(ldg(base + ya * t.width + xb), srgb, s.srgb_lut);
+    f4 c = texel(ldg(base + yb * t.width + xa), srgb, s.srgb_lut);
+    f4 d = texel(ldg(base + yb * t.width + xb), srgb, s.srgb_lut);
     f4 top = a * (1 - tx) + b * tx;
     f4 bottom = c * (1 - tx) + d * tx;
     return top * (1 - ty) + bottom * ty;

@@ -236,9 +236,9 @@ PH_DEV int alias_pick(const ph_alias_entry* table, int n, float u, float* pmf) {
     int i = int(x);
     if (i >= n) i = n - 1;
     float frac = x - float(i);
-    const ph_alias_entry e = table[i];
+    const ph_alias_entry e = ldg(table + i);
     int chosen = frac < e.prob ? i : e.alias;
-    *pmf = table[chosen].pmf;
+    *pmf = ldg(&table[chosen].pmf);
     return chosen;
 }
 
@@ -264,7 +264,7 @@ PH_DEV int cdf_search(const float* cdf, int n, float u) {
     int lo = 0, hi = n;
     while (lo + 1 < hi) {
         int mid = (lo + hi) / 2;
-        if (cdf[mid] <= u)
+        if (ldg(cdf + mid) <= u)
             lo = mid;
         else
             hi = mid;
@@ -295,11 +295,11 @@ PH_DEV bool sky_sample(const DScene& s, f2 u, f3& dir, float& pdf) {
     }
     const int w = s.env_w, h = s.env_h;
     int y = cdf_search(s.env_marginal, h, u.y);
-    float c0 = s.env_marginal[y], c1 = s.env_marginal[y + 1];
+    float c0 = ldg(s.env_marginal + y), c1 = ldg(s.env_marginal + y + 1);
     float dv = c1 > c0 ? (u.y - c0) / (c1 - c0) : 0.5f;
     const float* row = s.env_conditional + size_t(y) * size_t(w + 1);
     int x = cdf_search(row, w, u.x);
-    float r0 = row[x], r1 = row[x + 1];
+    float r0 = ldg(row + x), r1 = ldg(row + x + 1);
     float du = r1 > r0 ? (u.x - r0) / (r1 - r0) : 0.5f;
     f2 uv = mk2((float(x) + du) / float(w), (float(y) + dv) / float(h));
     f3 d = latlong_to_dir(uv);
@@ -316,7 +316,7 @@ PH_DEV f3 tri_emission(const DScene& s, int tri, const Mat& m, float u, float v)
     f3 e = m.emission;
     if (m.emissive_tex >= 0 && s.uvs) {
         const int* idx = s.indices + 3 * tri;
-        f2 uv = uv_at(s.uvs, idx[0]) * (1 - u - v) + uv_at(s.uvs, idx[1]) * u + uv_at(s.uvs, idx[2]) * v;
+        f2 uv = uv_at(s.uvs, index_at(idx, 0)) * (1 - u - v) + uv_at(s.uvs, index_at(idx, 1)) * u + uv_at(s.uvs, index_at(idx, 2)) * v;
         f4 t = tex_bilinear(s, m.emissive_tex, uv);
         e = e * mk3(t.x, t.y, t.z);
     }
@@ -325,14 +325,14 @@ PH_DEV f3 tri_emission(const DScene& s, int tri, const Mat& m, float u, float v)
 
 PH_DEV float emitter_pdf(const DScene& s, int tri, f3 dir, float dist) {
     if (s.num_emissive == 0 || !s.tri_emissive) return 0;
-    int k = s.tri_emissive[tri];
+    int k = ldg(s.tri_emissive + tri);
     if (k < 0) return 0;
-    const ph_emissive_tri e = s.emissive[k];
+    const ph_emissive_tri e = ldg(s.emissive + k);
     Tri g = load_tri(s, tri);
     f3 n = unit(cross(g.e1, g.e2));
     float cos_l = fabsf(dot(n, dir));
     if (cos_l <= 0) return 0;
-    return s.p_emissive * s.alias[k].pmf * dist * dist / (e.area * cos_l);
+    return s.p_emissive * ldg(&s.alias[k].pmf) * dist * dist / (e.area * cos_l);
 }
 
 PH_DEV float sky_pdf(const DScene& s, f3 dir) { return s.p_env * sky_pdf_dir(s, dir); }
@@ -347,7 +347,7 @@ PH_DEV LightDraw draw_light(const DScene& s, f3 p, float u_group, f2 u, float u_
     if (u_group < s.p_emissive && s.num_emissive > 0) {
         float pmf;
         int k = alias_pick(s.alias, s.num_emissive, u_light, &pmf);
-        const ph_emissive_tri e = s.emissive[k];
+        const ph_emissive_tri e = ldg(s.emissive + k);
         Tri g = load_tri(s, e.tri);
         f2 b = tri_uniform(u);
         f3 q = g.v0 + g.e1 * b.x + g.e2 * b.y;
@@ -371,7 +371,7 @@ PH_DEV LightDraw draw_light(const DScene& s, f3 p, float u_group, f2 u, float u_
     if (u_group < s.p_emissive + s.p_punctual && s.num_punctual > 0) {
         int k = int(u_light * float(s.num_punctual));
         if (k >= s.num_punctual) k = s.num_punctual - 1;
-        const ph_punctual_light l = s.punctual[k];
+        const ph_punctual_light l = ldg(s.punctual + k);
         float pick = s.p_punctual / float(s.num_punctual);
         ls.delta = true;
         ls.pdf = pick;
@@ -441,7 +441,7 @@ PH_DEV Surface shade_point(const DScene& s, f3 ro, f3 rd, float t, int tri, floa
     it.p = ro + rd * t;
     f3 ng = unit(cross(g.e1, g.e2));
     const int* idx = s.indices + 3 * tri;
-    const int i0 = idx[0], i1 = idx[1], i2 = idx[2];
+    const int i0 = index_at(idx, 0), i1 = index_at(idx, 1), i2 = index_at(idx, 2);
     f3 ns = ng;
     if (s.normals) {
         f3 n = n_at(s.normals, i0) * w + n_at(s.normals, i1) * hu + n_at(s.normals, i2) * hv;
@@ -455,7 +455,7 @@ PH_DEV Surface shade_point(const DScene& s, f3 ro, f3 rd, float t, int tri, floa
         f3 tt = mk3(t4.x, t4.y, t4.z);
         if (len2(tt) > 1e-12f) {
             tt = unit(tt - ns * dot(ns, tt));
-            f3 bt = cross(ns, tt) * (s.tangents[4 * i0 + 3] < 0 ? -1.0f : 1.0f);
+            f3 bt = cross(ns, tt) * (ldg(s.tangents + 4 * i0 + 3) < 0 ? -1.0f : 1.0f);
             f4 tn = tex_bilinear(s, m.normal_tex, uv);
             f3 nt = mk3(tn.x * 2 - 1, tn.y * 2 - 1, tn.z * 2 - 1);
             nt.x *= m.normal_scale;

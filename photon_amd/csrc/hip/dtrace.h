@@ -84,91 +84,119 @@ PH_DEV f3 guarded_reciprocal(f3 d) {
 // True when the masked triangle lets the ray through at (u, v) (traverse.h:55-64).
 PH_DEV bool alpha_cut(const DScene& s, int tri, int material, float u, float v) {
     const float4* mp = s.materials + 5 * material;
-    float alpha = mp[0].w;
-    float4 d = mp[3];
+    float alpha = ldg(mp).w;
+    float4 d = ldg(mp + 3);
     float cutoff = d.x;
     int tex = __float_as_int(d.w);
     if (tex >= 0 && s.uvs) {
         const int* idx = s.indices + 3 * tri;
-        f2 uv = uv_at(s.uvs, idx[0]) * (1 - u - v) + uv_at(s.uvs, idx[1]) * u + uv_at(s.uvs, idx[2]) * v;
+        f2 uv = uv_at(s.uvs, index_at(idx, 0)) * (1 - u - v) + uv_at(s.uvs, index_at(idx, 1)) * u + uv_at(s.uvs, index_at(idx, 2)) * v;
         alpha *= tex_bilinear(s, tex, uv).w;
     }
     return alpha < cutoff;
 }
 
+// What traversal reads of a scene, held in registers: the node and triangle arrays, and the whole
+// scene for the (rare) alpha tests.
+struct TraceScene {
+    const float4* nodes;
+    const float4* tris;
+    const DScene* full;
+};
+
+PH_DEV TraceScene trace_scene(const DScene& s) { return TraceScene{s.nodes, s.tris, &s}; }
+
+// One ray's traversal position: walk_step visits one node at a time, so that a lane can start a new
+// ray as soon as its own finishes (the persistent kernels refill lanes between steps).
+struct Walker {
+    f3 o, d, inv;
+    float tmax;
+    int node, sp;
+    bool found;
+};
+
+PH_DEV Walker walker(f3 o, f3 d, float tmax) { return Walker{o, d, guarded_reciprocal(d), tmax, 0, 0, false}; }
+
+// Visits the node at w.node: tests both child boxes against the current tmax, intersects leaf
+// children in place, then moves to the nearer interior child (pushing the farther) or pops the stack.
+// Returns false once the traversal is over (any-hit: at the first accepted triangle).
+template <bool AnyHit, bool Count, class Stack>
+PH_DEV bool walk_step(const TraceScene& ts, Walker& w, HitRec& hit, Stack& stack, TraceCount* cnt) {
+    const float4* np = ts.nodes + 4 * w.node;
+    const float4 lmin = ldg(np), lmax = ldg(np + 1), rmin = ldg(np + 2), rmax = ldg(np + 3);
+    if (Count) cnt->nodes++;
+    const float tl = slab(lmin, lmax, w.o, w.inv, w.tmax);
+    const float tr = slab(rmin, rmax, w.o, w.inv, w.tmax);
+    int next0 = 0, next1 = 0;
+    float tn0 = 0, tn1 = 0;
+    int nn = 0;
+#pragma unroll
+    for (int side = 0; side < 2; side++) {
+        const float tc = side == 0 ? tl : tr;
+        const int count = __float_as_int(side == 0 ? lmax.w : rmax.w);
+        const int idx = __float_as_int(side == 0 ? lmin.w : rmin.w);
+        if (tc == kInf) continue;
+        if (count > 0) {
+            for (int i = idx; i < idx + count; i++) {
+                const float4* tp = ts.tris + 3 * i;
+                const float4 a = ldg(tp), b = ldg(tp + 1), c = ldg(tp + 2);
+                float t, u, v;
+                if (Count) cnt->tris++;
+                if (mt_hit(xyz(a), xyz(b), xyz(c), w.o, w.d, w.tmax, t, u, v)) {
+                    if (__float_as_int(b.w) & PH_TRI_ALPHA_TESTED) {
+                        if (Count) cnt->alpha++;
+                        if (alpha_cut(*ts.full, i, __float_as_int(a.w), u, v)) continue;
+                    }
+                    hit.t = t;
+                    hit.tri = i;
+                    w.found = true;
+                    if (AnyHit) return false;
+                    w.tmax = t;
+                    hit.u = u;
+                    hit.v = v;
+                }
+            }
+        } else if (idx >= 0) {
+            if (nn == 0) {
+                next0 = idx;
+                tn0 = tc;
+            } else {
+                next1 = idx;
+                tn1 = tc;
+            }
+            nn++;
+        }
+    }
+    if (nn == 2) {
+        if (tn0 > tn1) {
+            int a = next0;
+            next0 = next1;
+            next1 = a;
+            float b = tn0;
+            tn0 = tn1;
+            tn1 = b;
+        }
+        if (tn1 < w.tmax && w.sp < kMaxStack) stack.store(w.sp++, next1);
+        if (tn0 < w.tmax) {
+            w.node = next0;
+            return true;
+        }
+    } else if (nn == 1 && tn0 < w.tmax) {
+        w.node = next0;
+        return true;
+    }
+    if (w.sp == 0) return false;
+    w.node = stack.load(--w.sp);
+    return true;
+}
+
 template <bool AnyHit, bool Count, class Stack>
 PH_DEV bool traverse(const DScene& s, f3 o, f3 d, float tmax, HitRec& hit, Stack& stack, TraceCount* cnt) {
-    const f3 inv = guarded_reciprocal(d);
-    int sp = 0, node = 0;
-    bool found = false;
-    for (;;) {
-        const float4* np = s.nodes + 4 * node;
-        const float4 lmin = np[0], lmax = np[1], rmin = np[2], rmax = np[3];
-        if (Count) cnt->nodes++;
-        const float tl = slab(lmin, lmax, o, inv, tmax);
-        const float tr = slab(rmin, rmax, o, inv, tmax);
-        int next0 = 0, next1 = 0;
-        float tn0 = 0, tn1 = 0;
-        int nn = 0;
-#pragma unroll
-        for (int side = 0; side < 2; side++) {
-            const float tc = side == 0 ? tl : tr;
-            const int count = __float_as_int(side == 0 ? lmax.w : rmax.w);
-            const int idx = __float_as_int(side == 0 ? lmin.w : rmin.w);
-            if (tc == kInf) continue;
-            if (count > 0) {
-                for (int i = idx; i < idx + count; i++) {
-                    const float4* tp = s.tris + 3 * i;
-                    const float4 a = tp[0], b = tp[1], c = tp[2];
-                    float t, u, v;
-                    if (Count) cnt->tris++;
-                    if (mt_hit(xyz(a), xyz(b), xyz(c), o, d, tmax, t, u, v)) {
-                        if (__float_as_int(b.w) & PH_TRI_ALPHA_TESTED) {
-                            if (Count) cnt->alpha++;
-                            if (alpha_cut(s, i, __float_as_int(a.w), u, v)) continue;
-                        }
-                        hit.t = t;
-                        hit.tri = i;
-                        if (AnyHit) return true;
-                        tmax = t;
-                        hit.u = u;
-                        hit.v = v;
-                        found = true;
-                    }
-                }
-            } else if (idx >= 0) {
-                if (nn == 0) {
-                    next0 = idx;
-                    tn0 = tc;
-                } else {
-                    next1 = idx;
-                    tn1 = tc;
-                }
-                nn++;
-            }
-        }
-        if (nn == 2) {
-            if (tn0 > tn1) {
-                int a = next0;
-                next0 = next1;
-                next1 = a;
-                float b = tn0;
-                tn0 = tn1;
-                tn1 = b;
-            }
-            if (tn1 < tmax && sp < kMaxStack) stack.store(sp++, next1);
-            if (tn0 < tmax) {
-                node = next0;
-                continue;
-            }
-        } else if (nn == 1 && tn0 < tmax) {
-            node = next0;
-            continue;
-        }
-        if (sp == 0) break;
-        node = stack.load(--sp);
+    const TraceScene ts = trace_scene(s);
+    Walker w = walker(o, d, tmax);
+    while (walk_step<AnyHit, Count>(ts, w, hit, stack, cnt)) {
     }
-    return found;
+    return w.found;
 }
 
 }  // namespace phd

@@ -60,7 +60,6 @@ struct ph_server {
         if (cfg.memory_budget <= 0) cfg.memory_budget = int64_t(16) << 30;
         if (cfg.path_slots <= 0) cfg.path_slots = 1 << 21;
         if (cfg.iterations_per_step <= 0) cfg.iterations_per_step = 8;
-        cfg.iterations_per_step += cfg.iterations_per_step & 1;
         if (cfg.max_jobs <= 0 || cfg.max_jobs > WavefrontEngine::kMaxJobs) cfg.max_jobs = WavefrontEngine::kMaxJobs;
         if (cfg.pass_samples <= 0) cfg.pass_samples = int64_t(4) * cfg.path_slots;
         PH_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));

@@ -8,20 +8,28 @@
 // Passes are queued in order (chunks), and a job's next pass is queued only once its previous pass
 // has finished, so the samples of a pixel still run in order while large jobs yield to new ones.
 //
+// Every slot carries a one-byte stage (waiting to be traced, missed, hit a material class, finished,
+// idle) plus a pending-shadow-ray bit. Queues are not appended to: after each stage an ordered
+// compaction (a count pass and a scatter pass over the stage bytes) lists the slots of each next stage
+// in slot order. Slots are handed their pixels in 8x8 tiles, so the 64 lanes of a wave work on one
+// tile: camera rays stay coherent, and the per-slot state is read and written in long runs.
+//
 // One bounce of every path in flight is one iteration of these stages:
-//   bookkeep  fold ray counts, clear the per-iteration queue counters and cursors (one wave)
-//   wake      idle slots take newly admitted work (serving only)
-//   extend    closest-hit traversal of the extension queue; hits are binned by material class
+//   bookkeep  fold ray counts (one wave)
+//   extend    closest-hit traversal of the extension queue; the stage records miss or material class
+//   compact   -> miss queue and one queue per material class
 //   miss      environment contribution of escaped paths
 //   shade x4  surface interaction, NEE light sample, BSDF sample, Russian roulette; one launch per
 //             material class (metal / opaque / glass / mixed) so a wave runs one lobe structure
+//   compact   -> shadow queue and finished-path queue (serving: also idle slots while work waits)
 //   shadow    any-hit traversal of the NEE queue; unoccluded contributions are added to L
 //   regen     finished paths: add L to the pixel, start the next sample or the next work item
-// Traversal, shading and regeneration kernels are persistent: a fixed grid whose waves pull batches of
-// 64 queue entries from a cursor until the queue is drained, so uneven rays never leave a wave idle.
-// Queue appends are wave-aggregated (one atomic per wave). The bounce loop can be captured once into a
-// HIP graph of G iterations and replayed; kernels read their sizes from device memory, so the graph
-// stays valid while jobs come and go.
+//   compact   -> extension queue of the next iteration
+// Traversal, shading and regeneration kernels are persistent: a fixed grid whose waves take the
+// queue's 64-entry batches in turn (no atomics: atomics on one address serialize). Traversal lanes are
+// refilled one ray at a time from the wave's batches, so a wave is not held by its longest ray.
+// The bounce loop can be captured once into a HIP graph of G iterations and replayed; kernels read
+// their sizes from device memory, so the graph stays valid while jobs come and go.
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -37,24 +45,25 @@ namespace phk {
 
 namespace {
 
+// Queue lengths (written by the compactions; each set is contiguous) and work counters.
 enum : int {
-    C_EXT0 = 0,  // extension queue counts, by iteration parity
-    C_EXT1 = 1,
-    C_SHADOW = 2,
-    C_TERM = 3,
-    C_MISS = 4,
-    C_CLASS = 5,  // 4 counters: metal, opaque, glass, mixed
-    C_IDLE0 = 9,  // idle-slot queue counts, by iteration parity
-    C_IDLE1 = 10,
-    C_WORK = 11,    // next work item to hand out
-    C_TOTAL = 12,   // work items admitted so far
-    C_CHUNKS = 13,  // entries of the chunk table
-    C_CUR = 16,     // cursors: +0 extend, +1 miss, +2..5 shade, +6 shadow, +7 regen, +8 wake
-    C_COUNT = 32
+    C_EXT = 0,     // extension queue
+    C_SHADOW = 1,  // shadow queue
+    C_TERM = 2,    // finished paths (serving: and idle slots while work waits)
+    C_MISS = 3,
+    C_CLASS = 4,    // 4 queues: metal, opaque, glass, mixed
+    C_CHUNKS = 10,  // entries of the chunk table
+    C_COUNT = 16
 };
+
+// Slot stages (low 7 bits) and the pending-shadow-ray bit.
+enum : uint8_t { S_IDLE = 0, S_EXT = 1, S_MISS = 2, S_CLASS = 3, S_TERM = 7, S_SHADOW = 0x80, S_MASK = 0x7F };
 
 constexpr int kClasses = 4;
 constexpr int kMaxChunks = 1 << 16;
+// Ordered compaction: a block of 256 threads covers 4096 slots, 16 consecutive slots per thread.
+constexpr int kCompactThreads = 256, kCompactPerThread = 16, kCompactChunk = kCompactThreads * kCompactPerThread;
+constexpr int kMaxCategories = 5;
 
 struct JobParams {
     DScene scene;
@@ -73,6 +82,7 @@ struct State {
     int* pixel;
     int* sample;
     int* sample_end;  // end of the slot's pass (serving)
+    uint8_t* stage;
     uint64_t* rng;
     float4* ro;  // origin, prev_pdf
     float4* rd;  // direction, flags (depth | prev_delta << 16)
@@ -82,8 +92,7 @@ struct State {
     float4* so;  // shadow origin, tmax
     float4* sd;  // shadow direction
     float4* sc;  // shadow contribution
-    int* q[2];
-    int* q_idle[2];
+    int* q_ext;
     int* q_shadow;
     int* q_term;
     int* q_miss;
@@ -91,10 +100,13 @@ struct State {
 };
 
 // Work items in queue order: chunk c holds items [chunk_first[c], ...), the pixels of one pass,
-// described by chunk_pass[c] = (table entry, first sample, end sample).
+// described by chunk_pass[c] = (table entry, first sample, end sample). Item indices are 64-bit: a
+// server hands out items for as long as it runs. items[0] is the next item to hand out, items[1] the
+// number queued so far.
 struct Work {
     const JobParams* jobs;
-    const int* chunk_first;
+    unsigned long long* items;
+    const long long* chunk_first;
     const int4* chunk_pass;
     int* job_done;  // pixel passes finished, per table entry
 };
@@ -120,36 +132,26 @@ PH_DEV void flush_counts(const TraceCount& c, TraceTotals* totals) {
     }
 }
 
-// Appends `value` for every lane with `pred` using one atomic per wave. All lanes of the wave must call.
-PH_DEV void wave_append(int* queue, int* count, bool pred, int value) {
-    const uint64_t mask = __ballot(pred);
-    if (mask == 0) return;
-    const int lane = lane_id();
-    const int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(count, __popcll(mask));
-    base = __shfl(base, leader);
-    if (pred) queue[base + __popcll(mask & ((1ull << lane) - 1ull))] = value;
-}
-
 // Gives every lane with `pred` the next work item index (one atomic per wave). All lanes must call.
-PH_DEV int wave_take(int* counter, bool pred) {
+PH_DEV long long wave_take(unsigned long long* counter, bool pred) {
     const uint64_t mask = __ballot(pred);
     if (mask == 0) return 0;
     const int lane = lane_id();
     const int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
-    int first = 0;
-    if (lane == leader) first = atomicAdd(counter, __popcll(mask));
+    unsigned long long first = 0;
+    if (lane == leader) first = atomicAdd(counter, (unsigned long long)__popcll(mask));
     first = __shfl(first, leader);
-    return first + __popcll(mask & ((1ull << lane) - 1ull));
+    return (long long)first + __popcll(mask & ((1ull << lane) - 1ull));
 }
 
-// Persistent work distribution: a wave takes 64 consecutive queue entries per round.
-PH_DEV int wave_fetch(int* cursor) {
-    int base = 0;
-    if (lane_id() == 0) base = atomicAdd(cursor, 64);
-    return __shfl(base, 0);
+PH_DEV long long items_next(const Work& work) {
+    return (long long)__hip_atomic_load(work.items, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+
+// Persistent work distribution without atomics (atomics on one address serialize): the grid's waves
+// take the queue's 64-entry batches in turn, wave w taking batches w, w + W, w + 2W, ...
+PH_DEV int grid_wave() { return blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); }
+PH_DEV int grid_waves() { return gridDim.x * (blockDim.x >> 6); }
 
 // Work item -> pixel of a w x h image: 8-row bands, 8-column tiles inside a band (the last tile and band
 // may be narrower), row-major inside a tile, so a wave's 64 lanes start on one 8x8 pixel block.
@@ -166,7 +168,7 @@ PH_DEV int work_pixel(int w, int width, int height) {
 
 // Table entry owning work item `w` (the last chunk that starts at or before it).
 template <bool Multi>
-PH_DEV int chunk_of(const Work& work, int chunks, int w) {
+PH_DEV int chunk_of(const Work& work, int chunks, long long w) {
     if (!Multi) return 0;
     int lo = 0, hi = chunks;
     while (lo + 1 < hi) {
@@ -179,9 +181,24 @@ PH_DEV int chunk_of(const Work& work, int chunks, int w) {
     return lo;
 }
 
+// A kernel's copy of the single job (all lanes share it; unused when serving). The job table is written
+// only by the host between launches, so it is read through the constant address space: scalar loads,
+// once per kernel.
 template <bool Multi>
-PH_DEV const JobParams& job_of(const Work& work, const State& st, int slot) {
-    return Multi ? work.jobs[st.job[slot]] : work.jobs[0];
+PH_DEV JobParams single_job(const Work& work) {
+    JobParams j{};
+    if (!Multi) {
+        const uint32_t __attribute__((address_space(4)))* src = (const uint32_t __attribute__((address_space(4)))*)work.jobs;
+        uint32_t* dst = reinterpret_cast<uint32_t*>(&j);
+#pragma unroll
+        for (int i = 0; i < int(sizeof(JobParams) / 4); i++) dst[i] = src[i];
+    }
+    return j;
+}
+
+template <bool Multi>
+PH_DEV const JobParams& job_of(const Work& work, const State& st, int slot, const JobParams& single) {
+    return Multi ? work.jobs[st.job[slot]] : single;
 }
 
 PH_DEV void start_sample(const JobParams& jp, const State& st, int slot, int pixel, int k) {
@@ -202,11 +219,11 @@ PH_DEV void start_sample(const JobParams& jp, const State& st, int slot, int pix
 
 // Starts work item `w` on `slot`: finds its job and pixel, records them, seeds the first sample.
 template <bool Multi>
-PH_DEV void start_work(const Work& work, const State& st, int chunks, int slot, int w) {
+PH_DEV void start_work(const Work& work, const State& st, int chunks, int slot, long long w, const JobParams& single) {
     const int c = chunk_of<Multi>(work, chunks, w);
     const int4 pass = Multi ? work.chunk_pass[c] : make_int4(0, 0, 0, 0);
-    const JobParams& jp = work.jobs[pass.x];
-    const int local = w - (Multi ? work.chunk_first[c] : 0);
+    const JobParams& jp = Multi ? work.jobs[pass.x] : single;
+    const int local = int(w - (Multi ? work.chunk_first[c] : 0));
     if (Multi) {
         st.job[slot] = pass.x;
         st.sample_end[slot] = pass.z;
@@ -217,123 +234,260 @@ PH_DEV void start_work(const Work& work, const State& st, int chunks, int slot, 
 PH_DEV Pcg slot_rng(const State& st, int slot, int pixel) { return Pcg{st.rng[slot], (uint64_t(uint32_t(pixel)) << 1u) | 1u}; }
 
 // Single job: slot s starts work item s.
-__global__ void init_kernel(Work work, State st, int* counters, int slots) {
+__global__ void init_kernel(Work work, State st, int slots) {
     const int slot = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool has = slot < slots && slot < counters[C_TOTAL];
-    if (has) start_work<false>(work, st, 1, slot, slot);
-    wave_append(st.q[0], counters + C_EXT0, has, slot);
+    if (slot >= slots) return;
+    const bool has = slot < (long long)work.items[1];
+    if (has) start_work<false>(work, st, 1, slot, slot, work.jobs[0]);
+    st.stage[slot] = has ? S_EXT : S_IDLE;
 }
 
 // Serving: every slot starts idle.
-__global__ void init_idle_kernel(State st, int* counters, int slots) {
+__global__ void init_idle_kernel(State st, int slots) {
     const int slot = blockIdx.x * blockDim.x + threadIdx.x;
-    wave_append(st.q_idle[0], counters + C_IDLE0, slot < slots, slot);
+    if (slot < slots) st.stage[slot] = S_IDLE;
 }
 
-// Starts iteration `par`: folds the ray counts (this iteration's extension queue, the previous
-// iteration's shadow queue) into the 64-bit totals, then clears the per-iteration counters.
-__global__ void bookkeep_kernel(int* counters, int par, unsigned long long* rays) {
-    const int t = threadIdx.x;
-    if (t == 0) {
-        rays[0] += (unsigned long long)counters[C_EXT0 + par];
+// Starts an iteration: folds the ray counts (this iteration's extension queue, the previous iteration's
+// shadow queue) into the 64-bit totals.
+__global__ void bookkeep_kernel(const int* counters, unsigned long long* rays) {
+    if (threadIdx.x == 0) {
+        rays[0] += (unsigned long long)counters[C_EXT];
         rays[1] += (unsigned long long)counters[C_SHADOW];
     }
-    __syncthreads();
-    if (t == 0) counters[C_EXT0 + (1 - par)] = 0;
-    if (t == 1) counters[C_IDLE0 + (1 - par)] = 0;
-    if (t >= C_SHADOW && t < C_CLASS + kClasses) counters[t] = 0;
-    if (t >= C_CUR && t < C_CUR + 9) counters[t] = 0;
 }
 
-// Idle slots take newly admitted work items; the rest stay idle.
-__global__ __launch_bounds__(256) void wake_kernel(Work work, State st, int* counters, int par) {
-    const int n = counters[C_IDLE0 + par];
-    const int total = counters[C_TOTAL], chunks = counters[C_CHUNKS];
+// ---- ordered compaction -----------------------------------------------------------------------------
+enum { K_EXT = 0, K_HIT = 1, K_DONE = 2 };
+
+template <int Kind>
+struct Compact {
+    static constexpr int cats = Kind == K_EXT ? 1 : Kind == K_HIT ? 5 : 2;
+    static constexpr int counter = Kind == K_EXT ? C_EXT : Kind == K_HIT ? C_MISS : C_SHADOW;
+    // Category bits of a stage byte: extension; miss + 4 classes; shadow + finished (or idle with work).
+    static PH_DEV unsigned bits(uint8_t s, bool idle_takes_work) {
+        const int low = s & S_MASK;
+        if (Kind == K_EXT) return low == S_EXT ? 1u : 0u;
+        if (Kind == K_HIT) return (low >= S_MISS && low < S_CLASS + kClasses) ? 1u << (low - S_MISS) : 0u;
+        return ((s & S_SHADOW) ? 1u : 0u) | ((low == S_TERM || (idle_takes_work && low == S_IDLE)) ? 2u : 0u);
+    }
+};
+
+struct QueueOut {
+    int* q[kMaxCategories];
+};
+
+// The 16 stage bytes of one thread (zero past the end: no category, since idle slots only count when
+// the caller asks for them and the padding is never idle).
+PH_DEV void load_stages(const uint8_t* stage, int first, int n, uint8_t* b, bool& valid_all) {
+    valid_all = first + kCompactPerThread <= n;
+    if (valid_all) {
+        const uint4 v = *reinterpret_cast<const uint4*>(stage + first);
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+        for (int j = 0; j < 16; j++) b[j] = uint8_t(w[j >> 2] >> (8 * (j & 3)));
+    } else {
+        for (int j = 0; j < 16; j++) b[j] = first + j < n ? stage[first + j] : uint8_t(0);
+    }
+}
+
+template <int Kind, bool Multi>
+PH_DEV bool idle_takes_work(const unsigned long long* items) {
+    return Multi && Kind == K_DONE && items[0] < items[1];
+}
+
+// Sum over the block of one value per thread (all threads get it).
+PH_DEV int block_sum(int v, int* red) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    const int wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane_id() == 0) red[wave] = v;
+    __syncthreads();
+    int t = 0;
+    for (int w = 0; w < kCompactThreads / 64; w++) t += red[w];
+    return t;
+}
+
+template <int Kind, bool Multi>
+__global__ __launch_bounds__(kCompactThreads) void compact_count_kernel(const uint8_t* stage, int n, const unsigned long long* items, int* partial,
+                                                                       int nchunks) {
+    constexpr int K = Compact<Kind>::cats;
+    __shared__ int red[kCompactThreads / 64];
+    const bool idle = idle_takes_work<Kind, Multi>(items);
+    const int first = blockIdx.x * kCompactChunk + threadIdx.x * kCompactPerThread;
+    uint8_t b[16];
+    bool full;
+    load_stages(stage, first, n, b, full);
+    int cnt[K] = {};
+    for (int j = 0; j < 16; j++) {
+        const unsigned m = (first + j < n) ? Compact<Kind>::bits(b[j], idle) : 0u;
+        for (int k = 0; k < K; k++) cnt[k] += (m >> k) & 1u;
+    }
+    for (int k = 0; k < K; k++) {
+        const int t = block_sum(cnt[k], red);
+        if (threadIdx.x == 0) partial[k * nchunks + blockIdx.x] = t;
+    }
+}
+
+// Writes the slots of each category in slot order: chunk offset (sum of the earlier chunks' counts),
+// then the exclusive scan of the per-thread counts inside the chunk. The last chunk writes the totals.
+template <int Kind, bool Multi>
+__global__ __launch_bounds__(kCompactThreads) void compact_scatter_kernel(const uint8_t* stage, int n, int* counters, const unsigned long long* items,
+                                                                         const int* partial, int nchunks, QueueOut out) {
+    constexpr int K = Compact<Kind>::cats;
+    constexpr int W = kCompactThreads / 64;
+    __shared__ int red[W];
+    __shared__ int wave_tot[K][W];
+    const bool idle = idle_takes_work<Kind, Multi>(items);
+    const int c = blockIdx.x;
+    const int first = c * kCompactChunk + threadIdx.x * kCompactPerThread;
+    uint8_t b[16];
+    bool full;
+    load_stages(stage, first, n, b, full);
+    uint8_t m[16];
+    int cnt[K] = {};
+    for (int j = 0; j < 16; j++) {
+        m[j] = uint8_t((first + j < n) ? Compact<Kind>::bits(b[j], idle) : 0u);
+        for (int k = 0; k < K; k++) cnt[k] += (m[j] >> k) & 1;
+    }
+    const int lane = lane_id(), wave = threadIdx.x >> 6;
+    int pos[K], chunk_tot[K];
+    for (int k = 0; k < K; k++) {
+        int before = 0;
+        for (int i = threadIdx.x; i < c; i += kCompactThreads) before += partial[k * nchunks + i];
+        const int base = block_sum(before, red);
+        // Inclusive scan over the wave, then over the waves of the block.
+        int v = cnt[k];
+        for (int off = 1; off < 64; off <<= 1) {
+            const int u = __shfl_up(v, off);
+            if (lane >= off) v += u;
+        }
+        if (lane == 63) wave_tot[k][wave] = v;
+        __syncthreads();
+        int prev = 0, tot = 0;
+        for (int w = 0; w < W; w++) {
+            if (w < wave) prev += wave_tot[k][w];
+            tot += wave_tot[k][w];
+        }
+        pos[k] = base + prev + v - cnt[k];
+        chunk_tot[k] = base + tot;
+    }
+    for (int j = 0; j < 16; j++)
+        for (int k = 0; k < K; k++)
+            if ((m[j] >> k) & 1) out.q[k][pos[k]++] = first + j;
+    if (c == nchunks - 1 && threadIdx.x == 0)
+        for (int k = 0; k < K; k++) counters[Compact<Kind>::counter + k] = chunk_tot[k];
+}
+
+// Persistent traversal of a queue with lane refill. A lane whose ray is finished takes the next queue
+// entry as soon as at least `refill` lanes of its wave are free (or all of them are), with one atomic
+// per refill, so a wave is not held by its slowest ray. The wave leaves when the queue is drained and
+// none of its lanes has a ray. Each ray is walked exactly as traverse() walks it; only the lane it runs
+// on and the moment it starts change.
+template <bool AnyHit, bool Count, bool Multi, class Stack, class Load, class Finish>
+PH_DEV void trace_queue(const Work& work, const State& st, const int* queue, int n, int refill, Stack& stack, TraceCount& cnt, Load load,
+                        Finish finish) {
+    // This wave's entries, in order: batches wave, wave + W, ... of 64 entries; `pos` counts the
+    // entries already handed to lanes.
+    const int wave = grid_wave(), W = grid_waves();
+    int pos = 0;
+    bool live = false, drained = false;
+    int slot = 0, entry = 0;
+    Walker w{};
+    HitRec hit{};
+    TraceScene ts{};
+    if (!Multi) {
+        // One job: its traversal arrays are the same for every lane; keep them in scalar registers.
+        const JobParams j0 = single_job<false>(work);
+        ts = TraceScene{uniform_ptr(j0.scene.nodes), uniform_ptr(j0.scene.tris), &work.jobs[0].scene};
+    }
     for (;;) {
-        const int base = wave_fetch(counters + C_CUR + 8);
-        if (base >= n) break;
-        const int i = base + lane_id();
-        const bool active = i < n;
-        const int slot = active ? st.q_idle[par][i] : 0;
-        // Stop taking items once they run out (every lane past `total` stays idle).
-        const bool want = active && counters[C_WORK] < total;
-        const int w = wave_take(counters + C_WORK, want);
-        const bool got = want && w < total;
-        if (got) start_work<true>(work, st, chunks, slot, w);
-        wave_append(st.q[par], counters + C_EXT0 + par, got, slot);
-        wave_append(st.q_idle[1 - par], counters + C_IDLE0 + (1 - par), active && !got, slot);
+        if (!drained) {
+            const uint64_t need = __ballot(!live);
+            const int k = __popcll(need);
+            if (k >= refill || k == 64) {
+                const int lane = lane_id();
+                if (!live) {
+                    const int j = pos + __popcll(need & ((1ull << lane) - 1ull));
+                    const int i = (wave + (j >> 6) * W) * 64 + (j & 63);
+                    if (i < n) {
+                        slot = queue[i];
+                        if (Multi) {
+                            entry = st.job[slot];
+                            ts = trace_scene(work.jobs[entry].scene);
+                        }
+                        load(slot, w, hit);
+                        live = true;
+                    }
+                }
+                pos += k;
+                if ((wave + (pos >> 6) * W) * 64 >= n) drained = true;
+            }
+        }
+        if (__ballot(live) == 0) break;
+        if (live && !walk_step<AnyHit, Count>(ts, w, hit, stack, &cnt)) {
+            finish(slot, entry, ts, w, hit);
+            live = false;
+        }
     }
 }
 
 template <int Block, int Depth, bool Count, bool Multi>
-__global__ __launch_bounds__(Block) void extend_kernel(Work work, State st, int* counters, int par, TraceTotals* totals) {
+__global__ __launch_bounds__(Block) void extend_kernel(Work work, State st, int* counters, int sort, int refill, TraceTotals* totals) {
     __shared__ int lds[Depth * Block];
     LaneStack<Depth> stack{lds + threadIdx.x, Block};
-    const int n = counters[C_EXT0 + par];
-    const int* queue = st.q[par];
     TraceCount cnt{0, 0, 0};
-    for (;;) {
-        const int base = wave_fetch(counters + C_CUR + 0);
-        if (base >= n) break;
-        const int i = base + lane_id();
-        const bool active = i < n;
-        int slot = 0, cls = -1;
-        if (active) {
-            slot = queue[i];
-            const JobParams& jp = job_of<Multi>(work, st, slot);
-            const DScene& s = jp.scene;
+    trace_queue<false, Count, Multi>(
+        work, st, st.q_ext, counters[C_EXT], refill, stack, cnt,
+        [&](int slot, Walker& w, HitRec& hit) {
             const float4 o4 = st.ro[slot], d4 = st.rd[slot];
-            HitRec hit{kInf, -1, 0.0f, 0.0f};
-            traverse<false, Count>(s, xyz(o4), xyz(d4), kInf, hit, stack, &cnt);
+            w = walker(xyz(o4), xyz(d4), kInf);
+            hit = HitRec{kInf, -1, 0.0f, 0.0f};
+        },
+        [&](int slot, int entry, const TraceScene& ts, const Walker&, const HitRec& hit) {
             st.hit[slot] = make_float4(hit.t, __int_as_float(hit.tri), hit.u, hit.v);
-            if (hit.tri >= 0) cls = jp.mat_class[__float_as_int(s.tris[3 * hit.tri].w)];
-        }
-        wave_append(st.q_miss, counters + C_MISS, active && cls < 0, slot);
-        for (int c = 0; c < kClasses; c++) wave_append(st.q_class[c], counters + C_CLASS + c, active && cls == c, slot);
-    }
+            uint8_t next = S_MISS;
+            if (hit.tri >= 0) {
+                const int* cls = work.jobs[entry].mat_class;
+                next = uint8_t(S_CLASS + (sort ? ldg(cls + __float_as_int(ldg(ts.tris + 3 * hit.tri).w)) : kClasses - 1));
+            }
+            st.stage[slot] = next;
+        });
     if (Count) flush_counts(cnt, totals);
 }
 
 template <int Block, int Depth, bool Count, bool Multi>
-__global__ __launch_bounds__(Block) void shadow_kernel(Work work, State st, int* counters, TraceTotals* totals) {
+__global__ __launch_bounds__(Block) void shadow_kernel(Work work, State st, int* counters, int refill, TraceTotals* totals) {
     __shared__ int lds[Depth * Block];
     LaneStack<Depth> stack{lds + threadIdx.x, Block};
-    const int n = counters[C_SHADOW];
     TraceCount cnt{0, 0, 0};
-    for (;;) {
-        const int base = wave_fetch(counters + C_CUR + 6);
-        if (base >= n) break;
-        const int i = base + lane_id();
-        if (i < n) {
-            const int slot = st.q_shadow[i];
-            const DScene& s = job_of<Multi>(work, st, slot).scene;
+    trace_queue<true, Count, Multi>(
+        work, st, st.q_shadow, counters[C_SHADOW], refill, stack, cnt,
+        [&](int slot, Walker& w, HitRec& hit) {
             const float4 o4 = st.so[slot], d4 = st.sd[slot];
-            HitRec hit{kInf, -1, 0.0f, 0.0f};
-            if (!traverse<true, Count>(s, xyz(o4), xyz(d4), o4.w, hit, stack, &cnt)) {
-                const float4 c = st.sc[slot];
-                float4 L = st.L[slot];
-                L.x += c.x;
-                L.y += c.y;
-                L.z += c.z;
-                st.L[slot] = L;
-            }
-        }
-    }
+            w = walker(xyz(o4), xyz(d4), o4.w);
+            hit = HitRec{kInf, -1, 0.0f, 0.0f};
+        },
+        [&](int slot, int, const TraceScene&, const Walker& w, const HitRec&) {
+            if (w.found) return;
+            const float4 c = st.sc[slot];
+            float4 L = st.L[slot];
+            L.x += c.x;
+            L.y += c.y;
+            L.z += c.z;
+            st.L[slot] = L;
+        });
     if (Count) flush_counts(cnt, totals);
 }
 
 template <bool Multi>
 __global__ __launch_bounds__(256) void miss_kernel(Work work, State st, int* counters) {
     const int n = counters[C_MISS];
-    for (;;) {
-        const int base = wave_fetch(counters + C_CUR + 1);
-        if (base >= n) break;
+    const JobParams single = single_job<Multi>(work);
+    for (int base = grid_wave() * 64; base < n; base += grid_waves() * 64) {
         const int i = base + lane_id();
-        const bool active = i < n;
-        int slot = 0;
-        if (active) {
-            slot = st.q_miss[i];
-            const JobParams& jp = job_of<Multi>(work, st, slot);
+        if (i < n) {
+            const int slot = st.q_miss[i];
+            const JobParams& jp = job_of<Multi>(work, st, slot, single);
             const DScene& s = jp.scene;
             const float4 o4 = st.ro[slot], d4 = st.rd[slot];
             const f3 rd = xyz(d4);
@@ -351,26 +505,23 @@ __global__ __launch_bounds__(256) void miss_kernel(Work work, State st, int* cou
                 jp.albedo[p] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
                 jp.nd[p] = make_float4(-rd.x, -rd.y, -rd.z, 0.0f);
             }
+            st.stage[slot] = S_TERM;
         }
-        wave_append(st.q_term, counters + C_TERM, active, slot);
     }
 }
 
 // One path vertex (integrator.h:178-260) for the slots of one material class.
 template <int Lobes, bool Multi>
-__global__ __launch_bounds__(256) void shade_kernel(Work work, State st, int* counters, int par, int cls) {
+__global__ __launch_bounds__(256) void shade_kernel(Work work, State st, int* counters, int cls) {
     const int n = counters[C_CLASS + cls];
     const int* queue = st.q_class[cls];
-    for (;;) {
-        const int base = wave_fetch(counters + C_CUR + 2 + cls);
-        if (base >= n) break;
+    const JobParams single = single_job<Multi>(work);
+    for (int base = grid_wave() * 64; base < n; base += grid_waves() * 64) {
         const int i = base + lane_id();
-        const bool active = i < n;
-        int slot = 0;
-        bool cont = false, shadow = false;
-        if (active) {
-            slot = queue[i];
-            const JobParams& jp = job_of<Multi>(work, st, slot);
+        if (i < n) {
+            const int slot = queue[i];
+            bool cont = false, shadow = false;
+            const JobParams& jp = job_of<Multi>(work, st, slot, single);
             const DScene& s = jp.scene;
             const PathOpts opt = jp.opt;
             const bool has_lights = s.p_env + s.p_emissive + s.p_punctual > 0;
@@ -453,31 +604,29 @@ __global__ __launch_bounds__(256) void shade_kernel(Work work, State st, int* co
                 st.rng[slot] = rng.state;
             }
             st.L[slot] = make_float4(L.x, L.y, L.z, 0.0f);
+            st.stage[slot] = uint8_t((cont ? S_EXT : S_TERM) | (shadow ? S_SHADOW : 0));
         }
-        wave_append(st.q_shadow, counters + C_SHADOW, active && shadow, slot);
-        wave_append(st.q[1 - par], counters + C_EXT0 + (1 - par), active && cont, slot);
-        wave_append(st.q_term, counters + C_TERM, active && !cont, slot);
     }
 }
 
 // Finished paths: add the sample to its pixel (render_kernel's accumulation order), then start the
-// pixel's next sample, or take the next work item, or go idle.
+// pixel's next sample, or take the next work item, or go idle. Serving also lists idle slots here while
+// work waits; they only take work.
 template <bool Multi>
-__global__ __launch_bounds__(256) void regen_kernel(Work work, State st, int* counters, int par) {
+__global__ __launch_bounds__(256) void regen_kernel(Work work, State st, int* counters) {
     const int n = counters[C_TERM];
-    const int total = counters[C_TOTAL], chunks = counters[C_CHUNKS];
-    for (;;) {
-        const int base = wave_fetch(counters + C_CUR + 7);
-        if (base >= n) break;
+    const long long total = (long long)work.items[1];
+    const int chunks = counters[C_CHUNKS];
+    const JobParams single = single_job<Multi>(work);
+    for (int base = grid_wave() * 64; base < n; base += grid_waves() * 64) {
         const int i = base + lane_id();
         const bool active = i < n;
-        int slot = 0, pixel = -1, k = 0;
-        bool need_work = false;
-        if (active) {
-            slot = st.q_term[i];
-            const JobParams& jp = job_of<Multi>(work, st, slot);
-            pixel = st.pixel[slot];
-            k = st.sample[slot];
+        const int slot = active ? st.q_term[i] : 0;
+        bool need_work = false, ext = false;
+        if (active && (!Multi || (st.stage[slot] & S_MASK) == S_TERM)) {
+            const JobParams& jp = job_of<Multi>(work, st, slot, single);
+            const int pixel = st.pixel[slot];
+            int k = st.sample[slot];
             f3 L = xyz(st.L[slot]);
             if (!finite3(L) || L.x < 0 || L.y < 0 || L.z < 0) L = mk3(0);
             float4 a = jp.accum[pixel];
@@ -489,14 +638,21 @@ __global__ __launch_bounds__(256) void regen_kernel(Work work, State st, int* co
             jp.accum[pixel] = a;
             k++;
             need_work = k >= (Multi ? st.sample_end[slot] : jp.spp);
-            if (need_work) atomicAdd(work.job_done + (Multi ? st.job[slot] : 0), 1);
-            else start_sample(jp, st, slot, pixel, k);
+            if (need_work) {
+                atomicAdd(work.job_done + (Multi ? st.job[slot] : 0), 1);
+            } else {
+                start_sample(jp, st, slot, pixel, k);
+                ext = true;
+            }
+        } else if (active) {
+            need_work = true;  // an idle slot
         }
-        const int w = wave_take(counters + C_WORK, need_work);
-        const bool got = need_work && w < total;
-        if (got) start_work<Multi>(work, st, chunks, slot, w);
-        wave_append(st.q[1 - par], counters + C_EXT0 + (1 - par), active && (!need_work || got), slot);
-        wave_append(st.q_idle[1 - par], counters + C_IDLE0 + (1 - par), need_work && !got, slot);
+        // Skip the atomic when every work item has been handed out (the counter only grows).
+        const bool want = need_work && items_next(work) < total;
+        const long long w = wave_take(work.items, want);
+        const bool got = want && w < total;
+        if (got) start_work<Multi>(work, st, chunks, slot, w, single);
+        if (active) st.stage[slot] = (ext || got) ? S_EXT : S_IDLE;
     }
 }
 
@@ -507,22 +663,22 @@ int device_cus() {
     return cus;
 }
 
-using TraceLaunch = void (*)(hipStream_t, int, Work, State, int*, int, TraceTotals*, bool);
+using TraceLaunch = void (*)(hipStream_t, int, Work, State, int*, int, int, TraceTotals*, bool);
 
 template <int Block, int Depth, bool Multi>
-void launch_extend(hipStream_t st, int grid, Work w, State s, int* counters, int par, TraceTotals* tot, bool count) {
+void launch_extend(hipStream_t st, int grid, Work w, State s, int* counters, int sort, int refill, TraceTotals* tot, bool count) {
     if (count)
-        hipLaunchKernelGGL((extend_kernel<Block, Depth, true, Multi>), dim3(grid), dim3(Block), 0, st, w, s, counters, par, tot);
+        hipLaunchKernelGGL((extend_kernel<Block, Depth, true, Multi>), dim3(grid), dim3(Block), 0, st, w, s, counters, sort, refill, tot);
     else
-        hipLaunchKernelGGL((extend_kernel<Block, Depth, false, Multi>), dim3(grid), dim3(Block), 0, st, w, s, counters, par, tot);
+        hipLaunchKernelGGL((extend_kernel<Block, Depth, false, Multi>), dim3(grid), dim3(Block), 0, st, w, s, counters, sort, refill, tot);
 }
 
 template <int Block, int Depth, bool Multi>
-void launch_shadow(hipStream_t st, int grid, Work w, State s, int* counters, int, TraceTotals* tot, bool count) {
+void launch_shadow(hipStream_t st, int grid, Work w, State s, int* counters, int, int refill, TraceTotals* tot, bool count) {
     if (count)
-        hipLaunchKernelGGL((shadow_kernel<Block, Depth, true, Multi>), dim3(grid), dim3(Block), 0, st, w, s, counters, tot);
+        hipLaunchKernelGGL((shadow_kernel<Block, Depth, true, Multi>), dim3(grid), dim3(Block), 0, st, w, s, counters, refill, tot);
     else
-        hipLaunchKernelGGL((shadow_kernel<Block, Depth, false, Multi>), dim3(grid), dim3(Block), 0, st, w, s, counters, tot);
+        hipLaunchKernelGGL((shadow_kernel<Block, Depth, false, Multi>), dim3(grid), dim3(Block), 0, st, w, s, counters, refill, tot);
 }
 
 struct ShapeFns {
@@ -576,11 +732,14 @@ std::vector<int> material_classes(const DScene& scene, hipStream_t st) {
 }
 
 struct WavefrontEngine::Impl {
-    ph_wavefront_config cfg{128, 16, 0, 8, 1, 1, 16};
+    ph_wavefront_config cfg{128, 16, 0, 8, 1, 1, 16, 16};
     int slots = 0;
     int cus = 0;
-    DevBuf<int> job, pixel, sample, sample_end, counters, q0, q1, qi0, qi1, q_shadow, q_term, q_miss, q_cls[kClasses], mat_class;
-    DevBuf<int> chunk_first, job_done;
+    DevBuf<int> job, pixel, sample, sample_end, counters, q_ext, q_shadow, q_term, q_miss, q_cls[kClasses], mat_class, partial;
+    DevBuf<uint8_t> stage;
+    DevBuf<int> job_done;
+    DevBuf<long long> chunk_first;
+    DevBuf<unsigned long long> items;  // next work item, items queued
     DevBuf<int4> chunk_pass;
     DevBuf<uint64_t> rng;
     DevBuf<float4> ro, rd, beta, L, hit, so, sd, sc;
@@ -590,6 +749,8 @@ struct WavefrontEngine::Impl {
     int* host_counters = nullptr;
     int* host_done = nullptr;
     int* host_sizes = nullptr;
+    unsigned long long* host_items = nullptr;  // after a batch: next item, items queued
+    unsigned long long* host_stage = nullptr;  // staging of item counts for upload
     std::vector<hipEvent_t> events;
     hipGraphExec_t graph_exec = nullptr;
     std::tuple<int, int, int, int, int, bool> graph_key{};
@@ -599,7 +760,7 @@ struct WavefrontEngine::Impl {
     std::vector<bool> entry_used;
     std::vector<int> entry_pixels;
     struct Chunk {
-        int first;
+        long long first;
         int4 pass;
     };
     std::deque<Chunk> chunks;  // queue order
@@ -610,9 +771,13 @@ struct WavefrontEngine::Impl {
         PH_CHECK(hipHostMalloc(reinterpret_cast<void**>(&host_counters), C_COUNT * sizeof(int), hipHostMallocDefault));
         PH_CHECK(hipHostMalloc(reinterpret_cast<void**>(&host_done), kMaxJobs * sizeof(int), hipHostMallocDefault));
         PH_CHECK(hipHostMalloc(reinterpret_cast<void**>(&host_sizes), 2 * sizeof(int), hipHostMallocDefault));
+        PH_CHECK(hipHostMalloc(reinterpret_cast<void**>(&host_items), 2 * sizeof(unsigned long long), hipHostMallocDefault));
+        PH_CHECK(hipHostMalloc(reinterpret_cast<void**>(&host_stage), 2 * sizeof(unsigned long long), hipHostMallocDefault));
+        std::memset(host_items, 0, 2 * sizeof(unsigned long long));
         counters.allocate(C_COUNT);
         jobs.allocate(kMaxJobs);
         chunk_first.allocate(kMaxChunks);
+        items.allocate(2);
         chunk_pass.allocate(kMaxChunks);
         job_done.allocate(kMaxJobs);
         totals.allocate(2);
@@ -627,6 +792,8 @@ struct WavefrontEngine::Impl {
         if (host_counters) (void)hipHostFree(host_counters);
         if (host_done) (void)hipHostFree(host_done);
         if (host_sizes) (void)hipHostFree(host_sizes);
+        if (host_items) (void)hipHostFree(host_items);
+        if (host_stage) (void)hipHostFree(host_stage);
     }
 
     void drop_graph() {
@@ -637,8 +804,10 @@ struct WavefrontEngine::Impl {
     void ensure_slots(int n) {
         if (n == slots) return;
         drop_graph();
-        for (DevBuf<int>* b : {&job, &pixel, &sample, &sample_end, &q0, &q1, &qi0, &qi1, &q_shadow, &q_term, &q_miss}) b->allocate(size_t(n));
+        for (DevBuf<int>* b : {&job, &pixel, &sample, &sample_end, &q_ext, &q_shadow, &q_term, &q_miss}) b->allocate(size_t(n));
         for (auto& q : q_cls) q.allocate(size_t(n));
+        stage.allocate(size_t(n + 15) / 16 * 16);
+        partial.allocate(size_t(kMaxCategories) * size_t(chunks_of(n)));
         rng.allocate(size_t(n));
         for (DevBuf<float4>* b : {&ro, &rd, &beta, &L, &hit, &so, &sd, &sc}) b->allocate(size_t(n));
         slots = n;
@@ -659,10 +828,8 @@ struct WavefrontEngine::Impl {
         s.so = so.ptr;
         s.sd = sd.ptr;
         s.sc = sc.ptr;
-        s.q[0] = q0.ptr;
-        s.q[1] = q1.ptr;
-        s.q_idle[0] = qi0.ptr;
-        s.q_idle[1] = qi1.ptr;
+        s.stage = stage.ptr;
+        s.q_ext = q_ext.ptr;
         s.q_shadow = q_shadow.ptr;
         s.q_term = q_term.ptr;
         s.q_miss = q_miss.ptr;
@@ -670,7 +837,7 @@ struct WavefrontEngine::Impl {
         return s;
     }
 
-    Work work() const { return Work{jobs.ptr, chunk_first.ptr, chunk_pass.ptr, job_done.ptr}; }
+    Work work() { return Work{jobs.ptr, items.ptr, chunk_first.ptr, chunk_pass.ptr, job_done.ptr}; }
 
     ShapeFns shapes(bool multi) const {
         ShapeFns f{};
@@ -679,28 +846,51 @@ struct WavefrontEngine::Impl {
         return f;
     }
 
-    // One bounce (iteration parity `par`). Event pairs bracket the traversal launches when timing.
+    static int chunks_of(int n) { return (n + kCompactChunk - 1) / kCompactChunk; }
+
+    template <int Kind, bool Multi>
+    void compact(hipStream_t st, const State& s) {
+        const int nch = chunks_of(slots);
+        QueueOut out{};
+        if (Kind == K_EXT) out.q[0] = s.q_ext;
+        if (Kind == K_HIT) {
+            out.q[0] = s.q_miss;
+            for (int c = 0; c < kClasses; c++) out.q[1 + c] = s.q_class[c];
+        }
+        if (Kind == K_DONE) {
+            out.q[0] = s.q_shadow;
+            out.q[1] = s.q_term;
+        }
+        hipLaunchKernelGGL((compact_count_kernel<Kind, Multi>), dim3(nch), dim3(kCompactThreads), 0, st, s.stage, slots, items.ptr, partial.ptr, nch);
+        hipLaunchKernelGGL((compact_scatter_kernel<Kind, Multi>), dim3(nch), dim3(kCompactThreads), 0, st, s.stage, slots, counters.ptr, items.ptr,
+                           partial.ptr, nch, out);
+    }
+
+    // One bounce. Event pairs bracket the traversal launches when timing.
     template <bool Multi>
-    void iteration(hipStream_t st, const ShapeFns& fns, int par, bool count, hipEvent_t* ev) {
+    void iteration(hipStream_t st, const ShapeFns& fns, bool count, hipEvent_t* ev) {
         const int tgrid = cus * std::max(1, cfg.blocks_per_cu);
         const int sgrid = cus * 4;
         const State s = state();
         const Work w = work();
         int* cnt = counters.ptr;
-        hipLaunchKernelGGL(bookkeep_kernel, dim3(1), dim3(64), 0, st, cnt, par, ray_totals.ptr);
-        if (Multi) hipLaunchKernelGGL(wake_kernel, dim3(sgrid), dim3(256), 0, st, w, s, cnt, par);
+        hipLaunchKernelGGL(bookkeep_kernel, dim3(1), dim3(64), 0, st, cnt, ray_totals.ptr);
         if (ev) PH_CHECK(hipEventRecord(ev[0], st));
-        fns.extend(st, tgrid, w, s, cnt, par, totals.ptr, count);
+        const int refill = std::clamp(cfg.refill_lanes, 1, 64);
+        fns.extend(st, tgrid, w, s, cnt, cfg.sort_materials ? 1 : 0, refill, totals.ptr, count);
         if (ev) PH_CHECK(hipEventRecord(ev[1], st));
+        compact<K_HIT, Multi>(st, s);
         hipLaunchKernelGGL(miss_kernel<Multi>, dim3(sgrid), dim3(256), 0, st, w, s, cnt);
-        hipLaunchKernelGGL((shade_kernel<1, Multi>), dim3(sgrid), dim3(256), 0, st, w, s, cnt, par, 0);
-        hipLaunchKernelGGL((shade_kernel<2, Multi>), dim3(sgrid), dim3(256), 0, st, w, s, cnt, par, 1);
-        hipLaunchKernelGGL((shade_kernel<4, Multi>), dim3(sgrid), dim3(256), 0, st, w, s, cnt, par, 2);
-        hipLaunchKernelGGL((shade_kernel<7, Multi>), dim3(sgrid), dim3(256), 0, st, w, s, cnt, par, 3);
+        hipLaunchKernelGGL((shade_kernel<1, Multi>), dim3(sgrid), dim3(256), 0, st, w, s, cnt, 0);
+        hipLaunchKernelGGL((shade_kernel<2, Multi>), dim3(sgrid), dim3(256), 0, st, w, s, cnt, 1);
+        hipLaunchKernelGGL((shade_kernel<4, Multi>), dim3(sgrid), dim3(256), 0, st, w, s, cnt, 2);
+        hipLaunchKernelGGL((shade_kernel<7, Multi>), dim3(sgrid), dim3(256), 0, st, w, s, cnt, 3);
+        compact<K_DONE, Multi>(st, s);
         if (ev) PH_CHECK(hipEventRecord(ev[2], st));
-        fns.shadow(st, tgrid, w, s, cnt, par, totals.ptr + 1, count);
+        fns.shadow(st, tgrid, w, s, cnt, 0, refill, totals.ptr + 1, count);
         if (ev) PH_CHECK(hipEventRecord(ev[3], st));
-        hipLaunchKernelGGL(regen_kernel<Multi>, dim3(sgrid), dim3(256), 0, st, w, s, cnt, par);
+        hipLaunchKernelGGL(regen_kernel<Multi>, dim3(sgrid), dim3(256), 0, st, w, s, cnt);
+        compact<K_EXT, Multi>(st, s);
         PH_CHECK(hipGetLastError());
     }
 
@@ -724,9 +914,9 @@ struct WavefrontEngine::Impl {
                 PH_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
                 for (int it = 0; it < G; it++) {
                     if (multi)
-                        iteration<true>(st, fns, it & 1, false, nullptr);
+                        iteration<true>(st, fns, false, nullptr);
                     else
-                        iteration<false>(st, fns, it & 1, false, nullptr);
+                        iteration<false>(st, fns, false, nullptr);
                 }
                 PH_CHECK(hipStreamEndCapture(st, &g));
                 PH_CHECK(hipGraphInstantiate(&graph_exec, g, nullptr, nullptr, 0));
@@ -735,6 +925,7 @@ struct WavefrontEngine::Impl {
             }
             PH_CHECK(hipGraphLaunch(graph_exec, st));
             PH_CHECK(hipMemcpyAsync(host_counters, counters.ptr, C_COUNT * sizeof(int), hipMemcpyDeviceToHost, st));
+            PH_CHECK(hipMemcpyAsync(host_items, items.ptr, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
             PH_CHECK(hipStreamSynchronize(st));
             return {0.0, 0.0};
         }
@@ -743,11 +934,12 @@ struct WavefrontEngine::Impl {
         ensure_events(4 * G);
         for (int it = 0; it < G; it++) {
             if (multi)
-                iteration<true>(st, fns, it & 1, counting, &events[size_t(4 * it)]);
+                iteration<true>(st, fns, counting, &events[size_t(4 * it)]);
             else
-                iteration<false>(st, fns, it & 1, counting, &events[size_t(4 * it)]);
+                iteration<false>(st, fns, counting, &events[size_t(4 * it)]);
         }
         PH_CHECK(hipMemcpyAsync(host_counters, counters.ptr, C_COUNT * sizeof(int), hipMemcpyDeviceToHost, st));
+        PH_CHECK(hipMemcpyAsync(host_items, items.ptr, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
         PH_CHECK(hipStreamSynchronize(st));
         double me = 0, ms = 0;
         for (int it = 0; it < G; it++) {
@@ -762,6 +954,7 @@ struct WavefrontEngine::Impl {
 
     void reset_counters(hipStream_t st) {
         PH_CHECK(hipMemsetAsync(counters.ptr, 0, C_COUNT * sizeof(int), st));
+        PH_CHECK(hipMemsetAsync(items.ptr, 0, 2 * sizeof(unsigned long long), st));
         PH_CHECK(hipMemsetAsync(totals.ptr, 0, 2 * sizeof(TraceTotals), st));
         PH_CHECK(hipMemsetAsync(ray_totals.ptr, 0, 2 * sizeof(unsigned long long), st));
     }
@@ -792,8 +985,8 @@ void WavefrontEngine::set_config(const ph_wavefront_config& c) {
     ShapeFns f;
     if (!shape_fns<false>(c.block, c.stack_depth, f)) throw std::invalid_argument("unsupported block/stack depth for the wavefront engine");
     if (c.path_slots < 0 || c.blocks_per_cu <= 0 || c.blocks_per_cu > 64) throw std::invalid_argument("invalid wavefront configuration");
-    if (c.graph_iterations < 2 || c.graph_iterations > 256 || c.graph_iterations % 2)
-        throw std::invalid_argument("graph_iterations must be even and in [2, 256]");
+    if (c.graph_iterations < 1 || c.graph_iterations > 256) throw std::invalid_argument("graph_iterations must be in [1, 256]");
+    if (c.refill_lanes < 1 || c.refill_lanes > 64) throw std::invalid_argument("refill_lanes must be in [1, 64]");
     impl_->cfg = c;
     impl_->drop_graph();
 }
@@ -818,10 +1011,13 @@ void WavefrontEngine::render(hipStream_t st, const WavefrontJob& job, bool count
     PH_CHECK(hipMemcpyAsync(m.jobs.ptr, &jp, sizeof(jp), hipMemcpyHostToDevice, st));
     m.reset_counters(st);
     // Work items 0..slots-1 go to the slots directly; regeneration hands out the rest.
-    const int init[2] = {std::min(slots, pixels), pixels};
-    PH_CHECK(hipMemcpyAsync(m.counters.ptr + C_WORK, init, sizeof(init), hipMemcpyHostToDevice, st));
+    PH_CHECK(hipStreamSynchronize(st));  // the staging buffer may still feed an earlier copy
+    m.host_stage[0] = (unsigned long long)std::min(slots, pixels);
+    m.host_stage[1] = (unsigned long long)pixels;
+    PH_CHECK(hipMemcpyAsync(m.items.ptr, m.host_stage, 2 * sizeof(unsigned long long), hipMemcpyHostToDevice, st));
     PH_CHECK(hipMemsetAsync(m.job_done.ptr, 0, sizeof(int), st));
-    hipLaunchKernelGGL(init_kernel, dim3((slots + 255) / 256), dim3(256), 0, st, m.work(), m.state(), m.counters.ptr, slots);
+    hipLaunchKernelGGL(init_kernel, dim3((slots + 255) / 256), dim3(256), 0, st, m.work(), m.state(), slots);
+    m.compact<K_EXT, false>(st, m.state());
     PH_CHECK(hipGetLastError());
 
     const int G = m.cfg.graph_iterations;
@@ -834,8 +1030,7 @@ void WavefrontEngine::render(hipStream_t st, const WavefrontJob& job, bool count
         ms_extend += t.first;
         ms_shadow += t.second;
         iterations += G;
-        // G is even, so the next iteration reads queue 0.
-        if (m.host_counters[C_EXT0] == 0) break;
+        if (m.host_counters[C_EXT] == 0) break;
         if (iterations > bound) throw std::runtime_error("wavefront loop did not drain");
     }
     TraceTotals tot[2] = {};
@@ -895,6 +1090,12 @@ ph_wavefront_config WavefrontEngine::autotune(hipStream_t st, const WavefrontJob
         double t = time_of(c);
         if (t < best_ms) best_ms = t, best = c;
     }
+    for (int refill : {4, 16, 32, 64}) {
+        ph_wavefront_config c = best;
+        c.refill_lanes = refill;
+        double t = time_of(c);
+        if (t < best_ms) best_ms = t, best = c;
+    }
     const int pixels = job.width * job.height;
     for (int slots : {pixels / 4, pixels / 2, 0}) {
         if (slots > 0 && slots < 65536) continue;
@@ -928,11 +1129,12 @@ void WavefrontEngine::serve_begin(hipStream_t st, int slots) {
     m.total_work = 0;
     m.reset_counters(st);
     PH_CHECK(hipMemsetAsync(m.job_done.ptr, 0, kMaxJobs * sizeof(int), st));
-    hipLaunchKernelGGL(init_idle_kernel, dim3((slots + 255) / 256), dim3(256), 0, st, m.state(), m.counters.ptr, slots);
+    hipLaunchKernelGGL(init_idle_kernel, dim3((slots + 255) / 256), dim3(256), 0, st, m.state(), slots);
     PH_CHECK(hipGetLastError());
     PH_CHECK(hipMemcpyAsync(m.host_counters, m.counters.ptr, C_COUNT * sizeof(int), hipMemcpyDeviceToHost, st));
     PH_CHECK(hipStreamSynchronize(st));
     std::memset(m.host_done, 0, kMaxJobs * sizeof(int));
+    m.host_items[0] = m.host_items[1] = 0;
 }
 
 int WavefrontEngine::serve_admit(hipStream_t st, const WavefrontJob& job) {
@@ -962,7 +1164,7 @@ bool WavefrontEngine::serve_enqueue(int entry, int k0, int k1) {
     if (k0 < 0 || k1 <= k0) throw std::invalid_argument("empty pass");
     const int pixels = m.entry_pixels[size_t(entry)];
     // Drop chunks whose items have all been handed out (their pixels keep their pass through the slot).
-    const int handed = m.host_counters[C_WORK];
+    const long long handed = (long long)m.host_items[0];
     while (m.chunks.size() > 1 && m.chunks[1].first <= handed) {
         m.chunks.pop_front();
         m.chunks_dirty = true;
@@ -971,8 +1173,7 @@ bool WavefrontEngine::serve_enqueue(int entry, int k0, int k1) {
     // Lanes that looked for work when none was left still advanced the counter past the queued total;
     // the new items start after the counter so that none of them is skipped.
     const int64_t first = std::max<int64_t>(m.total_work, handed);
-    if (first + pixels > int64_t(INT32_MAX) - (int64_t(m.slots) + 64)) throw std::runtime_error("work index space exhausted; restart the server");
-    m.chunks.push_back(Impl::Chunk{int(first), make_int4(entry, k0, k1, 0)});
+    m.chunks.push_back(Impl::Chunk{first, make_int4(entry, k0, k1, 0)});
     m.total_work = first + pixels;
     m.chunks_dirty = true;
     return true;
@@ -982,24 +1183,26 @@ void WavefrontEngine::serve_run(hipStream_t st, int iterations) {
     Impl& m = *impl_;
     if (!m.serving) throw std::runtime_error("serve_begin was not called");
     if (m.chunks_dirty) {
-        std::vector<int> starts;
+        std::vector<long long> starts;
         std::vector<int4> passes;
         for (const auto& c : m.chunks) {
             starts.push_back(c.first);
             passes.push_back(c.pass);
         }
         if (!starts.empty()) {
-            PH_CHECK(hipMemcpyAsync(m.chunk_first.ptr, starts.data(), starts.size() * sizeof(int), hipMemcpyHostToDevice, st));
+            PH_CHECK(hipMemcpyAsync(m.chunk_first.ptr, starts.data(), starts.size() * sizeof(long long), hipMemcpyHostToDevice, st));
             PH_CHECK(hipMemcpyAsync(m.chunk_pass.ptr, passes.data(), passes.size() * sizeof(int4), hipMemcpyHostToDevice, st));
         }
-        // Staged in pinned memory: the copy runs after this call returns.
-        m.host_sizes[0] = int(m.total_work);
-        m.host_sizes[1] = int(m.chunks.size());
-        PH_CHECK(hipMemcpyAsync(m.counters.ptr + C_TOTAL, m.host_sizes, 2 * sizeof(int), hipMemcpyHostToDevice, st));
+        // Staged in pinned memory (the stream was synchronized at the end of the previous run).
+        m.host_stage[1] = (unsigned long long)m.total_work;
+        m.host_sizes[0] = int(m.chunks.size());
+        PH_CHECK(hipMemcpyAsync(m.items.ptr + 1, m.host_stage + 1, sizeof(unsigned long long), hipMemcpyHostToDevice, st));
+        PH_CHECK(hipMemcpyAsync(m.counters.ptr + C_CHUNKS, m.host_sizes, sizeof(int), hipMemcpyHostToDevice, st));
+        // The tables are read by the run below; the host vectors must outlive the copies.
+        PH_CHECK(hipStreamSynchronize(st));
         m.chunks_dirty = false;
     }
-    const int G = std::max(2, iterations + (iterations & 1));
-    m.run_batch(st, G, true, false);
+    m.run_batch(st, std::max(1, iterations), true, false);
     PH_CHECK(hipMemcpyAsync(m.host_done, m.job_done.ptr, kMaxJobs * sizeof(int), hipMemcpyDeviceToHost, st));
     PH_CHECK(hipStreamSynchronize(st));
 }
@@ -1010,8 +1213,8 @@ void WavefrontEngine::serve_release(int entry) {
     if (entry >= 0 && entry < kMaxJobs) impl_->entry_used[size_t(entry)] = false;
 }
 
-int WavefrontEngine::serve_in_flight() const { return impl_->host_counters[C_EXT0]; }
+int WavefrontEngine::serve_in_flight() const { return impl_->host_counters[C_EXT]; }
 
-int64_t WavefrontEngine::serve_work_remaining() const { return impl_->total_work - impl_->host_counters[C_WORK]; }
+int64_t WavefrontEngine::serve_work_remaining() const { return impl_->total_work - (long long)impl_->host_items[0]; }
 
 }  // namespace phk

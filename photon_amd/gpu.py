@@ -24,7 +24,8 @@ class GpuInfo(C.Structure):
 
 class WavefrontConfig(C.Structure):
     _fields_ = [("block", C.c_int32), ("stack_depth", C.c_int32), ("path_slots", C.c_int32), ("blocks_per_cu", C.c_int32),
-                ("sort_materials", C.c_int32), ("use_graph", C.c_int32), ("graph_iterations", C.c_int32)]
+                ("sort_materials", C.c_int32), ("use_graph", C.c_int32), ("graph_iterations", C.c_int32),
+                ("refill_lanes", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
