@@ -65,7 +65,7 @@ def bench_render(args, gpu, host, RenderSettings):
         r.reset()
         r.render(cam, rs, SPP)
         log(f"warmup {i}: {r.stats().ms_total:.1f} ms")
-    times, ext_ms, ext_launches, shadow_ms = [], 0.0, 0, 0.0
+    times = []
     rays_ext = rays_sh = 0
     for i in range(args.steps):
         r.reset()
@@ -73,14 +73,21 @@ def bench_render(args, gpu, host, RenderSettings):
         r.render(cam, rs, SPP)
         times.append(time.perf_counter() - t0)
         s = r.stats()
-        ext_ms += s.ms_extend
-        shadow_ms += s.ms_shadow
-        ext_launches += s.extend_launches
         rays_ext, rays_sh = s.rays_extend, s.rays_shadow
-        log(f"step {i}: {times[-1] * 1e3:.1f} ms, extend {s.ms_extend:.1f} ms, shadow {s.ms_shadow:.1f} ms, {s.iterations} iterations")
+        log(f"step {i}: {times[-1] * 1e3:.1f} ms, {s.iterations} iterations")
     total = sum(times)
     samples = W * H * SPP
     value = samples * args.steps / total / 1e6
+    # Per-kernel durations: the same render launched directly, with HIP events around every traversal
+    # launch on the library's stream (a graph replay cannot be timed per kernel).
+    tuned = r.wavefront_config()
+    r.set_wavefront_config(use_graph=0)
+    r.reset()
+    r.render(cam, rs, SPP)
+    es = r.stats()
+    r.set_wavefront_config(tuned)
+    ext_ms, ext_launches, shadow_ms = es.ms_extend, es.extend_launches, es.ms_shadow
+    log(f"event-timed render: {es.ms_total:.1f} ms, extend {ext_ms:.1f} ms over {ext_launches} launches, shadow {shadow_ms:.1f} ms")
     # Instrumented pass (same deterministic work) for the visit counts.
     r.reset()
     r.set_counting(True)
@@ -95,7 +102,7 @@ def bench_render(args, gpu, host, RenderSettings):
     out = dict(value=value, ms_per_step=total / args.steps * 1e3, samples=samples, config=cfg.as_dict() if cfg else r.wavefront_config().as_dict(),
                rays_extend=rays_ext, rays_shadow=rays_sh, rays_per_sample=(rays_ext + rays_sh) / samples,
                mrays_s=(rays_ext + rays_sh) * args.steps / total / 1e6,
-               extend_ms_total=ext_ms / args.steps, shadow_ms_total=shadow_ms / args.steps,
+               extend_ms_total=ext_ms, shadow_ms_total=shadow_ms,
                extend_avg_launch_ms=avg_launch_ms, extend_bytes_per_launch=bytes_per_launch, achieved_gbs=achieved,
                node_visits_per_ray=cs.node_visits / max(1, cs.rays_extend), tri_tests_per_ray=cs.tri_tests / max(1, cs.rays_extend),
                shadow_bytes=sh_bytes, extend_bytes=ext_bytes)
@@ -104,7 +111,7 @@ def bench_render(args, gpu, host, RenderSettings):
         r.reset()
         r.render(cam, rs, 1)
         r.reset()
-        mk_spp = max(1, SPP // args.megakernel_divisor)
+        mk_spp = max(1, SPP // max(1, args.megakernel_divisor))
         t0 = time.perf_counter()
         r.render(cam, rs, mk_spp)
         mk = time.perf_counter() - t0
@@ -205,14 +212,18 @@ def draw_jobs(n, seed):
     return jobs
 
 
-def bench_serving(args, gpu, host, field):
+def bench_serving(args, gpu, host, field, launch):
     """Open-loop Poisson arrivals at 80% of the saturated throughput over three resident scenes."""
     import numpy as np
     t = time.time()
     scenes = [host.Scene.field(10, seed=7, env_width=2048).build(),
               host.Scene.spheres(256, box=100.0, alpha_fraction=0.1, seed=1).build(), field]
     log(f"serving scenes: {[s.num_tris for s in scenes]} triangles, setup {time.time() - t:.1f} s")
-    srv = gpu.Server(path_slots=args.serve_slots, iterations_per_step=args.serve_iterations)
+    srv = gpu.Server(path_slots=args.serve_slots, iterations_per_step=args.serve_iterations,
+                     schedule=gpu.SCHEDULE_FIFO if args.serve_schedule == "fifo" else gpu.SCHEDULE_SRPT)
+    cfg = gpu.WavefrontConfig(**launch)
+    cfg.path_slots = 0
+    srv.set_wavefront_config(cfg)
     ids = [srv.add_scene(s) for s in scenes]
 
     def submit(job):
@@ -280,11 +291,30 @@ def bench_serving(args, gpu, host, field):
                msamples_s=round(samples.sum() / ((end - start) * 1e-3) / 1e6, 2), duration_s=round((end - start) * 1e-3, 2),
                latency_ms={"p50": round(float(np.percentile(L, 50)), 2), "p99": round(float(np.percentile(L, 99)), 2)},
                slowdown={"p50": round(float(np.percentile(slow, 50)), 2), "p99": round(float(np.percentile(slow, 99)), 2)},
-               by_job_size=by_size, path_slots=args.serve_slots, iterations_per_step=args.serve_iterations)
+               by_job_size=by_size, path_slots=args.serve_slots, iterations_per_step=args.serve_iterations, schedule=args.serve_schedule)
     res["latency_p99_over_p50"] = round(res["latency_ms"]["p99"] / res["latency_ms"]["p50"], 2)
     res["slowdown_p99_over_p50"] = round(res["slowdown"]["p99"] / res["slowdown"]["p50"], 2)
     log(f"serving: {res}")
     return res
+
+
+def measured_traffic(spp):
+    """HBM bytes per closest-hit launch from the committed rocprofv3 counter summary of this workload
+    (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the copy-kernel calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "extend_kernel_field_1080p_pmc.md")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if ":" in line and line.strip().startswith("- "):
+            k, v = line.strip()[2:].split(":", 1)
+            vals[k.strip()] = v.strip().split()[0]
+    try:
+        if int(vals.get("spp", -1)) != spp:
+            return None
+        return int(float(vals["traffic_bytes_per_launch"]))
+    except (KeyError, ValueError):
+        return None
 
 
 def cpu_baseline(scene, cam, rs):
@@ -316,7 +346,7 @@ def main():
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-megakernel", action="store_true")
-    ap.add_argument("--megakernel-divisor", type=int, default=4, help="time the plain kernel on spp/divisor samples")
+    ap.add_argument("--megakernel-divisor", type=int, default=1, help="time the plain kernel on spp/divisor samples")
     ap.add_argument("--no-traversal", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
@@ -327,6 +357,9 @@ def main():
     ap.add_argument("--serve-load", type=float, default=0.8)
     ap.add_argument("--serve-slots", type=int, default=1 << 21)
     ap.add_argument("--serve-iterations", type=int, default=8)
+    ap.add_argument("--serve-schedule", choices=("srpt", "fifo"), default="srpt")
+    ap.add_argument("--calibrate-copy-mb", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (the known-size kernel that scales counter readings)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: --gpus must be 1", "n_gpus": args.gpus}))
@@ -336,10 +369,18 @@ def main():
     assert gpu.gpu_available(), "no HIP device"
     info = gpu.device_info(0)
     log(f"device {info.name.decode()} {info.arch.decode()} CUs={info.compute_units}")
+    if args.calibrate_copy_mb > 0:
+        nbytes = args.calibrate_copy_mb << 20
+        src, dst = gpu.DeviceArray(nbytes), gpu.DeviceArray(nbytes)
+        src.zero()
+        gpu.lib().ph_copy_device(dst.ptr, src.ptr, nbytes)
+        log(f"calibration copy: {nbytes} bytes")
+        src.free()
+        dst.free()
     scene, cam, rs, rnd = bench_render(args, gpu, host, RenderSettings)
     trav = None if args.no_traversal else bench_traversal(gpu, host)
     svgf = None if args.no_svgf else bench_svgf(args, gpu, host, scene)
-    serve = None if args.no_serve else bench_serving(args, gpu, host, scene)
+    serve = None if args.no_serve else bench_serving(args, gpu, host, scene, rnd["config"])
     cpu = None if args.no_cpu else cpu_baseline(scene, cam, rs)
     line = {
         "metric": METRIC,
@@ -357,7 +398,7 @@ def main():
         "config": {"workload": "wavefront path trace, 1920x1080, 64 spp, max depth 16, 1,048,578-triangle sphere field, 2048x1024 HDR sky",
                    "engine": "wavefront", "launch": rnd["config"], "spp": args.spp},
         "roofline": {"bound": "hbm", "kernel": "extend_kernel (closest-hit traversal)", "achieved": round(rnd["achieved_gbs"], 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(rnd["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(rnd["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.spp),
                      "bytes_per_launch": int(rnd["extend_bytes_per_launch"]), "avg_launch_ms": round(rnd["extend_avg_launch_ms"], 4)},
         "cpu_baseline": cpu,
         "mrays_s": round(rnd["mrays_s"], 1),

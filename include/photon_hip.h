@@ -96,6 +96,7 @@ int ph_device_alloc(int64_t bytes, void** out_dev);
 int ph_device_free(void* dev);
 int ph_copy_to_device(void* dst_dev, const void* src, int64_t bytes);
 int ph_copy_to_host(void* dst, const void* src_dev, int64_t bytes);
+int ph_copy_device(void* dst_dev, const void* src_dev, int64_t bytes);
 int ph_device_memset(void* dev, int value, int64_t bytes);
 
 typedef struct ph_renderer ph_renderer;
@@ -161,7 +162,12 @@ typedef struct ph_server_config {
     int32_t iterations_per_step; /* bounces per ph_server_step (0: 8) */
     int32_t max_jobs;            /* jobs admitted at once, at most 1024 (0: 1024) */
     int64_t pass_samples;        /* pixel samples per pass of a job (0: 4 x path_slots) */
+    int32_t schedule;            /* PH_SCHEDULE_SRPT or PH_SCHEDULE_FIFO */
+    int32_t reserved;
 } ph_server_config;
+
+#define PH_SCHEDULE_SRPT 0 /* unstarted work of the job with the least remaining work first */
+#define PH_SCHEDULE_FIFO 1 /* passes in the order they were queued (round robin over running jobs) */
 
 typedef struct ph_job_desc {
     int32_t scene; /* id returned by ph_server_add_scene */
@@ -192,8 +198,8 @@ int ph_server_add_scene(ph_server* s, const ph_scene_desc* scene);
 int ph_server_submit(ph_server* s, const ph_job_desc* job);
 /* Admits queued jobs that fit, advances all paths in flight by iterations_per_step bounces (blocking),
  * queues the next pass of every job whose pass finished, and returns the number of jobs finished since
- * the server started. A job runs as passes of at most pass_samples pixel samples; passes are served
- * in the order they were queued, so running jobs share the path slots round-robin. */
+ * the server started. A job runs as passes of at most pass_samples pixel samples; unstarted work is
+ * handed to path slots in the order of the schedule. */
 int ph_server_step(ph_server* s);
 int ph_server_status(const ph_server* s, int job, ph_job_status* out);
 /* Average radiance of a finished job (3 floats per pixel); frees its frame buffer. */

@@ -175,6 +175,14 @@ int ph_copy_to_host(void* dst, const void* src, int64_t bytes) {
     });
 }
 
+int ph_copy_device(void* dst_dev, const void* src_dev, int64_t bytes) {
+    if (bytes < 0 || (bytes > 0 && (!dst_dev || !src_dev))) return fail("invalid argument");
+    return guarded([&] {
+        if (bytes) PH_CHECK(hipMemcpy(dst_dev, src_dev, size_t(bytes), hipMemcpyDeviceToDevice));
+        return 0;
+    });
+}
+
 int ph_device_memset(void* dev, int value, int64_t bytes) {
     if (bytes < 0 || (bytes > 0 && !dev)) return fail("invalid argument");
     return guarded([&] {

@@ -3,9 +3,9 @@
 //
 // A step admits the queued jobs that fit (frame buffer bytes within the budget, a free job-table entry),
 // then advances every path in flight by a fixed number of bounces. A job runs as passes of at most
-// `pass_samples` pixel samples; its next pass is queued behind everyone else's work once the previous
-// one has finished, so running jobs share the path slots round-robin and a short job is not stuck
-// behind a long one. Jobs finish independently, and a finished job's image can be read while the
+// `pass_samples` pixel samples; its next pass is queued once the previous one has finished. Unstarted
+// work is served shortest-remaining-job first (or passes first come, first served), so a short job is
+// not stuck behind a long one. Jobs finish independently, and a finished job's image can be read while the
 // others keep running. A pixel's samples are taken in order (pass after pass), so an image does not
 // depend on what else shared the wavefronts.
 #include <chrono>
@@ -62,6 +62,7 @@ struct ph_server {
         if (cfg.iterations_per_step <= 0) cfg.iterations_per_step = 8;
         if (cfg.max_jobs <= 0 || cfg.max_jobs > WavefrontEngine::kMaxJobs) cfg.max_jobs = WavefrontEngine::kMaxJobs;
         if (cfg.pass_samples <= 0) cfg.pass_samples = int64_t(4) * cfg.path_slots;
+        if (cfg.schedule != PH_SCHEDULE_SRPT && cfg.schedule != PH_SCHEDULE_FIFO) throw std::invalid_argument("unknown schedule");
         PH_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         engine.serve_begin(stream, cfg.path_slots);
     }
@@ -112,7 +113,9 @@ struct ph_server {
 
     void queue_next_pass(Job& j) {
         const int k0 = j.next_k, k1 = std::min(j.desc.spp, k0 + j.pass_spp);
-        if (!engine.serve_enqueue(j.entry, k0, k1)) {
+        // Shortest remaining work first, or passes in the order they are queued.
+        const double priority = cfg.schedule == PH_SCHEDULE_FIFO ? 0.0 : double(j.status.pixels) * double(j.desc.spp - k0);
+        if (!engine.serve_enqueue(j.entry, k0, k1, priority)) {
             j.pass_pending = true;
             return;
         }

@@ -45,13 +45,14 @@ public:
 
     // Serving. serve_begin makes `slots` idle paths; serve_admit places a job in a free table entry
     // (returns it, or -1 when the table is full); serve_enqueue queues one pass of an admitted job (all
-    // its pixels, samples k0..k1-1) behind the work already queued, and returns false when the
-    // chunk table is full; serve_run advances every path in flight by `iterations` bounces (blocking).
-    // serve_pixels_done counts the finished pixel passes of an entry; a pass must not be queued before
-    // the previous pass of the same job has finished. serve_release frees the entry.
+    // its pixels, samples k0..k1-1) and returns false when the pass table is full; serve_run lays out
+    // the unstarted items of the queued passes by priority (smaller first, then queue order) and
+    // advances every path in flight by `iterations` bounces (blocking). serve_pixels_done counts the
+    // finished pixel passes of an entry; a pass must not be queued before the previous pass of the same
+    // job has finished. serve_release frees the entry.
     void serve_begin(hipStream_t st, int slots);
     int serve_admit(hipStream_t st, const WavefrontJob& job);
-    bool serve_enqueue(int entry, int k0, int k1);
+    bool serve_enqueue(int entry, int k0, int k1, double priority);
     void serve_run(hipStream_t st, int iterations);
     int serve_pixels_done(int entry) const;
     void serve_release(int entry);

@@ -4,9 +4,10 @@
 // of one job at a time and runs that pixel's samples one after another, so each pixel's accumulation
 // is the same sequence of float additions as in the one-thread-per-pixel kernel (bit-identical images,
 // whatever else shares the wavefront); when a pixel's samples are done the slot takes the next work
-// item (regeneration). A work item is one pixel of a pass: a range of sample indices of one job.
-// Passes are queued in order (chunks), and a job's next pass is queued only once its previous pass
-// has finished, so the samples of a pixel still run in order while large jobs yield to new ones.
+// item (regeneration). A work item is one pixel of a pass: a range of sample indices of one job. A
+// job's next pass is queued only once its previous pass has finished, so the samples of a pixel still
+// run in order. Between runs the host lays out the passes' unstarted items in priority order (chunks of
+// consecutive item indices after those already handed out), so large jobs yield to new ones.
 //
 // Every slot carries a one-byte stage (waiting to be traced, missed, hit a material class, finished,
 // idle) plus a pending-shadow-ray bit. Queues are not appended to: after each stage an ordered
@@ -99,10 +100,10 @@ struct State {
     int* q_class[kClasses];
 };
 
-// Work items in queue order: chunk c holds items [chunk_first[c], ...), the pixels of one pass,
-// described by chunk_pass[c] = (table entry, first sample, end sample). Item indices are 64-bit: a
-// server hands out items for as long as it runs. items[0] is the next item to hand out, items[1] the
-// number queued so far.
+// Work items in queue order: chunk c holds items [chunk_first[c], ...), pixels of one pass, described
+// by chunk_pass[c] = (table entry, first sample, end sample, local index of the chunk's first item).
+// Item indices are 64-bit: a server hands out items for as long as it runs. items[0] is the next item
+// to hand out, items[1] the end of the items laid out so far.
 struct Work {
     const JobParams* jobs;
     unsigned long long* items;
@@ -223,7 +224,7 @@ PH_DEV void start_work(const Work& work, const State& st, int chunks, int slot, 
     const int c = chunk_of<Multi>(work, chunks, w);
     const int4 pass = Multi ? work.chunk_pass[c] : make_int4(0, 0, 0, 0);
     const JobParams& jp = Multi ? work.jobs[pass.x] : single;
-    const int local = int(w - (Multi ? work.chunk_first[c] : 0));
+    const int local = int(w - (Multi ? work.chunk_first[c] : 0)) + pass.w;
     if (Multi) {
         st.job[slot] = pass.x;
         st.sample_end[slot] = pass.z;
@@ -732,7 +733,7 @@ std::vector<int> material_classes(const DScene& scene, hipStream_t st) {
 }
 
 struct WavefrontEngine::Impl {
-    ph_wavefront_config cfg{128, 16, 0, 8, 1, 1, 16, 16};
+    ph_wavefront_config cfg{128, 24, 0, 16, 1, 1, 16, 8};
     int slots = 0;
     int cus = 0;
     DevBuf<int> job, pixel, sample, sample_end, counters, q_ext, q_shadow, q_term, q_miss, q_cls[kClasses], mat_class, partial;
@@ -759,13 +760,18 @@ struct WavefrontEngine::Impl {
     bool serving = false;
     std::vector<bool> entry_used;
     std::vector<int> entry_pixels;
-    struct Chunk {
+    // A queued pass: its unstarted items are [next, pixels) of the job's pixel order. While laid out,
+    // its items occupy [first, first + pixels - next) of the item index space.
+    struct Pass {
+        int entry, k0, k1, pixels, next;
+        double priority;
+        uint64_t order;
         long long first;
-        int4 pass;
     };
-    std::deque<Chunk> chunks;  // queue order
+    std::vector<Pass> passes;  // laid out in this order at the last run
+    uint64_t pass_counter = 0;
     bool chunks_dirty = false;
-    int64_t total_work = 0;
+    int64_t total_work = 0;  // end of the item layout
 
     Impl() {
         PH_CHECK(hipHostMalloc(reinterpret_cast<void**>(&host_counters), C_COUNT * sizeof(int), hipHostMallocDefault));
@@ -1124,7 +1130,7 @@ void WavefrontEngine::serve_begin(hipStream_t st, int slots) {
     m.serving = true;
     m.entry_used.assign(kMaxJobs, false);
     m.entry_pixels.assign(kMaxJobs, 0);
-    m.chunks.clear();
+    m.passes.clear();
     m.chunks_dirty = false;
     m.total_work = 0;
     m.reset_counters(st);
@@ -1158,23 +1164,12 @@ int WavefrontEngine::serve_admit(hipStream_t st, const WavefrontJob& job) {
     return entry;
 }
 
-bool WavefrontEngine::serve_enqueue(int entry, int k0, int k1) {
+bool WavefrontEngine::serve_enqueue(int entry, int k0, int k1, double priority) {
     Impl& m = *impl_;
     if (entry < 0 || entry >= kMaxJobs || !m.entry_used[size_t(entry)]) throw std::invalid_argument("not an admitted job");
     if (k0 < 0 || k1 <= k0) throw std::invalid_argument("empty pass");
-    const int pixels = m.entry_pixels[size_t(entry)];
-    // Drop chunks whose items have all been handed out (their pixels keep their pass through the slot).
-    const long long handed = (long long)m.host_items[0];
-    while (m.chunks.size() > 1 && m.chunks[1].first <= handed) {
-        m.chunks.pop_front();
-        m.chunks_dirty = true;
-    }
-    if (int(m.chunks.size()) >= kMaxChunks) return false;
-    // Lanes that looked for work when none was left still advanced the counter past the queued total;
-    // the new items start after the counter so that none of them is skipped.
-    const int64_t first = std::max<int64_t>(m.total_work, handed);
-    m.chunks.push_back(Impl::Chunk{first, make_int4(entry, k0, k1, 0)});
-    m.total_work = first + pixels;
+    if (int(m.passes.size()) >= kMaxChunks) return false;
+    m.passes.push_back(Impl::Pass{entry, k0, k1, m.entry_pixels[size_t(entry)], 0, priority, m.pass_counter++, -1});
     m.chunks_dirty = true;
     return true;
 }
@@ -1182,26 +1177,47 @@ bool WavefrontEngine::serve_enqueue(int entry, int k0, int k1) {
 void WavefrontEngine::serve_run(hipStream_t st, int iterations) {
     Impl& m = *impl_;
     if (!m.serving) throw std::runtime_error("serve_begin was not called");
-    if (m.chunks_dirty) {
-        std::vector<long long> starts;
-        std::vector<int4> passes;
-        for (const auto& c : m.chunks) {
-            starts.push_back(c.first);
-            passes.push_back(c.pass);
+    // Items below the counter have been handed to slots (and items past the layout's end were asked
+    // for when there was none). Account for what each laid-out pass handed out, drop finished passes,
+    // then lay the rest out again from the counter on, by priority.
+    const long long handed = (long long)m.host_items[0];
+    bool changed = m.chunks_dirty;
+    std::vector<Impl::Pass> keep;
+    for (Impl::Pass p : m.passes) {
+        if (p.first >= 0) {
+            const long long got = std::clamp<long long>(handed - p.first, 0, p.pixels - p.next);
+            if (got > 0) changed = true;
+            p.next += int(got);
         }
+        if (p.next < p.pixels) keep.push_back(p);
+    }
+    if (changed) {
+        std::stable_sort(keep.begin(), keep.end(),
+                         [](const Impl::Pass& a, const Impl::Pass& b) { return a.priority != b.priority ? a.priority < b.priority : a.order < b.order; });
+        std::vector<long long> starts;
+        std::vector<int4> descs;
+        long long at = handed;
+        for (Impl::Pass& p : keep) {
+            p.first = at;
+            starts.push_back(at);
+            descs.push_back(make_int4(p.entry, p.k0, p.k1, p.next));
+            at += p.pixels - p.next;
+        }
+        m.total_work = at;
         if (!starts.empty()) {
             PH_CHECK(hipMemcpyAsync(m.chunk_first.ptr, starts.data(), starts.size() * sizeof(long long), hipMemcpyHostToDevice, st));
-            PH_CHECK(hipMemcpyAsync(m.chunk_pass.ptr, passes.data(), passes.size() * sizeof(int4), hipMemcpyHostToDevice, st));
+            PH_CHECK(hipMemcpyAsync(m.chunk_pass.ptr, descs.data(), descs.size() * sizeof(int4), hipMemcpyHostToDevice, st));
         }
         // Staged in pinned memory (the stream was synchronized at the end of the previous run).
         m.host_stage[1] = (unsigned long long)m.total_work;
-        m.host_sizes[0] = int(m.chunks.size());
+        m.host_sizes[0] = int(starts.size());
         PH_CHECK(hipMemcpyAsync(m.items.ptr + 1, m.host_stage + 1, sizeof(unsigned long long), hipMemcpyHostToDevice, st));
         PH_CHECK(hipMemcpyAsync(m.counters.ptr + C_CHUNKS, m.host_sizes, sizeof(int), hipMemcpyHostToDevice, st));
         // The tables are read by the run below; the host vectors must outlive the copies.
         PH_CHECK(hipStreamSynchronize(st));
-        m.chunks_dirty = false;
     }
+    m.passes.swap(keep);
+    m.chunks_dirty = false;
     m.run_batch(st, std::max(1, iterations), true, false);
     PH_CHECK(hipMemcpyAsync(m.host_done, m.job_done.ptr, kMaxJobs * sizeof(int), hipMemcpyDeviceToHost, st));
     PH_CHECK(hipStreamSynchronize(st));

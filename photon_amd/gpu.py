@@ -45,7 +45,7 @@ class RenderStats(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("memory_budget", C.c_int64), ("path_slots", C.c_int32), ("iterations_per_step", C.c_int32),
-                ("max_jobs", C.c_int32), ("pass_samples", C.c_int64)]
+                ("max_jobs", C.c_int32), ("pass_samples", C.c_int64), ("schedule", C.c_int32), ("reserved", C.c_int32)]
 
 
 class JobDesc(C.Structure):
@@ -59,6 +59,7 @@ class JobStatus(C.Structure):
 
 
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_RELEASED = range(4)
+SCHEDULE_SRPT, SCHEDULE_FIFO = 0, 1
 
 
 class SvgfInput(C.Structure):
@@ -122,6 +123,7 @@ def lib():
         L.ph_copy_to_device.argtypes = [P, P, C.c_int64]
         L.ph_copy_to_host.argtypes = [P, P, C.c_int64]
         L.ph_device_memset.argtypes = [P, C.c_int, C.c_int64]
+        L.ph_copy_device.argtypes = [P, P, C.c_int64]
         L.ph_server_create.argtypes = [C.POINTER(ServerConfig), C.POINTER(P)]
         L.ph_server_destroy.argtypes = [P]
         L.ph_server_destroy.restype = None
@@ -359,9 +361,9 @@ class Svgf:
 class Server:
     """The serving loop (ph_server_*): resident scenes, jobs admitted by HBM footprint, continuous batching."""
 
-    def __init__(self, memory_budget=0, path_slots=0, iterations_per_step=0, max_jobs=0, pass_samples=0):
+    def __init__(self, memory_budget=0, path_slots=0, iterations_per_step=0, max_jobs=0, pass_samples=0, schedule=SCHEDULE_SRPT):
         self._h = C.c_void_p()
-        cfg = ServerConfig(memory_budget, path_slots, iterations_per_step, max_jobs, pass_samples)
+        cfg = ServerConfig(memory_budget, path_slots, iterations_per_step, max_jobs, pass_samples, schedule, 0)
         _check(lib().ph_server_create(C.byref(cfg), C.byref(self._h)))
         self._sizes = {}
 

@@ -60,7 +60,8 @@ def test_each_job_alone_equals_the_renderer(scenes):
 
 @pytest.mark.parametrize("cfg", [dict(path_slots=1 << 14), dict(path_slots=3000, iterations_per_step=2),
                                  dict(path_slots=1 << 16, max_jobs=2), dict(path_slots=5000, memory_budget=64 * 64 * 16),
-                                 dict(path_slots=4096, pass_samples=3000), dict(path_slots=2000, pass_samples=1)])
+                                 dict(path_slots=4096, pass_samples=3000), dict(path_slots=2000, pass_samples=1),
+                                 dict(path_slots=3000, pass_samples=5000, schedule=gpu.SCHEDULE_FIFO)])
 def test_jobs_in_a_mix_equal_jobs_alone(scenes, cfg):
     alone = {k: _serve(scenes, JOBS, [k], path_slots=1 << 14)[k] for k in range(len(JOBS))}
     order = [3, 0, 5, 1, 4, 2]
