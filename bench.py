@@ -141,11 +141,13 @@ def bench_traversal(gpu, host):
         hd = gpu.DeviceArray(len(rays) * HIT_DTYPE.itemsize)
         best = None
         for block, depth in ((64, 16), (128, 16), (128, 24), (256, 16)):
-            r.trace_config(rd.ptr, len(rays), hd.ptr, anyhit=anyhit, block=block, stack_depth=depth, persistent=True)
-            ms = min(r.trace_config(rd.ptr, len(rays), hd.ptr, anyhit=anyhit, block=block, stack_depth=depth, persistent=True) for _ in range(3))
-            if best is None or ms < best[0]:
-                best = (ms, block, depth)
-        res[name] = dict(mrays_s=len(rays) / (best[0] * 1e-3) / 1e6, ms=best[0], block=best[1], stack_depth=best[2])
+            for top in (0, 256, 512):
+                kw = dict(anyhit=anyhit, block=block, stack_depth=depth, persistent=True, lds_nodes=top)
+                r.trace_config(rd.ptr, len(rays), hd.ptr, **kw)
+                ms = min(r.trace_config(rd.ptr, len(rays), hd.ptr, **kw) for _ in range(3))
+                if best is None or ms < best[0]:
+                    best = (ms, block, depth, top)
+        res[name] = dict(mrays_s=len(rays) / (best[0] * 1e-3) / 1e6, ms=best[0], block=best[1], stack_depth=best[2], lds_nodes=best[3])
         log(f"traversal {name}: {res[name]}")
     r.close()
     return res

@@ -58,6 +58,7 @@ typedef struct ph_wavefront_config {
     int32_t use_graph;     /* replay the bounce loop from a captured HIP graph */
     int32_t graph_iterations; /* bounces per captured graph */
     int32_t refill_lanes;  /* a traversal wave takes new rays once this many of its 64 lanes are free */
+    int32_t lds_nodes;     /* top BVH nodes (breadth-first) each traversal block stages in LDS */
 } ph_wavefront_config;
 
 /* Statistics of the last render call. After render_denoised, ms_shade is the one-sample G-buffer trace
@@ -135,10 +136,11 @@ const float* ph_renderer_buffer(const ph_renderer* r, int which);
 /* Traversal of a ray buffer against the uploaded scene: closest hit (t, tri, u, v) or any hit
  * (t, tri). `counts_dev` (optional, 3 uint32 per ray): nodes visited, triangles tested, alpha tests. */
 int ph_trace(ph_renderer* r, const ph_ray* rays_dev, int n, int anyhit, ph_hit* hits_dev, uint32_t* counts_dev);
-/* The same as ph_trace with an explicit launch shape (block 64/128/256, stack depth 8..64), timed with
- * HIP events on the library's stream; ms may be NULL. */
+/* The same as ph_trace with an explicit launch shape (block 64/128/256, stack depth 8..64; a persistent
+ * grid stages the top lds_nodes BVH nodes in each block's LDS), timed with HIP events on the library's
+ * stream; ms may be NULL. */
 int ph_trace_config(ph_renderer* r, const ph_ray* rays_dev, int n, int anyhit, ph_hit* hits_dev, int block, int stack_depth, int persistent,
-                    float* ms);
+                    int lds_nodes, float* ms);
 
 /* SVGF on caller G-buffers (the reference's Svgf class). */
 typedef struct ph_svgf ph_svgf;

@@ -94,14 +94,18 @@ int or_render(const ph_scene_desc* s, const ph_camera_view* cam, const ph_render
 int or_render_samples(const ph_scene_desc* s, const ph_camera_view* cam, const ph_render_settings* rs, int n, const int32_t* xs, const int32_t* ys,
                       const uint32_t* frames, float* rgb, uint32_t* ray_counts) {
     if (!s || !cam || !rs || n < 0) return -1;
-    PathOpts opt = opts_of(rs);
-    for (int i = 0; i < n; i++) {
-        uint32_t rays = 0;
-        opt.ray_count = ray_counts ? &rays : nullptr;
-        V3 L = pixel_sample(*s, *cam, xs[i], ys[i], rs->width, rs->height, frames[i], rs->seed, opt, nullptr, nullptr);
-        if (rgb) put3(rgb + 3 * i, L);
-        if (ray_counts) ray_counts[i] = rays;
-    }
+    // Entries are independent; chunks of them run on the host threads.
+    const int chunk = 64;
+    parallel_rows((n + chunk - 1) / chunk, 0, [&](int c) {
+        PathOpts opt = opts_of(rs);
+        for (int i = c * chunk; i < std::min(n, (c + 1) * chunk); i++) {
+            uint32_t rays = 0;
+            opt.ray_count = ray_counts ? &rays : nullptr;
+            V3 L = pixel_sample(*s, *cam, xs[i], ys[i], rs->width, rs->height, frames[i], rs->seed, opt, nullptr, nullptr);
+            if (rgb) put3(rgb + 3 * i, L);
+            if (ray_counts) ray_counts[i] = rays;
+        }
+    });
     return 0;
 }
 

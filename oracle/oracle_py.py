@@ -115,6 +115,20 @@ def render(desc, cam, rs, spp, frame0=0, threads=0, gbuffer=False):
     return img
 
 
+def render_samples(desc, cam, rs, xs, ys, frames):
+    """Radiance (n, 3) of single samples at pixels (xs, ys) with the given frame indices, and the rays
+    each path traced."""
+    xs = np.ascontiguousarray(xs, np.int32)
+    ys = np.ascontiguousarray(ys, np.int32)
+    fr = np.ascontiguousarray(frames, np.uint32)
+    n = len(xs)
+    rgb = np.zeros(n * 3, np.float32)
+    rays = np.zeros(n, np.uint32)
+    rc = oracle().or_render_samples(C.byref(desc), C.byref(cam), C.byref(rs), n, vptr(xs), vptr(ys), vptr(fr), fptr(rgb), vptr(rays))
+    assert rc == 0
+    return rgb.reshape(n, 3), rays
+
+
 def render_sum(desc, cam, rs, spp, frame0=0, threads=0):
     s = np.zeros(rs.width * rs.height * 3, np.float32)
     assert oracle().or_render(C.byref(desc), C.byref(cam), C.byref(rs), spp, frame0, threads, fptr(s), None, None) == 0

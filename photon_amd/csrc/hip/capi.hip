@@ -68,7 +68,6 @@ struct ph_renderer {
     DevBuf<float4> accum, color, albedo, nd, position, emission, denoised_buf;
     DevBuf<float> heat;
     DevBuf<uint32_t> display;
-    DevBuf<int> work_counter;
     ph_camera_view last_camera{};
     SvgfState svgf;
 
@@ -82,7 +81,6 @@ struct ph_renderer {
         PH_CHECK(hipEventCreate(&ev0));
         PH_CHECK(hipEventCreate(&ev1));
         PH_CHECK(hipEventCreate(&ev_mid));
-        work_counter.allocate(1);
         wavefront = std::make_unique<WavefrontEngine>();
     }
     ~ph_renderer() {
@@ -469,10 +467,12 @@ const float* ph_renderer_buffer(const ph_renderer* r, int which) {
     return b[which]->ptr ? &b[which]->ptr->x : nullptr;
 }
 
-int ph_trace_config(ph_renderer* r, const ph_ray* rays, int n, int anyhit, ph_hit* hits, int block, int depth, int persistent, float* ms) {
+int ph_trace_config(ph_renderer* r, const ph_ray* rays, int n, int anyhit, ph_hit* hits, int block, int depth, int persistent, int lds_nodes,
+                    float* ms) {
     if (!r || (n > 0 && (!rays || !hits))) return fail("null argument");
     if (n < 0) return fail("negative ray count");
     if (!r->uploaded) return fail("no scene uploaded");
+    if (lds_nodes < 0 || size_t(block) * size_t(depth) * 4 + size_t(lds_nodes) * 64 > 65536) return fail("lds_nodes does not fit the block's LDS");
     return guarded([&] {
         ph_renderer& m = *r;
         int grid = 0;
@@ -481,10 +481,9 @@ int ph_trace_config(ph_renderer* r, const ph_ray* rays, int n, int anyhit, ph_hi
             PH_CHECK(hipGetDevice(&dev));
             PH_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
             grid = cus * (2048 / block);
-            PH_CHECK(hipMemsetAsync(m.work_counter.ptr, 0, sizeof(int), m.stream));
         }
         PH_CHECK(hipEventRecord(m.ev0, m.stream));
-        if (!launch_trace(m.stream, m.view, rays, n, anyhit != 0, hits, nullptr, block, depth, persistent != 0, m.work_counter.ptr, grid))
+        if (!launch_trace(m.stream, m.view, rays, n, anyhit != 0, hits, nullptr, block, depth, persistent != 0, lds_nodes, grid))
             return fail("unsupported traversal launch shape");
         PH_CHECK(hipEventRecord(m.ev1, m.stream));
         PH_CHECK(hipEventSynchronize(m.ev1));
@@ -501,7 +500,7 @@ int ph_trace(ph_renderer* r, const ph_ray* rays, int n, int anyhit, ph_hit* hits
     if (!r->uploaded) return fail("no scene uploaded");
     return guarded([&] {
         ph_renderer& m = *r;
-        if (!launch_trace(m.stream, m.view, rays, n, anyhit != 0, hits, counts, 128, 32, false, m.work_counter.ptr, 0))
+        if (!launch_trace(m.stream, m.view, rays, n, anyhit != 0, hits, counts, 128, 32, false, 0, 0))
             return fail("unsupported traversal launch shape");
         PH_CHECK(hipStreamSynchronize(m.stream));
         return 0;

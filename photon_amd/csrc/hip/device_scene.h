@@ -4,13 +4,53 @@
 
 #include <cmath>
 #include <cstring>
+#include <deque>
+#include <vector>
 
 #include "internal.h"
 
 namespace phk {
 
+// Node relabelling for the device copy of the BVH: the first `top` nodes in breadth-first order from the
+// root come first, the others follow in their original order (so each deeper subtree stays contiguous).
+// Child links are renumbered; leaves keep their triangle ranges. The tree, the traversal order and every
+// hit are unchanged; the top levels, which every ray visits, form a prefix that kernels can stage in LDS.
+inline std::vector<ph_bvh_node> relabel_top_first(const ph_bvh_node* nodes, int n, int top, int& top_count) {
+    std::vector<int> order;
+    order.reserve(static_cast<size_t>(n));
+    std::vector<int> index(static_cast<size_t>(n), -1);
+    std::deque<int> bfs{0};
+    while (!bfs.empty() && int(order.size()) < top) {
+        const int i = bfs.front();
+        bfs.pop_front();
+        if (i < 0 || i >= n || index[size_t(i)] >= 0) continue;
+        index[size_t(i)] = int(order.size());
+        order.push_back(i);
+        const ph_bvh_node& nd = nodes[i];
+        if (nd.lcount == 0 && nd.left >= 0) bfs.push_back(nd.left);
+        if (nd.rcount == 0 && nd.right >= 0) bfs.push_back(nd.right);
+    }
+    top_count = int(order.size());
+    for (int i = 0; i < n; i++)
+        if (index[size_t(i)] < 0) {
+            index[size_t(i)] = int(order.size());
+            order.push_back(i);
+        }
+    std::vector<ph_bvh_node> out(static_cast<size_t>(n));
+    for (int k = 0; k < n; k++) {
+        ph_bvh_node nd = nodes[order[size_t(k)]];
+        if (nd.lcount == 0 && nd.left >= 0) nd.left = index[size_t(nd.left)];
+        if (nd.rcount == 0 && nd.right >= 0) nd.right = index[size_t(nd.right)];
+        out[size_t(k)] = nd;
+    }
+    return out;
+}
+
 class DeviceScene {
 public:
+    // Nodes relabelled to the front of the device BVH (at most this many can be staged in LDS).
+    static constexpr int kTopNodes = 4096;
+
     const phd::DScene& view() const { return view_; }
     float extent() const { return extent_; }
     bool ready() const { return ready_; }
@@ -38,7 +78,10 @@ public:
     // Copies every array (GpuRenderer::upload, gpu_renderer.cu:243-297). Synchronous.
     void upload(const ph_scene_desc& d, hipStream_t st) {
         ready_ = false;
-        nodes_.upload(d.nodes, size_t(d.num_nodes) * 4, st);
+        int top = 0;
+        const std::vector<ph_bvh_node> relabelled = relabel_top_first(d.nodes, d.num_nodes, kTopNodes, top);
+        nodes_.upload(relabelled.data(), size_t(d.num_nodes) * 4, st);
+        PH_CHECK(hipStreamSynchronize(st));  // the host copy above goes out of scope
         indices_.upload(d.indices, size_t(d.num_tris) * 3, st);
         normals_.upload(d.normals, d.normals ? size_t(d.num_vertices) * 3 : 0, st);
         uvs_.upload(d.uvs, d.uvs ? size_t(d.num_vertices) * 2 : 0, st);
@@ -60,6 +103,7 @@ public:
         v.punctual = or_null(punctual_);
         v.num_tris = d.num_tris;
         v.num_nodes = d.num_nodes;
+        v.top_nodes = top;
         v.num_punctual = d.num_punctual;
         v.num_textures = d.num_textures;
         float dx = d.bounds_max[0] - d.bounds_min[0], dy = d.bounds_max[1] - d.bounds_min[1], dz = d.bounds_max[2] - d.bounds_min[2];

@@ -34,6 +34,7 @@ struct DScene {
     const float* env_marginal;
     const float* env_conditional;
     int num_tris, num_nodes, num_emissive, num_punctual;
+    int top_nodes;  // nodes [0, top_nodes) are the top of the tree in breadth-first order
     int env_w, env_h;
     float env_intensity, env_rotation, env_integral;
     float env_color[3];

@@ -96,15 +96,23 @@ PH_DEV bool alpha_cut(const DScene& s, int tri, int material, float u, float v) 
     return alpha < cutoff;
 }
 
-// What traversal reads of a scene, held in registers: the node and triangle arrays, and the whole
-// scene for the (rare) alpha tests.
+// What traversal reads of a scene, held in registers: the node and triangle arrays, the whole scene
+// for the (rare) alpha tests, and optionally the top `top_n` nodes staged in LDS by the block.
 struct TraceScene {
     const float4* nodes;
     const float4* tris;
     const DScene* full;
+    const float4* top;
+    int top_n;
 };
 
-PH_DEV TraceScene trace_scene(const DScene& s) { return TraceScene{s.nodes, s.tris, &s}; }
+PH_DEV TraceScene trace_scene(const DScene& s) { return TraceScene{s.nodes, s.tris, &s, nullptr, 0}; }
+
+// Copies nodes [0, n) into the block's LDS array `top` (4 float4 per node). All threads call.
+PH_DEV void stage_top_nodes(float4* top, const float4* nodes, int n) {
+    for (int i = threadIdx.x; i < 4 * n; i += blockDim.x) top[i] = ldg(nodes + i);
+    __syncthreads();
+}
 
 // One ray's traversal position: walk_step visits one node at a time, so that a lane can start a new
 // ray as soon as its own finishes (the persistent kernels refill lanes between steps).
@@ -122,8 +130,20 @@ PH_DEV Walker walker(f3 o, f3 d, float tmax) { return Walker{o, d, guarded_recip
 // Returns false once the traversal is over (any-hit: at the first accepted triangle).
 template <bool AnyHit, bool Count, class Stack>
 PH_DEV bool walk_step(const TraceScene& ts, Walker& w, HitRec& hit, Stack& stack, TraceCount* cnt) {
-    const float4* np = ts.nodes + 4 * w.node;
-    const float4 lmin = ldg(np), lmax = ldg(np + 1), rmin = ldg(np + 2), rmax = ldg(np + 3);
+    float4 lmin, lmax, rmin, rmax;
+    if (w.node < ts.top_n) {
+        const float4* np = ts.top + 4 * w.node;
+        lmin = np[0];
+        lmax = np[1];
+        rmin = np[2];
+        rmax = np[3];
+    } else {
+        const float4* np = ts.nodes + 4 * w.node;
+        lmin = ldg(np);
+        lmax = ldg(np + 1);
+        rmin = ldg(np + 2);
+        rmax = ldg(np + 3);
+    }
     if (Count) cnt->nodes++;
     const float tl = slab(lmin, lmax, w.o, w.inv, w.tmax);
     const float tr = slab(rmin, rmax, w.o, w.inv, w.tmax);
@@ -188,6 +208,70 @@ PH_DEV bool walk_step(const TraceScene& ts, Walker& w, HitRec& hit, Stack& stack
     if (w.sp == 0) return false;
     w.node = stack.load(--w.sp);
     return true;
+}
+
+// Persistent work distribution without atomics (atomics on one address serialize). Workgroups are
+// dispatched to the 8 XCDs in turn (block b runs on XCD b % 8), and each XCD has its own L2: a list of
+// n entries is cut into 8 contiguous parts, one per XCD, so the waves that share an L2 work on
+// neighbouring entries (neighbouring pixels, hence nearby rays and surfaces). Inside its part, an XCD's
+// waves take the 64-entry batches in turn: wave k of the XCD takes batches k, k + W, k + 2W, ... where
+// W counts the XCD's waves.
+struct WaveShare {
+    int first;    // first batch of this XCD's part
+    int batches;  // batches in the part
+    int wave;     // this wave's rank among the XCD's waves
+    int waves;    // waves on the XCD
+};
+
+PH_DEV WaveShare wave_share(int n) {
+    const int parts = gridDim.x >= 8 ? 8 : 1;
+    const int wpb = blockDim.x >> 6;
+    const int part = blockIdx.x % parts;
+    const int blocks_here = (int(gridDim.x) - part + parts - 1) / parts;
+    const int total = (n + 63) / 64;
+    const int per = (total + parts - 1) / parts;
+    const int first = min(total, part * per), last = min(total, first + per);
+    return WaveShare{first, last - first, int(blockIdx.x / parts) * wpb + int(threadIdx.x >> 6), blocks_here * wpb};
+}
+
+// Persistent traversal of entries [0, n) with lane refill. A lane whose ray is finished takes the next
+// entry of its wave's share as soon as at least `refill` lanes of the wave are free (or all are), so a
+// wave is not held by its slowest ray; the wave leaves when its share is drained and none of its lanes
+// has a ray. Each ray is walked exactly as traverse() walks it: only the lane it runs on and the moment
+// it starts change. `load(i, ts, w, hit)` starts entry i (and may set the lane's scene);
+// `finish(ts, w, hit)` stores its result.
+template <bool AnyHit, bool Count, class Stack, class Load, class Finish>
+PH_DEV void trace_stream(int n, int refill, TraceScene ts, Stack& stack, TraceCount& cnt, Load load, Finish finish) {
+    const WaveShare ws = wave_share(n);
+    int pos = 0;  // entries of this wave's share already handed to lanes
+    bool drained = ws.wave >= ws.batches;
+    bool live = false;
+    Walker w{};
+    HitRec hit{};
+    for (;;) {
+        if (!drained) {
+            const uint64_t need = __ballot(!live);
+            const int k = __popcll(need);
+            if (k >= refill || k == 64) {
+                if (!live) {
+                    const int j = pos + __popcll(need & ((1ull << __lane_id()) - 1ull));
+                    const int b = ws.wave + (j >> 6) * ws.waves;
+                    const int i = (ws.first + b) * 64 + (j & 63);
+                    if (b < ws.batches && i < n) {
+                        load(i, ts, w, hit);
+                        live = true;
+                    }
+                }
+                pos += k;
+                if (ws.wave + (pos >> 6) * ws.waves >= ws.batches) drained = true;
+            }
+        }
+        if (__ballot(live) == 0) break;
+        if (live && !walk_step<AnyHit, Count>(ts, w, hit, stack, &cnt)) {
+            finish(ts, w, hit);
+            live = false;
+        }
+    }
 }
 
 template <bool AnyHit, bool Count, class Stack>

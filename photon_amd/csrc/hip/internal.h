@@ -94,7 +94,8 @@ void launch_heatmap(hipStream_t st, const phd::DScene& s, const ph_camera_view& 
 void launch_display(hipStream_t st, const DisplayParams& p, const float4* accum, const float4* denoised, const float4* albedo, const float4* nd,
                     const float4* variance, const float* heat, uint32_t* out);
 // Returns false when (block, depth) is not an instantiated shape.
+// Persistent launches stage the top `lds_nodes` BVH nodes in each block's LDS.
 bool launch_trace(hipStream_t st, const phd::DScene& s, const ph_ray* rays, int n, bool anyhit, ph_hit* hits, uint32_t* counts, int block, int depth,
-                  bool persistent, int* work_counter, int grid_blocks);
+                  bool persistent, int lds_nodes, int grid_blocks);
 
 }  // namespace phk

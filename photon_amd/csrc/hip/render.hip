@@ -145,21 +145,30 @@ struct Tag {
 };
 
 template <int Block, int Depth, bool AnyHit, bool Count, bool Persistent>
-__global__ __launch_bounds__(Block) void trace_kernel(DScene s, const ph_ray* rays, int n, ph_hit* hits, uint32_t* counts, int* work) {
+__global__ __launch_bounds__(Block) void trace_kernel(DScene s, const ph_ray* rays, int n, ph_hit* hits, uint32_t* counts, int lds_nodes) {
     __shared__ int lds[Depth * Block];
+    extern __shared__ float4 top_nodes[];
     LaneStack<Depth> stack{lds + threadIdx.x, Block};
     if (Persistent) {
-        // Waves pull batches of 64 rays from a global counter until it passes n: uneven rays do not
-        // leave a wave idle while a whole block drains.
-        const int lane = threadIdx.x & 63;
-        for (;;) {
-            int base = 0;
-            if (lane == 0) base = atomicAdd(work, 64);
-            base = __shfl(base, 0);
-            if (base >= n) break;
-            const int i = base + lane;
-            if (i < n) trace_one(s, rays, i, AnyHit, hits, counts, stack, Tag<Count>{}, Tag<AnyHit>{});
-        }
+        // A fixed grid; each wave refills its lanes from its share of the rays (trace_stream), with the
+        // top of the tree staged in LDS.
+        TraceScene ts = trace_scene(s);
+        ts.top = top_nodes;
+        ts.top_n = min(lds_nodes, s.top_nodes);
+        stage_top_nodes(top_nodes, s.nodes, ts.top_n);
+        TraceCount c{0, 0, 0};
+        int idx = 0;  // this lane's ray
+        trace_stream<AnyHit, false>(
+            n, 16, ts, stack, c,
+            [&](int i, TraceScene&, Walker& w, HitRec& hit) {
+                idx = i;
+                const float* r = reinterpret_cast<const float*>(rays + i);
+                w = walker(mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), r[6]);
+                hit = HitRec{kInf, -1, 0.0f, 0.0f};
+            },
+            [&](const TraceScene&, const Walker&, const HitRec& hit) {
+                reinterpret_cast<float4*>(hits)[idx] = make_float4(hit.t, __int_as_float(hit.tri), hit.u, hit.v);
+            });
     } else {
         const int i = blockIdx.x * Block + threadIdx.x;
         if (i < n) trace_one(s, rays, i, AnyHit, hits, counts, stack, Tag<Count>{}, Tag<AnyHit>{});
@@ -168,10 +177,12 @@ __global__ __launch_bounds__(Block) void trace_kernel(DScene s, const ph_ray* ra
 
 template <int Block, int Depth>
 void trace_shape(hipStream_t st, const DScene& s, const ph_ray* rays, int n, bool anyhit, ph_hit* hits, uint32_t* counts, bool persistent,
-                 int* work, int grid_blocks) {
+                 int lds_nodes, int grid_blocks) {
     const int blocks = persistent ? grid_blocks : (n + Block - 1) / Block;
     if (blocks <= 0) return;
-#define PH_TRACE_LAUNCH(A, C, P) hipLaunchKernelGGL((trace_kernel<Block, Depth, A, C, P>), dim3(blocks), dim3(Block), 0, st, s, rays, n, hits, counts, work)
+    const int top = persistent && !counts ? lds_nodes : 0;
+#define PH_TRACE_LAUNCH(A, C, P) \
+    hipLaunchKernelGGL((trace_kernel<Block, Depth, A, C, P>), dim3(blocks), dim3(Block), size_t(top) * 64, st, s, rays, n, hits, counts, top)
     if (counts) {
         if (anyhit)
             PH_TRACE_LAUNCH(true, true, false);
@@ -221,10 +232,10 @@ void launch_display(hipStream_t st, const DisplayParams& p, const float4* accum,
 }
 
 bool launch_trace(hipStream_t st, const DScene& s, const ph_ray* rays, int n, bool anyhit, ph_hit* hits, uint32_t* counts, int block, int depth,
-                  bool persistent, int* work, int grid_blocks) {
-#define PH_SHAPE(B, D)                                                                      \
-    if (block == B && depth == D) {                                                         \
-        trace_shape<B, D>(st, s, rays, n, anyhit, hits, counts, persistent, work, grid_blocks); \
+                  bool persistent, int lds_nodes, int grid_blocks) {
+#define PH_SHAPE(B, D)                                                                           \
+    if (block == B && depth == D) {                                                              \
+        trace_shape<B, D>(st, s, rays, n, anyhit, hits, counts, persistent, lds_nodes, grid_blocks); \
         PH_CHECK(hipGetLastError());                                                        \
         return true;                                                                        \
     }

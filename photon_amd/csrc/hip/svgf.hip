@@ -17,6 +17,22 @@ namespace {
 
 constexpr int TX = 16, TY = 16;
 
+// Edge-stopping weights. The denoiser's tolerance is 1e-3 relative, so they use the hardware's
+// single-precision exp2/log2 (a few ulp) instead of correctly rounded exp/pow; a power-of-two normal
+// exponent (the default 128) is evaluated by exact repeated squaring.
+PH_DEV float weight_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504f); }
+
+PH_DEV float weight_pow(float x, float e) {
+    if (x <= 0.0f) return e > 0.0f ? 0.0f : 1.0f;
+    const int ie = int(e);
+    if (float(ie) == e && ie > 0 && ie <= 1024 && (ie & (ie - 1)) == 0) {
+        float r = x;
+        for (int k = ie; k > 1; k >>= 1) r *= r;
+        return r;
+    }
+    return __builtin_amdgcn_exp2f(e * __builtin_amdgcn_logf(x));
+}
+
 PH_DEV f3 demod(f3 color, f3 emission, f3 albedo) {
     f3 illum = vmax3(color - emission, mk3(0));
     return mk3(illum.x / fmaxf(albedo.x, 1e-3f), illum.y / fmaxf(albedo.y, 1e-3f), illum.z / fmaxf(albedo.z, 1e-3f));
@@ -139,8 +155,8 @@ __global__ __launch_bounds__(TX* TY) void variance_kernel(Planes in, int w, int 
                 const int ly = threadIdx.y + R + dy, lx = threadIdx.x + R + dx;
                 if (s_mat[ly][lx] != mat) continue;
                 const float4 nd = s_nd[ly][lx];
-                const float wn = pow_cr(fmaxf(0.0f, dot(n, xyz(nd))), phi_normal);
-                const float wz = exp_cr(-fabsf(nd.w - z) / (0.05f * z + 1e-3f));
+                const float wn = weight_pow(fmaxf(0.0f, dot(n, xyz(nd))), phi_normal);
+                const float wz = weight_exp(-fabsf(nd.w - z) / (0.05f * z + 1e-3f));
                 const float wt = wn * wz;
                 const int j = yy * w + xx;
                 const float4 m = moments[j];
@@ -234,10 +250,10 @@ __global__ __launch_bounds__(TX* TY) void atrous_kernel(Planes in, int w, int h,
             const float4 sv = SRC(xx, yy);
             const float4 nd = ND(xx, yy);
             const float k = kernel[dx < 0 ? -dx : dx] * kernel[dy < 0 ? -dy : dy];
-            const float wl = exp_cr(-fabsf(lum(xyz(sv)) - lum_c) / sigma_l);
-            const float wn = pow_cr(fmaxf(0.0f, dot(n, xyz(nd))), phi_normal);
+            const float wl = weight_exp(-fabsf(lum(xyz(sv)) - lum_c) / sigma_l);
+            const float wn = weight_pow(fmaxf(0.0f, dot(n, xyz(nd))), phi_normal);
             const float dist = __builtin_sqrtf(float(dx * dx + dy * dy)) * float(step);
-            const float wz = exp_cr(-fabsf(nd.w - z) / (phi_depth * grad * dist + 1e-3f * z + 1e-4f));
+            const float wz = weight_exp(-fabsf(nd.w - z) / (phi_depth * grad * dist + 1e-3f * z + 1e-4f));
             const float wt = k * wl * wn * wz;
             sum = sum + xyz(sv) * wt;
             vsum += wt * wt * sv.w;

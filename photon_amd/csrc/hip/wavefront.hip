@@ -38,6 +38,7 @@
 #include <map>
 #include <tuple>
 
+#include "device_scene.h"
 #include "wavefront.h"
 
 using namespace phd;
@@ -148,11 +149,6 @@ PH_DEV long long wave_take(unsigned long long* counter, bool pred) {
 PH_DEV long long items_next(const Work& work) {
     return (long long)__hip_atomic_load(work.items, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-
-// Persistent work distribution without atomics (atomics on one address serialize): the grid's waves
-// take the queue's 64-entry batches in turn, wave w taking batches w, w + W, w + 2W, ...
-PH_DEV int grid_wave() { return blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); }
-PH_DEV int grid_waves() { return gridDim.x * (blockDim.x >> 6); }
 
 // Work item -> pixel of a w x h image: 8-row bands, 8-column tiles inside a band (the last tile and band
 // may be narrower), row-major inside a tile, so a wave's 64 lanes start on one 8x8 pixel block.
@@ -379,66 +375,40 @@ __global__ __launch_bounds__(kCompactThreads) void compact_scatter_kernel(const 
         for (int k = 0; k < K; k++) counters[Compact<Kind>::counter + k] = chunk_tot[k];
 }
 
-// Persistent traversal of a queue with lane refill. A lane whose ray is finished takes the next queue
-// entry as soon as at least `refill` lanes of its wave are free (or all of them are), with one atomic
-// per refill, so a wave is not held by its slowest ray. The wave leaves when the queue is drained and
-// none of its lanes has a ray. Each ray is walked exactly as traverse() walks it; only the lane it runs
-// on and the moment it starts change.
+// Traversal of a queue of slots (trace_stream). One job: every lane shares the scene, whose traversal
+// arrays sit in scalar registers and whose top `lds_nodes` nodes the block stages in LDS. Serving: each
+// lane reads its slot's scene.
 template <bool AnyHit, bool Count, bool Multi, class Stack, class Load, class Finish>
-PH_DEV void trace_queue(const Work& work, const State& st, const int* queue, int n, int refill, Stack& stack, TraceCount& cnt, Load load,
-                        Finish finish) {
-    // This wave's entries, in order: batches wave, wave + W, ... of 64 entries; `pos` counts the
-    // entries already handed to lanes.
-    const int wave = grid_wave(), W = grid_waves();
-    int pos = 0;
-    bool live = false, drained = false;
-    int slot = 0, entry = 0;
-    Walker w{};
-    HitRec hit{};
-    TraceScene ts{};
+PH_DEV void trace_queue(const Work& work, const State& st, const int* queue, int n, int refill, int lds_nodes, float4* top, Stack& stack,
+                        TraceCount& cnt, Load load, Finish finish) {
+    TraceScene ts0{};
     if (!Multi) {
-        // One job: its traversal arrays are the same for every lane; keep them in scalar registers.
         const JobParams j0 = single_job<false>(work);
-        ts = TraceScene{uniform_ptr(j0.scene.nodes), uniform_ptr(j0.scene.tris), &work.jobs[0].scene};
+        ts0 = TraceScene{uniform_ptr(j0.scene.nodes), uniform_ptr(j0.scene.tris), &work.jobs[0].scene, top, min(lds_nodes, j0.scene.top_nodes)};
+        stage_top_nodes(top, ts0.nodes, ts0.top_n);
     }
-    for (;;) {
-        if (!drained) {
-            const uint64_t need = __ballot(!live);
-            const int k = __popcll(need);
-            if (k >= refill || k == 64) {
-                const int lane = lane_id();
-                if (!live) {
-                    const int j = pos + __popcll(need & ((1ull << lane) - 1ull));
-                    const int i = (wave + (j >> 6) * W) * 64 + (j & 63);
-                    if (i < n) {
-                        slot = queue[i];
-                        if (Multi) {
-                            entry = st.job[slot];
-                            ts = trace_scene(work.jobs[entry].scene);
-                        }
-                        load(slot, w, hit);
-                        live = true;
-                    }
-                }
-                pos += k;
-                if ((wave + (pos >> 6) * W) * 64 >= n) drained = true;
+    int slot = 0, entry = 0;
+    trace_stream<AnyHit, Count>(
+        n, refill, ts0, stack, cnt,
+        [&](int i, TraceScene& ts, Walker& w, HitRec& hit) {
+            slot = queue[i];
+            if (Multi) {
+                entry = st.job[slot];
+                ts = trace_scene(work.jobs[entry].scene);
             }
-        }
-        if (__ballot(live) == 0) break;
-        if (live && !walk_step<AnyHit, Count>(ts, w, hit, stack, &cnt)) {
-            finish(slot, entry, ts, w, hit);
-            live = false;
-        }
-    }
+            load(slot, w, hit);
+        },
+        [&](const TraceScene& ts, const Walker& w, const HitRec& hit) { finish(slot, entry, ts, w, hit); });
 }
 
 template <int Block, int Depth, bool Count, bool Multi>
-__global__ __launch_bounds__(Block) void extend_kernel(Work work, State st, int* counters, int sort, int refill, TraceTotals* totals) {
+__global__ __launch_bounds__(Block) void extend_kernel(Work work, State st, int* counters, int sort, int refill, int lds_nodes, TraceTotals* totals) {
     __shared__ int lds[Depth * Block];
+    extern __shared__ float4 top_nodes[];
     LaneStack<Depth> stack{lds + threadIdx.x, Block};
     TraceCount cnt{0, 0, 0};
     trace_queue<false, Count, Multi>(
-        work, st, st.q_ext, counters[C_EXT], refill, stack, cnt,
+        work, st, st.q_ext, counters[C_EXT], refill, lds_nodes, top_nodes, stack, cnt,
         [&](int slot, Walker& w, HitRec& hit) {
             const float4 o4 = st.ro[slot], d4 = st.rd[slot];
             w = walker(xyz(o4), xyz(d4), kInf);
@@ -457,12 +427,13 @@ __global__ __launch_bounds__(Block) void extend_kernel(Work work, State st, int*
 }
 
 template <int Block, int Depth, bool Count, bool Multi>
-__global__ __launch_bounds__(Block) void shadow_kernel(Work work, State st, int* counters, int refill, TraceTotals* totals) {
+__global__ __launch_bounds__(Block) void shadow_kernel(Work work, State st, int* counters, int refill, int lds_nodes, TraceTotals* totals) {
     __shared__ int lds[Depth * Block];
+    extern __shared__ float4 top_nodes[];
     LaneStack<Depth> stack{lds + threadIdx.x, Block};
     TraceCount cnt{0, 0, 0};
     trace_queue<true, Count, Multi>(
-        work, st, st.q_shadow, counters[C_SHADOW], refill, stack, cnt,
+        work, st, st.q_shadow, counters[C_SHADOW], refill, lds_nodes, top_nodes, stack, cnt,
         [&](int slot, Walker& w, HitRec& hit) {
             const float4 o4 = st.so[slot], d4 = st.sd[slot];
             w = walker(xyz(o4), xyz(d4), o4.w);
@@ -484,8 +455,9 @@ template <bool Multi>
 __global__ __launch_bounds__(256) void miss_kernel(Work work, State st, int* counters) {
     const int n = counters[C_MISS];
     const JobParams single = single_job<Multi>(work);
-    for (int base = grid_wave() * 64; base < n; base += grid_waves() * 64) {
-        const int i = base + lane_id();
+    const WaveShare ws = wave_share(n);
+    for (int b = ws.wave; b < ws.batches; b += ws.waves) {
+        const int i = (ws.first + b) * 64 + lane_id();
         if (i < n) {
             const int slot = st.q_miss[i];
             const JobParams& jp = job_of<Multi>(work, st, slot, single);
@@ -517,8 +489,9 @@ __global__ __launch_bounds__(256) void shade_kernel(Work work, State st, int* co
     const int n = counters[C_CLASS + cls];
     const int* queue = st.q_class[cls];
     const JobParams single = single_job<Multi>(work);
-    for (int base = grid_wave() * 64; base < n; base += grid_waves() * 64) {
-        const int i = base + lane_id();
+    const WaveShare ws = wave_share(n);
+    for (int b = ws.wave; b < ws.batches; b += ws.waves) {
+        const int i = (ws.first + b) * 64 + lane_id();
         if (i < n) {
             const int slot = queue[i];
             bool cont = false, shadow = false;
@@ -619,8 +592,9 @@ __global__ __launch_bounds__(256) void regen_kernel(Work work, State st, int* co
     const long long total = (long long)work.items[1];
     const int chunks = counters[C_CHUNKS];
     const JobParams single = single_job<Multi>(work);
-    for (int base = grid_wave() * 64; base < n; base += grid_waves() * 64) {
-        const int i = base + lane_id();
+    const WaveShare ws = wave_share(n);
+    for (int b = ws.wave; b < ws.batches; b += ws.waves) {
+        const int i = (ws.first + b) * 64 + lane_id();
         const bool active = i < n;
         const int slot = active ? st.q_term[i] : 0;
         bool need_work = false, ext = false;
@@ -664,22 +638,26 @@ int device_cus() {
     return cus;
 }
 
-using TraceLaunch = void (*)(hipStream_t, int, Work, State, int*, int, int, TraceTotals*, bool);
+using TraceLaunch = void (*)(hipStream_t, int, Work, State, int*, int, int, int, TraceTotals*, bool);
 
 template <int Block, int Depth, bool Multi>
-void launch_extend(hipStream_t st, int grid, Work w, State s, int* counters, int sort, int refill, TraceTotals* tot, bool count) {
+void launch_extend(hipStream_t st, int grid, Work w, State s, int* counters, int sort, int refill, int lds_nodes, TraceTotals* tot, bool count) {
+    const size_t lds = Multi ? 0 : size_t(lds_nodes) * 64;
     if (count)
-        hipLaunchKernelGGL((extend_kernel<Block, Depth, true, Multi>), dim3(grid), dim3(Block), 0, st, w, s, counters, sort, refill, tot);
+        hipLaunchKernelGGL((extend_kernel<Block, Depth, true, Multi>), dim3(grid), dim3(Block), lds, st, w, s, counters, sort, refill, Multi ? 0 : lds_nodes,
+                           tot);
     else
-        hipLaunchKernelGGL((extend_kernel<Block, Depth, false, Multi>), dim3(grid), dim3(Block), 0, st, w, s, counters, sort, refill, tot);
+        hipLaunchKernelGGL((extend_kernel<Block, Depth, false, Multi>), dim3(grid), dim3(Block), lds, st, w, s, counters, sort, refill, Multi ? 0 : lds_nodes,
+                           tot);
 }
 
 template <int Block, int Depth, bool Multi>
-void launch_shadow(hipStream_t st, int grid, Work w, State s, int* counters, int, int refill, TraceTotals* tot, bool count) {
+void launch_shadow(hipStream_t st, int grid, Work w, State s, int* counters, int, int refill, int lds_nodes, TraceTotals* tot, bool count) {
+    const size_t lds = Multi ? 0 : size_t(lds_nodes) * 64;
     if (count)
-        hipLaunchKernelGGL((shadow_kernel<Block, Depth, true, Multi>), dim3(grid), dim3(Block), 0, st, w, s, counters, refill, tot);
+        hipLaunchKernelGGL((shadow_kernel<Block, Depth, true, Multi>), dim3(grid), dim3(Block), lds, st, w, s, counters, refill, Multi ? 0 : lds_nodes, tot);
     else
-        hipLaunchKernelGGL((shadow_kernel<Block, Depth, false, Multi>), dim3(grid), dim3(Block), 0, st, w, s, counters, refill, tot);
+        hipLaunchKernelGGL((shadow_kernel<Block, Depth, false, Multi>), dim3(grid), dim3(Block), lds, st, w, s, counters, refill, Multi ? 0 : lds_nodes, tot);
 }
 
 struct ShapeFns {
@@ -733,7 +711,7 @@ std::vector<int> material_classes(const DScene& scene, hipStream_t st) {
 }
 
 struct WavefrontEngine::Impl {
-    ph_wavefront_config cfg{128, 24, 0, 16, 1, 1, 16, 8};
+    ph_wavefront_config cfg{128, 24, 0, 16, 1, 1, 16, 8, 0};
     int slots = 0;
     int cus = 0;
     DevBuf<int> job, pixel, sample, sample_end, counters, q_ext, q_shadow, q_term, q_miss, q_cls[kClasses], mat_class, partial;
@@ -883,7 +861,7 @@ struct WavefrontEngine::Impl {
         hipLaunchKernelGGL(bookkeep_kernel, dim3(1), dim3(64), 0, st, cnt, ray_totals.ptr);
         if (ev) PH_CHECK(hipEventRecord(ev[0], st));
         const int refill = std::clamp(cfg.refill_lanes, 1, 64);
-        fns.extend(st, tgrid, w, s, cnt, cfg.sort_materials ? 1 : 0, refill, totals.ptr, count);
+        fns.extend(st, tgrid, w, s, cnt, cfg.sort_materials ? 1 : 0, refill, cfg.lds_nodes, totals.ptr, count);
         if (ev) PH_CHECK(hipEventRecord(ev[1], st));
         compact<K_HIT, Multi>(st, s);
         hipLaunchKernelGGL(miss_kernel<Multi>, dim3(sgrid), dim3(256), 0, st, w, s, cnt);
@@ -893,7 +871,7 @@ struct WavefrontEngine::Impl {
         hipLaunchKernelGGL((shade_kernel<7, Multi>), dim3(sgrid), dim3(256), 0, st, w, s, cnt, 3);
         compact<K_DONE, Multi>(st, s);
         if (ev) PH_CHECK(hipEventRecord(ev[2], st));
-        fns.shadow(st, tgrid, w, s, cnt, 0, refill, totals.ptr + 1, count);
+        fns.shadow(st, tgrid, w, s, cnt, 0, refill, cfg.lds_nodes, totals.ptr + 1, count);
         if (ev) PH_CHECK(hipEventRecord(ev[3], st));
         hipLaunchKernelGGL(regen_kernel<Multi>, dim3(sgrid), dim3(256), 0, st, w, s, cnt);
         compact<K_EXT, Multi>(st, s);
@@ -993,6 +971,8 @@ void WavefrontEngine::set_config(const ph_wavefront_config& c) {
     if (c.path_slots < 0 || c.blocks_per_cu <= 0 || c.blocks_per_cu > 64) throw std::invalid_argument("invalid wavefront configuration");
     if (c.graph_iterations < 1 || c.graph_iterations > 256) throw std::invalid_argument("graph_iterations must be in [1, 256]");
     if (c.refill_lanes < 1 || c.refill_lanes > 64) throw std::invalid_argument("refill_lanes must be in [1, 64]");
+    if (c.lds_nodes < 0 || c.lds_nodes > DeviceScene::kTopNodes || size_t(c.block) * size_t(c.stack_depth) * 4 + size_t(c.lds_nodes) * 64 > 65536)
+        throw std::invalid_argument("lds_nodes does not fit the block's LDS with its stacks");
     impl_->cfg = c;
     impl_->drop_graph();
 }
@@ -1099,6 +1079,13 @@ ph_wavefront_config WavefrontEngine::autotune(hipStream_t st, const WavefrontJob
     for (int refill : {4, 16, 32, 64}) {
         ph_wavefront_config c = best;
         c.refill_lanes = refill;
+        double t = time_of(c);
+        if (t < best_ms) best_ms = t, best = c;
+    }
+    for (int top : {0, 128, 256, 512}) {
+        ph_wavefront_config c = best;
+        c.lds_nodes = top;
+        if (size_t(c.block) * size_t(c.stack_depth) * 4 + size_t(top) * 64 > 65536) continue;
         double t = time_of(c);
         if (t < best_ms) best_ms = t, best = c;
     }

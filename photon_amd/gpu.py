@@ -25,7 +25,7 @@ class GpuInfo(C.Structure):
 class WavefrontConfig(C.Structure):
     _fields_ = [("block", C.c_int32), ("stack_depth", C.c_int32), ("path_slots", C.c_int32), ("blocks_per_cu", C.c_int32),
                 ("sort_materials", C.c_int32), ("use_graph", C.c_int32), ("graph_iterations", C.c_int32),
-                ("refill_lanes", C.c_int32)]
+                ("refill_lanes", C.c_int32), ("lds_nodes", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -107,7 +107,7 @@ def lib():
         L.ph_renderer_buffer.argtypes = [R, C.c_int]
         L.ph_renderer_buffer.restype = C.c_void_p
         L.ph_trace.argtypes = [R, P, C.c_int, C.c_int, P, P]
-        L.ph_trace_config.argtypes = [R, P, C.c_int, C.c_int, P, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float)]
+        L.ph_trace_config.argtypes = [R, P, C.c_int, C.c_int, P, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float)]
         L.ph_svgf_create.argtypes = [C.POINTER(P)]
         L.ph_svgf_destroy.argtypes = [P]
         L.ph_svgf_destroy.restype = None
@@ -325,9 +325,9 @@ class GpuRenderer:
     def trace(self, rays_dev, n, hits_dev, anyhit=False, counts_dev=None):
         _check(lib().ph_trace(self._h, rays_dev, n, int(anyhit), hits_dev, counts_dev))
 
-    def trace_config(self, rays_dev, n, hits_dev, anyhit=False, block=128, stack_depth=16, persistent=True):
+    def trace_config(self, rays_dev, n, hits_dev, anyhit=False, block=128, stack_depth=16, persistent=True, lds_nodes=0):
         ms = C.c_float()
-        _check(lib().ph_trace_config(self._h, rays_dev, n, int(anyhit), hits_dev, block, stack_depth, int(persistent), C.byref(ms)))
+        _check(lib().ph_trace_config(self._h, rays_dev, n, int(anyhit), hits_dev, block, stack_depth, int(persistent), lds_nodes, C.byref(ms)))
         return ms.value
 
 

@@ -64,9 +64,9 @@ def test_traversal_counters_match_oracle(spheres):
     assert np.array_equal(gc.astype(np.int64), oc)
 
 
-@pytest.mark.parametrize("block,depth,persistent", [(64, 8, True), (128, 16, True), (128, 24, False), (256, 32, True), (128, 64, False),
-                                                    (64, 16, False)])
-def test_launch_shapes_give_identical_hits(spheres, block, depth, persistent):
+@pytest.mark.parametrize("block,depth,persistent,top", [(64, 8, True, 0), (128, 16, True, 0), (128, 24, False, 0), (256, 32, True, 0),
+                                                        (128, 64, False, 0), (64, 16, False, 0), (128, 16, True, 512), (256, 16, True, 100)])
+def test_launch_shapes_give_identical_hits(spheres, block, depth, persistent, top):
     cam_rays, shadow = sphere_rays(spheres, 128, 128, 16384)
     r = uploaded(spheres)
     ref_c = O.trace(spheres.desc(), cam_rays)
@@ -74,7 +74,7 @@ def test_launch_shapes_give_identical_hits(spheres, block, depth, persistent):
     for rays, ref, anyhit in ((cam_rays, ref_c, False), (shadow, ref_s, True)):
         rd = gpu.DeviceArray.from_numpy(rays)
         hd = gpu.DeviceArray(len(rays) * 16)
-        r.trace_config(rd.ptr, len(rays), hd.ptr, anyhit=anyhit, block=block, stack_depth=depth, persistent=persistent)
+        r.trace_config(rd.ptr, len(rays), hd.ptr, anyhit=anyhit, block=block, stack_depth=depth, persistent=persistent, lds_nodes=top)
         assert hd.numpy(HIT_DTYPE).tobytes() == ref.tobytes()
 
 
