@@ -294,6 +294,9 @@ def bench_serving(args, gpu, host, field, launch):
                latency_ms={"p50": round(float(np.percentile(L, 50)), 2), "p99": round(float(np.percentile(L, 99)), 2)},
                slowdown={"p50": round(float(np.percentile(slow, 50)), 2), "p99": round(float(np.percentile(slow, 99)), 2)},
                by_job_size=by_size, path_slots=args.serve_slots, iterations_per_step=args.serve_iterations, schedule=args.serve_schedule)
+    worst = np.argsort(slow)[-5:][::-1]
+    res["worst_slowdown"] = [dict(res=f"{jobs[k]['w']}x{jobs[k]['h']}", spp=jobs[k]["spp"], depth=jobs[k]["depth"], scene=jobs[k]["scene"],
+                                  latency_ms=round(float(L[k]), 1), slowdown=round(float(slow[k]), 1)) for k in worst]
     res["latency_p99_over_p50"] = round(res["latency_ms"]["p99"] / res["latency_ms"]["p50"], 2)
     res["slowdown_p99_over_p50"] = round(res["slowdown"]["p99"] / res["slowdown"]["p50"], 2)
     log(f"serving: {res}")

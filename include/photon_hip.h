@@ -59,6 +59,8 @@ typedef struct ph_wavefront_config {
     int32_t graph_iterations; /* bounces per captured graph */
     int32_t refill_lanes;  /* a traversal wave takes new rays once this many of its 64 lanes are free */
     int32_t lds_nodes;     /* top BVH nodes (breadth-first) each traversal block stages in LDS */
+    int32_t ray_bins;      /* trace camera rays, then continuing rays by direction octant (1), or all together (0) */
+    int32_t xcd_parts;     /* 8: each XCD takes a contiguous eighth of every queue (L2 locality); 1: all waves interleave */
 } ph_wavefront_config;
 
 /* Statistics of the last render call. After render_denoised, ms_shade is the one-sample G-buffer trace
@@ -154,8 +156,9 @@ float ph_svgf_last_ms(const ph_svgf* f);
 
 /* Serving: several scenes stay resident; render jobs of mixed resolution, sample count and depth are
  * admitted by HBM footprint while others run, and their paths share the same wavefronts (continuous
- * batching). Each pixel is rendered by one path slot in sample order, so a job's image is identical
- * whether it runs alone or among others. No counterpart in the reference. */
+ * batching). A job's image depends only on the job and the server's min_items, so it is identical
+ * whether the job runs alone or among others; with min_items = 1 it is bit-identical to the renderer's.
+ * No counterpart in the reference. */
 typedef struct ph_server ph_server;
 
 typedef struct ph_server_config {
@@ -165,7 +168,8 @@ typedef struct ph_server_config {
     int32_t max_jobs;            /* jobs admitted at once, at most 1024 (0: 1024) */
     int64_t pass_samples;        /* pixel samples per pass of a job (0: 4 x path_slots) */
     int32_t schedule;            /* PH_SCHEDULE_SRPT or PH_SCHEDULE_FIFO */
-    int32_t reserved;
+    int32_t min_items;           /* a job with fewer pixels runs each pixel's samples as sub-streams so that
+                                    pixels x sub-streams reaches this (0: path_slots / 8; 1: never split) */
 } ph_server_config;
 
 #define PH_SCHEDULE_SRPT 0 /* unstarted work of the job with the least remaining work first */

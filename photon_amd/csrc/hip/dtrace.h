@@ -210,12 +210,12 @@ PH_DEV bool walk_step(const TraceScene& ts, Walker& w, HitRec& hit, Stack& stack
     return true;
 }
 
-// Persistent work distribution without atomics (atomics on one address serialize). Workgroups are
-// dispatched to the 8 XCDs in turn (block b runs on XCD b % 8), and each XCD has its own L2: a list of
-// n entries is cut into 8 contiguous parts, one per XCD, so the waves that share an L2 work on
-// neighbouring entries (neighbouring pixels, hence nearby rays and surfaces). Inside its part, an XCD's
-// waves take the 64-entry batches in turn: wave k of the XCD takes batches k, k + W, k + 2W, ... where
-// W counts the XCD's waves.
+// Persistent work distribution without atomics (atomics on one address serialize): the waves take the
+// list's 64-entry batches in turn (wave k takes batches k, k + W, k + 2W, ...). With `parts` = 8 the
+// list is first cut into 8 contiguous parts, one per XCD (workgroups are dispatched to the 8 XCDs in
+// turn, block b on XCD b % 8, and each XCD has its own L2), so the waves sharing an L2 work on
+// neighbouring entries; that trades L2 locality against balance between XCDs, whose parts of an image
+// can cost very differently.
 struct WaveShare {
     int first;    // first batch of this XCD's part
     int batches;  // batches in the part
@@ -223,8 +223,8 @@ struct WaveShare {
     int waves;    // waves on the XCD
 };
 
-PH_DEV WaveShare wave_share(int n) {
-    const int parts = gridDim.x >= 8 ? 8 : 1;
+PH_DEV WaveShare wave_share(int n, int xcd_parts) {
+    const int parts = (xcd_parts == 8 && gridDim.x >= 8) ? 8 : 1;
     const int wpb = blockDim.x >> 6;
     const int part = blockIdx.x % parts;
     const int blocks_here = (int(gridDim.x) - part + parts - 1) / parts;
@@ -241,8 +241,8 @@ PH_DEV WaveShare wave_share(int n) {
 // it starts change. `load(i, ts, w, hit)` starts entry i (and may set the lane's scene);
 // `finish(ts, w, hit)` stores its result.
 template <bool AnyHit, bool Count, class Stack, class Load, class Finish>
-PH_DEV void trace_stream(int n, int refill, TraceScene ts, Stack& stack, TraceCount& cnt, Load load, Finish finish) {
-    const WaveShare ws = wave_share(n);
+PH_DEV void trace_stream(int n, int parts, int refill, TraceScene ts, Stack& stack, TraceCount& cnt, Load load, Finish finish) {
+    const WaveShare ws = wave_share(n, parts);
     int pos = 0;  // entries of this wave's share already handed to lanes
     bool drained = ws.wave >= ws.batches;
     bool live = false;

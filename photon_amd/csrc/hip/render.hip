@@ -159,7 +159,7 @@ __global__ __launch_bounds__(Block) void trace_kernel(DScene s, const ph_ray* ra
         TraceCount c{0, 0, 0};
         int idx = 0;  // this lane's ray
         trace_stream<AnyHit, false>(
-            n, 16, ts, stack, c,
+            n, 1, 16, ts, stack, c,
             [&](int i, TraceScene&, Walker& w, HitRec& hit) {
                 idx = i;
                 const float* r = reinterpret_cast<const float*>(rays + i);

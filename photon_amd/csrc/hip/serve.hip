@@ -5,9 +5,14 @@
 // then advances every path in flight by a fixed number of bounces. A job runs as passes of at most
 // `pass_samples` pixel samples; its next pass is queued once the previous one has finished. Unstarted
 // work is served shortest-remaining-job first (or passes first come, first served), so a short job is
-// not stuck behind a long one. Jobs finish independently, and a finished job's image can be read while the
-// others keep running. A pixel's samples are taken in order (pass after pass), so an image does not
-// depend on what else shared the wavefronts.
+// not stuck behind a long one. Jobs finish independently, and a finished job's image can be read while
+// the others keep running.
+//
+// A job with fewer than `min_items` pixels runs each pixel's samples as S sub-streams (S divides spp)
+// in separate partial images, which the result adds in sub-stream order: its paths can then fill more
+// slots than it has pixels. Within a sub-stream the samples are taken in order, pass after pass. An
+// image therefore depends only on the job and the server's min_items, never on what else shared the
+// wavefronts; with S = 1 it is bit-identical to the renderer's.
 #include <chrono>
 #include <deque>
 #include <memory>
@@ -32,6 +37,8 @@ struct Job {
     ph_job_status status{};
     int entry = -1;
     size_t bytes = 0;
+    int split = 1;          // sub-streams per pixel
+    int64_t items = 0;      // work items of a pass: pixels x split
     int pass_spp = 1;       // samples per pass
     int next_k = 0;         // first sample of the next pass to queue
     int passes_queued = 0;  // passes queued so far
@@ -63,6 +70,8 @@ struct ph_server {
         if (cfg.max_jobs <= 0 || cfg.max_jobs > WavefrontEngine::kMaxJobs) cfg.max_jobs = WavefrontEngine::kMaxJobs;
         if (cfg.pass_samples <= 0) cfg.pass_samples = int64_t(4) * cfg.path_slots;
         if (cfg.schedule != PH_SCHEDULE_SRPT && cfg.schedule != PH_SCHEDULE_FIFO) throw std::invalid_argument("unknown schedule");
+        if (cfg.min_items == 0) cfg.min_items = std::max(1, cfg.path_slots / 8);
+        if (cfg.min_items < 0) cfg.min_items = 1;
         PH_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         engine.serve_begin(stream, cfg.path_slots);
     }
@@ -78,7 +87,7 @@ struct ph_server {
             Job& j = *jobs[size_t(queue.front())];
             // A job larger than the whole budget is admitted alone rather than never.
             if (mem_used + int64_t(j.bytes) > cfg.memory_budget && !running.empty()) break;
-            j.accum.allocate(size_t(j.desc.width) * size_t(j.desc.height));
+            j.accum.allocate(size_t(j.items));
             j.accum.zero(stream);
             const ServedScene& sc = *scenes[size_t(j.desc.scene)];
             WavefrontJob w{};
@@ -94,6 +103,7 @@ struct ph_server {
             w.albedo = nullptr;
             w.nd = nullptr;
             w.mat_class = sc.mat_class.ptr;
+            w.split = j.split;
             const int entry = engine.serve_admit(stream, w);
             if (entry < 0) {
                 j.accum.release();
@@ -102,8 +112,8 @@ struct ph_server {
             j.entry = entry;
             j.status.state = PH_JOB_RUNNING;
             j.status.admit_ms = now_ms();
-            const int64_t per_pass = cfg.pass_samples / std::max(1, j.status.pixels);
-            j.pass_spp = int(std::min<int64_t>(std::max<int64_t>(per_pass, 1), j.desc.spp));
+            const int64_t per_pass = cfg.pass_samples / std::max<int64_t>(1, j.items);
+            j.pass_spp = int(std::min<int64_t>(std::max<int64_t>(per_pass, 1), j.desc.spp / j.split));
             queue_next_pass(j);
             mem_used += int64_t(j.bytes);
             running.push_back(queue.front());
@@ -112,9 +122,10 @@ struct ph_server {
     }
 
     void queue_next_pass(Job& j) {
-        const int k0 = j.next_k, k1 = std::min(j.desc.spp, k0 + j.pass_spp);
+        const int span = j.desc.spp / j.split;
+        const int k0 = j.next_k, k1 = std::min(span, k0 + j.pass_spp);
         // Shortest remaining work first, or passes in the order they are queued.
-        const double priority = cfg.schedule == PH_SCHEDULE_FIFO ? 0.0 : double(j.status.pixels) * double(j.desc.spp - k0);
+        const double priority = cfg.schedule == PH_SCHEDULE_FIFO ? 0.0 : double(j.items) * double(span - k0);
         if (!engine.serve_enqueue(j.entry, k0, k1, priority)) {
             j.pass_pending = true;
             return;
@@ -134,14 +145,14 @@ struct ph_server {
         std::vector<int> still;
         for (int id : running) {
             Job& j = *jobs[size_t(id)];
-            const int64_t px = j.status.pixels;
-            const int64_t done = engine.serve_pixels_done(j.entry);  // pixel passes, all passes so far
+            const int64_t px = j.items;
+            const int64_t done = engine.serve_pixels_done(j.entry);  // finished items, all passes so far
             const int64_t in_pass = done - px * (j.passes_queued - 1);
             j.status.samples_done = px * j.pass_k0 + in_pass * (j.pass_k1 - j.pass_k0);
             if (j.pass_pending) {
                 queue_next_pass(j);
             } else if (done == px * j.passes_queued) {
-                if (j.next_k < j.desc.spp) {
+                if (j.next_k < j.desc.spp / j.split) {
                     queue_next_pass(j);
                 } else {
                     j.status.state = PH_JOB_DONE;
@@ -199,6 +210,7 @@ int ph_server_submit(ph_server* s, const ph_job_desc* d) {
     if (d->scene < 0 || d->scene >= int(s->scenes.size())) return set_error("unknown scene id");
     if (d->width <= 0 || d->height <= 0 || d->spp <= 0) return set_error("a job needs a positive size and sample count");
     if (int64_t(d->width) * d->height * d->spp >= int64_t(INT32_MAX)) return set_error("job too large: pixels x spp must stay below 2^31");
+    if (d->width > 65536 || d->height > 65536) return set_error("job image too large");
     if (d->path.max_depth <= 0 || d->path.max_depth > 0xFFFF) return set_error("max_depth out of range");
     return guarded([&] {
         auto j = std::make_unique<Job>();
@@ -207,7 +219,12 @@ int ph_server_submit(ph_server* s, const ph_job_desc* d) {
         j->status.pixels = d->width * d->height;
         j->status.samples = int64_t(j->status.pixels) * d->spp;
         j->status.submit_ms = s->now_ms();
-        j->bytes = size_t(d->width) * size_t(d->height) * sizeof(float4);
+        // Sub-streams: the largest divisor of spp that keeps pixels x split within min_items (or 1).
+        const int64_t want = (int64_t(s->cfg.min_items) + j->status.pixels - 1) / j->status.pixels;
+        for (int k = 1; k <= d->spp && k <= want; k++)
+            if (d->spp % k == 0) j->split = k;
+        j->items = int64_t(j->status.pixels) * j->split;
+        j->bytes = size_t(j->items) * sizeof(float4);
         s->jobs.push_back(std::move(j));
         const int id = int(s->jobs.size()) - 1;
         s->queue.push_back(id);
@@ -237,14 +254,22 @@ int ph_server_result(ph_server* s, int job, float* rgb) {
     if (j.status.state != PH_JOB_DONE) return set_error(j.status.state == PH_JOB_RELEASED ? "the job's result was already read" : "the job is not done");
     return guarded([&] {
         const size_t n = size_t(j.status.pixels);
-        std::vector<float4> raw(n);
-        PH_CHECK(hipMemcpyAsync(raw.data(), j.accum.ptr, n * sizeof(float4), hipMemcpyDeviceToHost, s->stream));
+        std::vector<float4> raw(size_t(j.items));
+        PH_CHECK(hipMemcpyAsync(raw.data(), j.accum.ptr, raw.size() * sizeof(float4), hipMemcpyDeviceToHost, s->stream));
         PH_CHECK(hipStreamSynchronize(s->stream));
         const float inv = 1.0f / float(j.desc.spp);
         for (size_t i = 0; i < n; i++) {
-            rgb[3 * i] = raw[i].x * inv;
-            rgb[3 * i + 1] = raw[i].y * inv;
-            rgb[3 * i + 2] = raw[i].z * inv;
+            // The partial images in sub-stream order (one plane: the accumulation itself).
+            float r = raw[i].x, g = raw[i].y, b = raw[i].z;
+            for (int p = 1; p < j.split; p++) {
+                const float4& v = raw[size_t(p) * n + i];
+                r += v.x;
+                g += v.y;
+                b += v.z;
+            }
+            rgb[3 * i] = r * inv;
+            rgb[3 * i + 1] = g * inv;
+            rgb[3 * i + 2] = b * inv;
         }
         j.accum.release();
         s->mem_used -= int64_t(j.bytes);

@@ -23,6 +23,9 @@ struct WavefrontJob {
     float4* albedo;
     float4* nd;
     const int* mat_class;  // per-material class table in device memory (nullptr: computed per render)
+    // Serving: the samples of each pixel run as `split` sub-streams of spp / split samples, each added
+    // to its own partial image (accum holds split planes of width x height); 1 for the renderer.
+    int split = 1;
 };
 
 // Per-material class of the shading queues: 0 metal only, 1 opaque only, 2 glass only, 3 mixed.
@@ -44,12 +47,13 @@ public:
     ph_wavefront_config autotune(hipStream_t st, const WavefrontJob& job);
 
     // Serving. serve_begin makes `slots` idle paths; serve_admit places a job in a free table entry
-    // (returns it, or -1 when the table is full); serve_enqueue queues one pass of an admitted job (all
-    // its pixels, samples k0..k1-1) and returns false when the pass table is full; serve_run lays out
+    // (returns it, or -1 when the table is full); serve_enqueue queues one pass of an admitted job (every
+    // pixel of every sub-stream, samples k0..k1-1 of the sub-stream) and returns false when the pass
+    // table is full; serve_run lays out
     // the unstarted items of the queued passes by priority (smaller first, then queue order) and
     // advances every path in flight by `iterations` bounces (blocking). serve_pixels_done counts the
-    // finished pixel passes of an entry; a pass must not be queued before the previous pass of the same
-    // job has finished. serve_release frees the entry.
+    // finished items (pixel sub-streams) of an entry's passes; a pass must not be queued before the
+    // previous pass of the same job has finished. serve_release frees the entry.
     void serve_begin(hipStream_t st, int slots);
     int serve_admit(hipStream_t st, const WavefrontJob& job);
     bool serve_enqueue(int entry, int k0, int k1, double priority);

@@ -55,22 +55,31 @@ enum : int {
     C_MISS = 3,
     C_CLASS = 4,    // 4 queues: metal, opaque, glass, mixed
     C_CHUNKS = 10,  // entries of the chunk table
+    C_PARTS = 11,   // 1, or 8 for per-XCD parts of the queues (wave_share)
     C_COUNT = 16
 };
 
-// Slot stages (low 7 bits) and the pending-shadow-ray bit.
-enum : uint8_t { S_IDLE = 0, S_EXT = 1, S_MISS = 2, S_CLASS = 3, S_TERM = 7, S_SHADOW = 0x80, S_MASK = 0x7F };
+// Slot stages: the stage code (low 3 bits), the extension ray's bin (bits 3-6: 0 camera rays, 1-8
+// continuing rays by direction octant) and the pending-shadow-ray bit.
+enum : uint8_t { S_IDLE = 0, S_EXT = 1, S_MISS = 2, S_CLASS = 3, S_TERM = 7, S_CODE = 7, S_SHADOW = 0x80 };
+constexpr int kBins = 9;
+
+PH_DEV uint8_t ext_stage(int bin) { return uint8_t(S_EXT | (bin << 3)); }
+
+// Bin of a continuing ray: its direction octant, or one shared bin when binning is off.
+PH_DEV int ray_bin(f3 d, int bins) { return bins ? 1 + (d.x < 0 ? 1 : 0) + (d.y < 0 ? 2 : 0) + (d.z < 0 ? 4 : 0) : 1; }
 
 constexpr int kClasses = 4;
 constexpr int kMaxChunks = 1 << 16;
 // Ordered compaction: a block of 256 threads covers 4096 slots, 16 consecutive slots per thread.
 constexpr int kCompactThreads = 256, kCompactPerThread = 16, kCompactChunk = kCompactThreads * kCompactPerThread;
-constexpr int kMaxCategories = 5;
+constexpr int kMaxCategories = 9;
 
 struct JobParams {
     DScene scene;
     ph_camera_view camera;
     int width, height, spp, pixels;
+    int split, span;  // sub-streams of samples per pixel (serving), samples per sub-stream
     uint32_t frame0, seed;
     PathOpts opt;
     float4* accum;
@@ -84,6 +93,7 @@ struct State {
     int* pixel;
     int* sample;
     int* sample_end;  // end of the slot's pass (serving)
+    int* part;        // the slot's sub-stream (serving: accumulates into partial image `part`)
     uint8_t* stage;
     uint64_t* rng;
     float4* ro;  // origin, prev_pdf
@@ -220,12 +230,19 @@ PH_DEV void start_work(const Work& work, const State& st, int chunks, int slot, 
     const int c = chunk_of<Multi>(work, chunks, w);
     const int4 pass = Multi ? work.chunk_pass[c] : make_int4(0, 0, 0, 0);
     const JobParams& jp = Multi ? work.jobs[pass.x] : single;
-    const int local = int(w - (Multi ? work.chunk_first[c] : 0)) + pass.w;
+    int local = int(w - (Multi ? work.chunk_first[c] : 0)) + pass.w;
+    int k0 = pass.y;
     if (Multi) {
+        // Items of a pass: sub-stream `part` of every pixel, for parts 0 .. split-1 in turn; the pass's
+        // sample range is relative to the sub-stream's first sample.
+        const int part = local / jp.pixels;
+        local -= part * jp.pixels;
+        k0 += part * jp.span;
         st.job[slot] = pass.x;
-        st.sample_end[slot] = pass.z;
+        st.part[slot] = part;
+        st.sample_end[slot] = part * jp.span + pass.z;
     }
-    start_sample(jp, st, slot, work_pixel(local, jp.width, jp.height), pass.y);
+    start_sample(jp, st, slot, work_pixel(local, jp.width, jp.height), k0);
 }
 
 PH_DEV Pcg slot_rng(const State& st, int slot, int pixel) { return Pcg{st.rng[slot], (uint64_t(uint32_t(pixel)) << 1u) | 1u}; }
@@ -236,7 +253,7 @@ __global__ void init_kernel(Work work, State st, int slots) {
     if (slot >= slots) return;
     const bool has = slot < (long long)work.items[1];
     if (has) start_work<false>(work, st, 1, slot, slot, work.jobs[0]);
-    st.stage[slot] = has ? S_EXT : S_IDLE;
+    st.stage[slot] = has ? ext_stage(0) : S_IDLE;
 }
 
 // Serving: every slot starts idle.
@@ -259,14 +276,16 @@ enum { K_EXT = 0, K_HIT = 1, K_DONE = 2 };
 
 template <int Kind>
 struct Compact {
-    static constexpr int cats = Kind == K_EXT ? 1 : Kind == K_HIT ? 5 : 2;
+    static constexpr int cats = Kind == K_EXT ? kBins : Kind == K_HIT ? 5 : 2;
     static constexpr int counter = Kind == K_EXT ? C_EXT : Kind == K_HIT ? C_MISS : C_SHADOW;
-    // Category bits of a stage byte: extension; miss + 4 classes; shadow + finished (or idle with work).
+    // The extension queue is one list, its bins one after another; the others are one list each.
+    static constexpr bool concatenated = Kind == K_EXT;
+    // Category bits of a stage byte: extension bins; miss + 4 classes; shadow + finished (or idle with work).
     static PH_DEV unsigned bits(uint8_t s, bool idle_takes_work) {
-        const int low = s & S_MASK;
-        if (Kind == K_EXT) return low == S_EXT ? 1u : 0u;
-        if (Kind == K_HIT) return (low >= S_MISS && low < S_CLASS + kClasses) ? 1u << (low - S_MISS) : 0u;
-        return ((s & S_SHADOW) ? 1u : 0u) | ((low == S_TERM || (idle_takes_work && low == S_IDLE)) ? 2u : 0u);
+        const int code = s & S_CODE;
+        if (Kind == K_EXT) return code == S_EXT ? 1u << ((s >> 3) & 15) : 0u;
+        if (Kind == K_HIT) return (code >= S_MISS && code < S_CLASS + kClasses) ? 1u << (code - S_MISS) : 0u;
+        return ((s & S_SHADOW) ? 1u : 0u) | ((code == S_TERM || (idle_takes_work && code == S_IDLE)) ? 2u : 0u);
     }
 };
 
@@ -340,18 +359,24 @@ __global__ __launch_bounds__(kCompactThreads) void compact_scatter_kernel(const 
     uint8_t b[16];
     bool full;
     load_stages(stage, first, n, b, full);
-    uint8_t m[16];
+    uint16_t m[16];
     int cnt[K] = {};
     for (int j = 0; j < 16; j++) {
-        m[j] = uint8_t((first + j < n) ? Compact<Kind>::bits(b[j], idle) : 0u);
+        m[j] = uint16_t((first + j < n) ? Compact<Kind>::bits(b[j], idle) : 0u);
         for (int k = 0; k < K; k++) cnt[k] += (m[j] >> k) & 1;
     }
     const int lane = lane_id(), wave = threadIdx.x >> 6;
     int pos[K], chunk_tot[K];
+    int offset = 0;  // concatenated lists: entries of the earlier categories
     for (int k = 0; k < K; k++) {
-        int before = 0;
-        for (int i = threadIdx.x; i < c; i += kCompactThreads) before += partial[k * nchunks + i];
-        const int base = block_sum(before, red);
+        int before = 0, all = 0;
+        for (int i = threadIdx.x; i < nchunks; i += kCompactThreads) {
+            const int v = partial[k * nchunks + i];
+            if (i < c) before += v;
+            all += v;
+        }
+        const int base = block_sum(before, red) + offset;
+        if (Compact<Kind>::concatenated) offset += block_sum(all, red);
         // Inclusive scan over the wave, then over the waves of the block.
         int v = cnt[k];
         for (int off = 1; off < 64; off <<= 1) {
@@ -370,16 +395,20 @@ __global__ __launch_bounds__(kCompactThreads) void compact_scatter_kernel(const 
     }
     for (int j = 0; j < 16; j++)
         for (int k = 0; k < K; k++)
-            if ((m[j] >> k) & 1) out.q[k][pos[k]++] = first + j;
-    if (c == nchunks - 1 && threadIdx.x == 0)
-        for (int k = 0; k < K; k++) counters[Compact<Kind>::counter + k] = chunk_tot[k];
+            if ((m[j] >> k) & 1) out.q[Compact<Kind>::concatenated ? 0 : k][pos[k]++] = first + j;
+    if (c == nchunks - 1 && threadIdx.x == 0) {
+        if (Compact<Kind>::concatenated)
+            counters[Compact<Kind>::counter] = chunk_tot[K - 1];
+        else
+            for (int k = 0; k < K; k++) counters[Compact<Kind>::counter + k] = chunk_tot[k];
+    }
 }
 
 // Traversal of a queue of slots (trace_stream). One job: every lane shares the scene, whose traversal
 // arrays sit in scalar registers and whose top `lds_nodes` nodes the block stages in LDS. Serving: each
 // lane reads its slot's scene.
 template <bool AnyHit, bool Count, bool Multi, class Stack, class Load, class Finish>
-PH_DEV void trace_queue(const Work& work, const State& st, const int* queue, int n, int refill, int lds_nodes, float4* top, Stack& stack,
+PH_DEV void trace_queue(const Work& work, const State& st, const int* queue, int n, int parts, int refill, int lds_nodes, float4* top, Stack& stack,
                         TraceCount& cnt, Load load, Finish finish) {
     TraceScene ts0{};
     if (!Multi) {
@@ -389,7 +418,7 @@ PH_DEV void trace_queue(const Work& work, const State& st, const int* queue, int
     }
     int slot = 0, entry = 0;
     trace_stream<AnyHit, Count>(
-        n, refill, ts0, stack, cnt,
+        n, parts, refill, ts0, stack, cnt,
         [&](int i, TraceScene& ts, Walker& w, HitRec& hit) {
             slot = queue[i];
             if (Multi) {
@@ -408,7 +437,7 @@ __global__ __launch_bounds__(Block) void extend_kernel(Work work, State st, int*
     LaneStack<Depth> stack{lds + threadIdx.x, Block};
     TraceCount cnt{0, 0, 0};
     trace_queue<false, Count, Multi>(
-        work, st, st.q_ext, counters[C_EXT], refill, lds_nodes, top_nodes, stack, cnt,
+        work, st, st.q_ext, counters[C_EXT], counters[C_PARTS], refill, lds_nodes, top_nodes, stack, cnt,
         [&](int slot, Walker& w, HitRec& hit) {
             const float4 o4 = st.ro[slot], d4 = st.rd[slot];
             w = walker(xyz(o4), xyz(d4), kInf);
@@ -433,7 +462,7 @@ __global__ __launch_bounds__(Block) void shadow_kernel(Work work, State st, int*
     LaneStack<Depth> stack{lds + threadIdx.x, Block};
     TraceCount cnt{0, 0, 0};
     trace_queue<true, Count, Multi>(
-        work, st, st.q_shadow, counters[C_SHADOW], refill, lds_nodes, top_nodes, stack, cnt,
+        work, st, st.q_shadow, counters[C_SHADOW], counters[C_PARTS], refill, lds_nodes, top_nodes, stack, cnt,
         [&](int slot, Walker& w, HitRec& hit) {
             const float4 o4 = st.so[slot], d4 = st.sd[slot];
             w = walker(xyz(o4), xyz(d4), o4.w);
@@ -455,7 +484,7 @@ template <bool Multi>
 __global__ __launch_bounds__(256) void miss_kernel(Work work, State st, int* counters) {
     const int n = counters[C_MISS];
     const JobParams single = single_job<Multi>(work);
-    const WaveShare ws = wave_share(n);
+    const WaveShare ws = wave_share(n, counters[C_PARTS]);
     for (int b = ws.wave; b < ws.batches; b += ws.waves) {
         const int i = (ws.first + b) * 64 + lane_id();
         if (i < n) {
@@ -485,16 +514,17 @@ __global__ __launch_bounds__(256) void miss_kernel(Work work, State st, int* cou
 
 // One path vertex (integrator.h:178-260) for the slots of one material class.
 template <int Lobes, bool Multi>
-__global__ __launch_bounds__(256) void shade_kernel(Work work, State st, int* counters, int cls) {
+__global__ __launch_bounds__(256) void shade_kernel(Work work, State st, int* counters, int cls, int bins) {
     const int n = counters[C_CLASS + cls];
     const int* queue = st.q_class[cls];
     const JobParams single = single_job<Multi>(work);
-    const WaveShare ws = wave_share(n);
+    const WaveShare ws = wave_share(n, counters[C_PARTS]);
     for (int b = ws.wave; b < ws.batches; b += ws.waves) {
         const int i = (ws.first + b) * 64 + lane_id();
         if (i < n) {
             const int slot = queue[i];
             bool cont = false, shadow = false;
+            int bin = 1;
             const JobParams& jp = job_of<Multi>(work, st, slot, single);
             const DScene& s = jp.scene;
             const PathOpts opt = jp.opt;
@@ -568,6 +598,7 @@ __global__ __launch_bounds__(256) void shade_kernel(Work work, State st, int* co
                                     beta = beta * (1.0f / (1.0f - q));
                             }
                             if (cont) {
+                                bin = ray_bin(nd, bins);
                                 st.ro[slot] = make_float4(no.x, no.y, no.z, bs.pdf);
                                 st.rd[slot] = make_float4(nd.x, nd.y, nd.z, __int_as_float((depth + 1) | ((nee ? 0 : 1) << 16)));
                                 st.beta[slot] = make_float4(beta.x, beta.y, beta.z, 0.0f);
@@ -578,7 +609,7 @@ __global__ __launch_bounds__(256) void shade_kernel(Work work, State st, int* co
                 st.rng[slot] = rng.state;
             }
             st.L[slot] = make_float4(L.x, L.y, L.z, 0.0f);
-            st.stage[slot] = uint8_t((cont ? S_EXT : S_TERM) | (shadow ? S_SHADOW : 0));
+            st.stage[slot] = uint8_t((cont ? ext_stage(bin) : S_TERM) | (shadow ? S_SHADOW : 0));
         }
     }
 }
@@ -592,25 +623,26 @@ __global__ __launch_bounds__(256) void regen_kernel(Work work, State st, int* co
     const long long total = (long long)work.items[1];
     const int chunks = counters[C_CHUNKS];
     const JobParams single = single_job<Multi>(work);
-    const WaveShare ws = wave_share(n);
+    const WaveShare ws = wave_share(n, counters[C_PARTS]);
     for (int b = ws.wave; b < ws.batches; b += ws.waves) {
         const int i = (ws.first + b) * 64 + lane_id();
         const bool active = i < n;
         const int slot = active ? st.q_term[i] : 0;
         bool need_work = false, ext = false;
-        if (active && (!Multi || (st.stage[slot] & S_MASK) == S_TERM)) {
+        if (active && (!Multi || (st.stage[slot] & S_CODE) == S_TERM)) {
             const JobParams& jp = job_of<Multi>(work, st, slot, single);
             const int pixel = st.pixel[slot];
             int k = st.sample[slot];
             f3 L = xyz(st.L[slot]);
             if (!finite3(L) || L.x < 0 || L.y < 0 || L.z < 0) L = mk3(0);
-            float4 a = jp.accum[pixel];
+            float4* acc = jp.accum + (Multi ? st.part[slot] * jp.pixels : 0) + pixel;
+            float4 a = *acc;
             a.x += L.x;
             a.y += L.y;
             a.z += L.z;
             const float l = lum(L);
             a.w += l * l;
-            jp.accum[pixel] = a;
+            *acc = a;
             k++;
             need_work = k >= (Multi ? st.sample_end[slot] : jp.spp);
             if (need_work) {
@@ -627,7 +659,7 @@ __global__ __launch_bounds__(256) void regen_kernel(Work work, State st, int* co
         const long long w = wave_take(work.items, want);
         const bool got = want && w < total;
         if (got) start_work<Multi>(work, st, chunks, slot, w, single);
-        if (active) st.stage[slot] = (ext || got) ? S_EXT : S_IDLE;
+        if (active) st.stage[slot] = (ext || got) ? ext_stage(0) : S_IDLE;
     }
 }
 
@@ -711,10 +743,10 @@ std::vector<int> material_classes(const DScene& scene, hipStream_t st) {
 }
 
 struct WavefrontEngine::Impl {
-    ph_wavefront_config cfg{128, 24, 0, 16, 1, 1, 16, 8, 0};
+    ph_wavefront_config cfg{128, 24, 0, 16, 1, 1, 16, 16, 0, 1, 1};
     int slots = 0;
     int cus = 0;
-    DevBuf<int> job, pixel, sample, sample_end, counters, q_ext, q_shadow, q_term, q_miss, q_cls[kClasses], mat_class, partial;
+    DevBuf<int> job, pixel, sample, sample_end, part, counters, q_ext, q_shadow, q_term, q_miss, q_cls[kClasses], mat_class, partial;
     DevBuf<uint8_t> stage;
     DevBuf<int> job_done;
     DevBuf<long long> chunk_first;
@@ -788,7 +820,7 @@ struct WavefrontEngine::Impl {
     void ensure_slots(int n) {
         if (n == slots) return;
         drop_graph();
-        for (DevBuf<int>* b : {&job, &pixel, &sample, &sample_end, &q_ext, &q_shadow, &q_term, &q_miss}) b->allocate(size_t(n));
+        for (DevBuf<int>* b : {&job, &pixel, &sample, &sample_end, &part, &q_ext, &q_shadow, &q_term, &q_miss}) b->allocate(size_t(n));
         for (auto& q : q_cls) q.allocate(size_t(n));
         stage.allocate(size_t(n + 15) / 16 * 16);
         partial.allocate(size_t(kMaxCategories) * size_t(chunks_of(n)));
@@ -803,6 +835,7 @@ struct WavefrontEngine::Impl {
         s.pixel = pixel.ptr;
         s.sample = sample.ptr;
         s.sample_end = sample_end.ptr;
+        s.part = part.ptr;
         s.rng = rng.ptr;
         s.ro = ro.ptr;
         s.rd = rd.ptr;
@@ -865,10 +898,11 @@ struct WavefrontEngine::Impl {
         if (ev) PH_CHECK(hipEventRecord(ev[1], st));
         compact<K_HIT, Multi>(st, s);
         hipLaunchKernelGGL(miss_kernel<Multi>, dim3(sgrid), dim3(256), 0, st, w, s, cnt);
-        hipLaunchKernelGGL((shade_kernel<1, Multi>), dim3(sgrid), dim3(256), 0, st, w, s, cnt, 0);
-        hipLaunchKernelGGL((shade_kernel<2, Multi>), dim3(sgrid), dim3(256), 0, st, w, s, cnt, 1);
-        hipLaunchKernelGGL((shade_kernel<4, Multi>), dim3(sgrid), dim3(256), 0, st, w, s, cnt, 2);
-        hipLaunchKernelGGL((shade_kernel<7, Multi>), dim3(sgrid), dim3(256), 0, st, w, s, cnt, 3);
+        const int bins = cfg.ray_bins;
+        hipLaunchKernelGGL((shade_kernel<1, Multi>), dim3(sgrid), dim3(256), 0, st, w, s, cnt, 0, bins);
+        hipLaunchKernelGGL((shade_kernel<2, Multi>), dim3(sgrid), dim3(256), 0, st, w, s, cnt, 1, bins);
+        hipLaunchKernelGGL((shade_kernel<4, Multi>), dim3(sgrid), dim3(256), 0, st, w, s, cnt, 2, bins);
+        hipLaunchKernelGGL((shade_kernel<7, Multi>), dim3(sgrid), dim3(256), 0, st, w, s, cnt, 3, bins);
         compact<K_DONE, Multi>(st, s);
         if (ev) PH_CHECK(hipEventRecord(ev[2], st));
         fns.shadow(st, tgrid, w, s, cnt, 0, refill, cfg.lds_nodes, totals.ptr + 1, count);
@@ -938,6 +972,9 @@ struct WavefrontEngine::Impl {
 
     void reset_counters(hipStream_t st) {
         PH_CHECK(hipMemsetAsync(counters.ptr, 0, C_COUNT * sizeof(int), st));
+        PH_CHECK(hipStreamSynchronize(st));  // the pinned staging value may feed an earlier copy
+        host_sizes[1] = cfg.xcd_parts == 8 ? 8 : 1;
+        PH_CHECK(hipMemcpyAsync(counters.ptr + C_PARTS, host_sizes + 1, sizeof(int), hipMemcpyHostToDevice, st));
         PH_CHECK(hipMemsetAsync(items.ptr, 0, 2 * sizeof(unsigned long long), st));
         PH_CHECK(hipMemsetAsync(totals.ptr, 0, 2 * sizeof(TraceTotals), st));
         PH_CHECK(hipMemsetAsync(ray_totals.ptr, 0, 2 * sizeof(unsigned long long), st));
@@ -951,6 +988,8 @@ struct WavefrontEngine::Impl {
         jp.height = job.height;
         jp.spp = job.spp;
         jp.pixels = job.width * job.height;
+        jp.split = std::max(1, job.split);
+        jp.span = job.spp / jp.split;
         jp.frame0 = job.frame0;
         jp.seed = job.seed;
         jp.opt = job.opt;
@@ -971,6 +1010,8 @@ void WavefrontEngine::set_config(const ph_wavefront_config& c) {
     if (c.path_slots < 0 || c.blocks_per_cu <= 0 || c.blocks_per_cu > 64) throw std::invalid_argument("invalid wavefront configuration");
     if (c.graph_iterations < 1 || c.graph_iterations > 256) throw std::invalid_argument("graph_iterations must be in [1, 256]");
     if (c.refill_lanes < 1 || c.refill_lanes > 64) throw std::invalid_argument("refill_lanes must be in [1, 64]");
+    if (c.ray_bins != 0 && c.ray_bins != 1) throw std::invalid_argument("ray_bins must be 0 or 1");
+    if (c.xcd_parts != 1 && c.xcd_parts != 8) throw std::invalid_argument("xcd_parts must be 1 or 8");
     if (c.lds_nodes < 0 || c.lds_nodes > DeviceScene::kTopNodes || size_t(c.block) * size_t(c.stack_depth) * 4 + size_t(c.lds_nodes) * 64 > 65536)
         throw std::invalid_argument("lds_nodes does not fit the block's LDS with its stacks");
     impl_->cfg = c;
@@ -1082,6 +1123,18 @@ ph_wavefront_config WavefrontEngine::autotune(hipStream_t st, const WavefrontJob
         double t = time_of(c);
         if (t < best_ms) best_ms = t, best = c;
     }
+    for (int bins : {0, 1}) {
+        ph_wavefront_config c = best;
+        c.ray_bins = bins;
+        double t = time_of(c);
+        if (t < best_ms) best_ms = t, best = c;
+    }
+    for (int parts : {1, 8}) {
+        ph_wavefront_config c = best;
+        c.xcd_parts = parts;
+        double t = time_of(c);
+        if (t < best_ms) best_ms = t, best = c;
+    }
     for (int top : {0, 128, 256, 512}) {
         ph_wavefront_config c = best;
         c.lds_nodes = top;
@@ -1135,6 +1188,8 @@ int WavefrontEngine::serve_admit(hipStream_t st, const WavefrontJob& job) {
     if (!m.serving) throw std::runtime_error("serve_begin was not called");
     if (job.width <= 0 || job.height <= 0 || job.spp <= 0) throw std::invalid_argument("empty job");
     if (!job.mat_class) throw std::invalid_argument("serving jobs need a per-scene material class table");
+    if (job.split < 1 || job.spp % job.split) throw std::invalid_argument("the sub-stream count must divide the sample count");
+    if (int64_t(job.width) * job.height * job.split >= int64_t(INT32_MAX)) throw std::invalid_argument("job too large");
     int entry = -1;
     for (int e = 0; e < kMaxJobs; e++)
         if (!m.entry_used[size_t(e)]) {
@@ -1147,7 +1202,7 @@ int WavefrontEngine::serve_admit(hipStream_t st, const WavefrontJob& job) {
     PH_CHECK(hipMemsetAsync(m.job_done.ptr + entry, 0, sizeof(int), st));
     m.host_done[entry] = 0;
     m.entry_used[size_t(entry)] = true;
-    m.entry_pixels[size_t(entry)] = job.width * job.height;
+    m.entry_pixels[size_t(entry)] = job.width * job.height * job.split;
     return entry;
 }
 

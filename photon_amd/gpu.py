@@ -25,7 +25,7 @@ class GpuInfo(C.Structure):
 class WavefrontConfig(C.Structure):
     _fields_ = [("block", C.c_int32), ("stack_depth", C.c_int32), ("path_slots", C.c_int32), ("blocks_per_cu", C.c_int32),
                 ("sort_materials", C.c_int32), ("use_graph", C.c_int32), ("graph_iterations", C.c_int32),
-                ("refill_lanes", C.c_int32), ("lds_nodes", C.c_int32)]
+                ("refill_lanes", C.c_int32), ("lds_nodes", C.c_int32), ("ray_bins", C.c_int32), ("xcd_parts", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -45,7 +45,7 @@ class RenderStats(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("memory_budget", C.c_int64), ("path_slots", C.c_int32), ("iterations_per_step", C.c_int32),
-                ("max_jobs", C.c_int32), ("pass_samples", C.c_int64), ("schedule", C.c_int32), ("reserved", C.c_int32)]
+                ("max_jobs", C.c_int32), ("pass_samples", C.c_int64), ("schedule", C.c_int32), ("min_items", C.c_int32)]
 
 
 class JobDesc(C.Structure):
@@ -361,9 +361,9 @@ class Svgf:
 class Server:
     """The serving loop (ph_server_*): resident scenes, jobs admitted by HBM footprint, continuous batching."""
 
-    def __init__(self, memory_budget=0, path_slots=0, iterations_per_step=0, max_jobs=0, pass_samples=0, schedule=SCHEDULE_SRPT):
+    def __init__(self, memory_budget=0, path_slots=0, iterations_per_step=0, max_jobs=0, pass_samples=0, schedule=SCHEDULE_SRPT, min_items=0):
         self._h = C.c_void_p()
-        cfg = ServerConfig(memory_budget, path_slots, iterations_per_step, max_jobs, pass_samples, schedule, 0)
+        cfg = ServerConfig(memory_budget, path_slots, iterations_per_step, max_jobs, pass_samples, schedule, min_items)
         _check(lib().ph_server_create(C.byref(cfg), C.byref(self._h)))
         self._sizes = {}
 

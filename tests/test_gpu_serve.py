@@ -1,9 +1,10 @@
 """The serving loop: jobs over resident scenes, admitted by frame-buffer footprint, batched together.
 
-A pixel's samples are taken in order by one path slot, so a job's image does not depend on what else
-shares the wavefronts: it must be bit-identical to the same job rendered alone, and to the single-job
-renderer (which follows the reference's GpuRenderer::render), whatever the arrival order, slot count
-and admission budget.
+A job's image depends only on the job and the server's min_items (how small jobs are split into sample
+sub-streams), never on what else shares the wavefronts: it must be bit-identical to the same job served
+alone, whatever the arrival order, slot count, schedule and admission budget. Without splitting it is
+also bit-identical to the single-job renderer (which follows the reference's GpuRenderer::render); with
+splitting it is within the workload's radiance tolerance of it.
 """
 import numpy as np
 import pytest
@@ -11,7 +12,7 @@ import pytest
 from photon_amd import gpu, host
 from photon_amd.types import RenderSettings
 
-from gpu_util import uploaded
+from gpu_util import radiance_agreement, uploaded
 
 pytestmark = pytest.mark.gpu
 
@@ -54,8 +55,21 @@ def _serve(scenes, jobs, order, **cfg):
 def test_each_job_alone_equals_the_renderer(scenes):
     for k, (n, w, h, spp, depth, seed) in enumerate(JOBS):
         ref = _alone(scenes[n], w, h, spp, depth, seed)
-        got = _serve(scenes, JOBS, [k], path_slots=1 << 14)[k]
+        got = _serve(scenes, JOBS, [k], path_slots=1 << 14, min_items=1)[k]
         assert got.tobytes() == ref.tobytes(), f"job {k} ({n} {w}x{h})"
+
+
+def test_split_jobs_match_the_renderer_and_themselves(scenes):
+    """Every job split into sample sub-streams: within tolerance of the renderer, bitwise alone vs mix."""
+    cfg = dict(path_slots=1 << 14, min_items=50000)
+    alone = {k: _serve(scenes, JOBS, [k], **cfg)[k] for k in range(len(JOBS))}
+    mixed = _serve(scenes, JOBS, [5, 2, 0, 4, 1, 3], path_slots=3000, min_items=50000, pass_samples=7000)
+    for k, (n, w, h, spp, depth, seed) in enumerate(JOBS):
+        ref = _alone(scenes[n], w, h, spp, depth, seed)
+        agree = radiance_agreement(alone[k], ref)
+        print(f"job {k}: split agreement {100 * agree:.3f}%")
+        assert agree >= 0.999
+        assert mixed[k].tobytes() == alone[k].tobytes(), f"job {k}"
 
 
 @pytest.mark.parametrize("cfg", [dict(path_slots=1 << 14), dict(path_slots=3000, iterations_per_step=2),
@@ -63,16 +77,17 @@ def test_each_job_alone_equals_the_renderer(scenes):
                                  dict(path_slots=4096, pass_samples=3000), dict(path_slots=2000, pass_samples=1),
                                  dict(path_slots=3000, pass_samples=5000, schedule=gpu.SCHEDULE_FIFO)])
 def test_jobs_in_a_mix_equal_jobs_alone(scenes, cfg):
-    alone = {k: _serve(scenes, JOBS, [k], path_slots=1 << 14)[k] for k in range(len(JOBS))}
+    # The same splitting of small jobs in both runs (it is part of what defines a job's image).
+    alone = {k: _serve(scenes, JOBS, [k], path_slots=1 << 14, min_items=2048)[k] for k in range(len(JOBS))}
     order = [3, 0, 5, 1, 4, 2]
-    mixed = _serve(scenes, JOBS, order, **cfg)
+    mixed = _serve(scenes, JOBS, order, min_items=2048, **cfg)
     for k in order:
         assert mixed[k].tobytes() == alone[k].tobytes(), f"job {k} differs in the mix ({cfg})"
 
 
 def test_jobs_submitted_while_others_run(scenes):
     """Admission between steps: new work items start after those already handed out."""
-    srv = gpu.Server(path_slots=2048, iterations_per_step=2)
+    srv = gpu.Server(path_slots=2048, iterations_per_step=2, min_items=1)
     ids = {n: srv.add_scene(s) for n, s in scenes.items()}
     jid = {}
     pending = list(range(len(JOBS)))

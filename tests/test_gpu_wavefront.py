@@ -45,6 +45,7 @@ def test_wavefront_bitwise_equals_megakernel(name, w, h, spp):
 @pytest.mark.parametrize("cfg", [dict(use_graph=0), dict(use_graph=1, graph_iterations=4), dict(use_graph=1, graph_iterations=3),
                                  dict(path_slots=1000), dict(refill_lanes=1), dict(refill_lanes=64), dict(sort_materials=0),
                                  dict(block=256, stack_depth=16, lds_nodes=512), dict(block=128, stack_depth=24, lds_nodes=100),
+                                 dict(ray_bins=0), dict(ray_bins=1, path_slots=777), dict(xcd_parts=8), dict(xcd_parts=8, path_slots=100),
                                  dict(block=64, stack_depth=8, blocks_per_cu=2), dict(block=256, stack_depth=24, blocks_per_cu=4)])
 def test_wavefront_config_does_not_change_the_image(cfg):
     scene = host.Scene.builtin("cornell-c1").build()
