@@ -147,45 +147,50 @@ PH_DEV bool walk_step(const TraceScene& ts, Walker& w, HitRec& hit, Stack& stack
     if (Count) cnt->nodes++;
     const float tl = slab(lmin, lmax, w.o, w.inv, w.tmax);
     const float tr = slab(rmin, rmax, w.o, w.inv, w.tmax);
+    const int lcount = __float_as_int(lmax.w), rcount = __float_as_int(rmax.w);
+    const int lidx = __float_as_int(lmin.w), ridx = __float_as_int(rmin.w);
+    // Leaf children: the left leaf's triangles, then the right leaf's, as one loop (a wave runs it for
+    // the longest lane's combined count instead of the longest left plus the longest right).
+    const int nl = (tl != kInf && lcount > 0) ? lcount : 0;
+    const int nr = (tr != kInf && rcount > 0) ? rcount : 0;
+    for (int t = 0; t < nl + nr; t++) {
+        const int i = t < nl ? lidx + t : ridx + (t - nl);
+        const float4* tp = ts.tris + 3 * i;
+        const float4 a = ldg(tp), b = ldg(tp + 1), c = ldg(tp + 2);
+        float th, u, v;
+        if (Count) cnt->tris++;
+        if (mt_hit(xyz(a), xyz(b), xyz(c), w.o, w.d, w.tmax, th, u, v)) {
+            if (__float_as_int(b.w) & PH_TRI_ALPHA_TESTED) {
+                if (Count) cnt->alpha++;
+                if (alpha_cut(*ts.full, i, __float_as_int(a.w), u, v)) continue;
+            }
+            hit.t = th;
+            hit.tri = i;
+            w.found = true;
+            if (AnyHit) return false;
+            w.tmax = th;
+            hit.u = u;
+            hit.v = v;
+        }
+    }
+    // Interior children, left before right.
     int next0 = 0, next1 = 0;
     float tn0 = 0, tn1 = 0;
     int nn = 0;
-#pragma unroll
-    for (int side = 0; side < 2; side++) {
-        const float tc = side == 0 ? tl : tr;
-        const int count = __float_as_int(side == 0 ? lmax.w : rmax.w);
-        const int idx = __float_as_int(side == 0 ? lmin.w : rmin.w);
-        if (tc == kInf) continue;
-        if (count > 0) {
-            for (int i = idx; i < idx + count; i++) {
-                const float4* tp = ts.tris + 3 * i;
-                const float4 a = ldg(tp), b = ldg(tp + 1), c = ldg(tp + 2);
-                float t, u, v;
-                if (Count) cnt->tris++;
-                if (mt_hit(xyz(a), xyz(b), xyz(c), w.o, w.d, w.tmax, t, u, v)) {
-                    if (__float_as_int(b.w) & PH_TRI_ALPHA_TESTED) {
-                        if (Count) cnt->alpha++;
-                        if (alpha_cut(*ts.full, i, __float_as_int(a.w), u, v)) continue;
-                    }
-                    hit.t = t;
-                    hit.tri = i;
-                    w.found = true;
-                    if (AnyHit) return false;
-                    w.tmax = t;
-                    hit.u = u;
-                    hit.v = v;
-                }
-            }
-        } else if (idx >= 0) {
-            if (nn == 0) {
-                next0 = idx;
-                tn0 = tc;
-            } else {
-                next1 = idx;
-                tn1 = tc;
-            }
-            nn++;
+    if (tl != kInf && lcount <= 0 && lidx >= 0) {
+        next0 = lidx;
+        tn0 = tl;
+        nn = 1;
+    }
+    if (tr != kInf && rcount <= 0 && ridx >= 0) {
+        if (nn == 0) {
+            next0 = ridx;
+            tn0 = tr;
+        } else {
+            next1 = ridx;
+            tn1 = tr;
         }
+        nn++;
     }
     if (nn == 2) {
         if (tn0 > tn1) {
