@@ -57,7 +57,11 @@ def bench_render(args, gpu, host, RenderSettings):
     r.resize(W, H)
     r.set_engine(gpu.ENGINE_WAVEFRONT)
     cfg = None
-    if not args.no_autotune:
+    if args.launch:
+        kw = {k: int(v) for k, v in (item.split("=") for item in args.launch.split(","))}
+        cfg = r.set_wavefront_config(**kw)
+        log(f"launch configuration {cfg.as_dict()}")
+    elif not args.no_autotune:
         t = time.time()
         cfg = r.autotune(cam, rs, 1)
         log(f"autotune {time.time() - t:.1f} s -> {cfg.as_dict()}")
@@ -222,7 +226,8 @@ def bench_serving(args, gpu, host, field, launch):
               host.Scene.spheres(256, box=100.0, alpha_fraction=0.1, seed=1).build(), field]
     log(f"serving scenes: {[s.num_tris for s in scenes]} triangles, setup {time.time() - t:.1f} s")
     srv = gpu.Server(path_slots=args.serve_slots, iterations_per_step=args.serve_iterations,
-                     schedule=gpu.SCHEDULE_FIFO if args.serve_schedule == "fifo" else gpu.SCHEDULE_SRPT)
+                     schedule=gpu.SCHEDULE_FIFO if args.serve_schedule == "fifo" else gpu.SCHEDULE_SRPT,
+                     pass_samples=args.serve_pass_samples, min_items=args.serve_min_items)
     cfg = gpu.WavefrontConfig(**launch)
     cfg.path_slots = 0
     srv.set_wavefront_config(cfg)
@@ -293,7 +298,8 @@ def bench_serving(args, gpu, host, field, launch):
                msamples_s=round(samples.sum() / ((end - start) * 1e-3) / 1e6, 2), duration_s=round((end - start) * 1e-3, 2),
                latency_ms={"p50": round(float(np.percentile(L, 50)), 2), "p99": round(float(np.percentile(L, 99)), 2)},
                slowdown={"p50": round(float(np.percentile(slow, 50)), 2), "p99": round(float(np.percentile(slow, 99)), 2)},
-               by_job_size=by_size, path_slots=args.serve_slots, iterations_per_step=args.serve_iterations, schedule=args.serve_schedule)
+               by_job_size=by_size, path_slots=args.serve_slots, iterations_per_step=args.serve_iterations, schedule=args.serve_schedule,
+               pass_samples=args.serve_pass_samples, min_items=args.serve_min_items)
     worst = np.argsort(slow)[-5:][::-1]
     res["worst_slowdown"] = [dict(res=f"{jobs[k]['w']}x{jobs[k]['h']}", spp=jobs[k]["spp"], depth=jobs[k]["depth"], scene=jobs[k]["scene"],
                                   latency_ms=round(float(L[k]), 1), slowdown=round(float(slow[k]), 1)) for k in worst]
@@ -322,8 +328,9 @@ def measured_traffic(spp):
         return None
 
 
-def cpu_baseline(scene, cam, rs):
-    """The reference CPU renderer (or the oracle) on full 1080p at 1 spp with all host threads."""
+def cpu_baseline(scene, cam, rs, spp):
+    """The reference CPU renderer (or the oracle) on the full 1080p frame at `spp` samples with all host threads:
+    a bounded sample of the workload (about 20 s of CPU time at 4 spp on 16 threads)."""
     from oracle import oracle_py as O
     threads = os.cpu_count() or 1
     kind = "reference" if O.ref_available() else "port"
@@ -332,15 +339,15 @@ def cpu_baseline(scene, cam, rs):
         ref.build()
         ref.render(cam, rs, 1, threads)  # warm-up as the reference bench does (render_cli.cpp:319-327)
         t0 = time.perf_counter()
-        ref.render(cam, rs, 1, threads)
+        ref.render(cam, rs, spp, threads)
         dt = time.perf_counter() - t0
     else:
         O.render(scene.desc(), cam, rs, 1, threads=threads)
         t0 = time.perf_counter()
-        O.render(scene.desc(), cam, rs, 1, threads=threads)
+        O.render(scene.desc(), cam, rs, spp, threads=threads)
         dt = time.perf_counter() - t0
-    return dict(value=rs.width * rs.height / dt / 1e6, unit="Msamples/s", cores=threads, kind=kind,
-                sample=f"{rs.width}x{rs.height} at 1 spp, max depth {rs.path.max_depth}, same scene and sequences; {dt:.2f} s wall")
+    return dict(value=rs.width * rs.height * spp / dt / 1e6, unit="Msamples/s", cores=threads, kind=kind,
+                sample=f"{rs.width}x{rs.height} at {spp} spp, max depth {rs.path.max_depth}, same scene and sequences; {dt:.2f} s wall")
 
 
 def main():
@@ -354,6 +361,7 @@ def main():
     ap.add_argument("--megakernel-divisor", type=int, default=1, help="time the plain kernel on spp/divisor samples")
     ap.add_argument("--no-traversal", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-spp", type=int, default=4, help="samples per pixel of the CPU baseline's bounded sample")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-svgf", action="store_true")
     ap.add_argument("--svgf-frames", type=int, default=64)
@@ -363,6 +371,9 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=1 << 21)
     ap.add_argument("--serve-iterations", type=int, default=8)
     ap.add_argument("--serve-schedule", choices=("srpt", "fifo"), default="srpt")
+    ap.add_argument("--serve-pass-samples", type=int, default=0, help="pixel samples per job pass (0: the server's default)")
+    ap.add_argument("--serve-min-items", type=int, default=0, help="split smaller jobs into sample sub-streams (0: the server's default)")
+    ap.add_argument("--launch", default="", help="wavefront launch configuration as key=value,... (skips the autotuner)")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (the known-size kernel that scales counter readings)")
     args = ap.parse_args()
@@ -386,7 +397,7 @@ def main():
     trav = None if args.no_traversal else bench_traversal(gpu, host)
     svgf = None if args.no_svgf else bench_svgf(args, gpu, host, scene)
     serve = None if args.no_serve else bench_serving(args, gpu, host, scene, rnd["config"])
-    cpu = None if args.no_cpu else cpu_baseline(scene, cam, rs)
+    cpu = None if args.no_cpu else cpu_baseline(scene, cam, rs, args.cpu_spp)
     line = {
         "metric": METRIC,
         "value": round(rnd["value"], 3),
