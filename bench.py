@@ -227,7 +227,9 @@ def bench_serving(args, gpu, host, field, launch):
     log(f"serving scenes: {[s.num_tris for s in scenes]} triangles, setup {time.time() - t:.1f} s")
     srv = gpu.Server(path_slots=args.serve_slots, iterations_per_step=args.serve_iterations,
                      schedule=gpu.SCHEDULE_FIFO if args.serve_schedule == "fifo" else gpu.SCHEDULE_SRPT,
-                     pass_samples=args.serve_pass_samples, min_items=args.serve_min_items)
+                     pass_samples=args.serve_pass_samples, min_items=args.serve_min_items,
+                     express_slots=args.serve_express_slots, express_samples=args.serve_express_samples,
+                     express_iterations=args.serve_express_iterations)
     cfg = gpu.WavefrontConfig(**launch)
     cfg.path_slots = 0
     srv.set_wavefront_config(cfg)
@@ -283,11 +285,20 @@ def bench_serving(args, gpu, host, field, launch):
         elif nxt < len(jobs):
             time.sleep(max(0.0, (arrival[nxt] - srv.now_ms()) * 1e-3))
     end = srv.now_ms()
+    # Isolated service time: each job again, alone on the idle server (submission to completion).
+    iso = np.zeros(len(jobs))
+    for k, job in enumerate(jobs):
+        j = submit(job)
+        while srv.status(j).state != gpu.JOB_DONE:
+            srv.step()
+        st = srv.status(j)
+        iso[k] = st.done_ms - st.submit_ms
+        srv.release(j)
     srv.close()
     L = np.array([lat[k] for k in range(len(jobs))])
     samples = np.array([j["w"] * j["h"] * j["spp"] for j in jobs], np.float64)
-    ideal = samples / peak * 1e3  # ms at the saturated rate
-    slow = L / ideal
+    slow = L / iso
+    slow_rate = L / (samples / peak * 1e3)  # against the job's work at the saturated rate
     by_size = {}
     for lo, hi, name in ((0, 2e6, "<2M samples"), (2e6, 5e7, "2M-50M"), (5e7, 1e12, ">50M")):
         sel = (samples >= lo) & (samples < hi)
@@ -298,11 +309,15 @@ def bench_serving(args, gpu, host, field, launch):
                msamples_s=round(samples.sum() / ((end - start) * 1e-3) / 1e6, 2), duration_s=round((end - start) * 1e-3, 2),
                latency_ms={"p50": round(float(np.percentile(L, 50)), 2), "p99": round(float(np.percentile(L, 99)), 2)},
                slowdown={"p50": round(float(np.percentile(slow, 50)), 2), "p99": round(float(np.percentile(slow, 99)), 2)},
+               slowdown_vs_peak_rate={"p50": round(float(np.percentile(slow_rate, 50)), 2), "p99": round(float(np.percentile(slow_rate, 99)), 2)},
+               isolated_ms={"p50": round(float(np.percentile(iso, 50)), 2), "p99": round(float(np.percentile(iso, 99)), 2)},
                by_job_size=by_size, path_slots=args.serve_slots, iterations_per_step=args.serve_iterations, schedule=args.serve_schedule,
-               pass_samples=args.serve_pass_samples, min_items=args.serve_min_items)
+               pass_samples=args.serve_pass_samples, min_items=args.serve_min_items, express_slots=args.serve_express_slots,
+               express_samples=args.serve_express_samples, express_iterations=args.serve_express_iterations)
     worst = np.argsort(slow)[-5:][::-1]
     res["worst_slowdown"] = [dict(res=f"{jobs[k]['w']}x{jobs[k]['h']}", spp=jobs[k]["spp"], depth=jobs[k]["depth"], scene=jobs[k]["scene"],
-                                  latency_ms=round(float(L[k]), 1), slowdown=round(float(slow[k]), 1)) for k in worst]
+                                  latency_ms=round(float(L[k]), 1), isolated_ms=round(float(iso[k]), 1), slowdown=round(float(slow[k]), 1))
+                             for k in worst]
     res["latency_p99_over_p50"] = round(res["latency_ms"]["p99"] / res["latency_ms"]["p50"], 2)
     res["slowdown_p99_over_p50"] = round(res["slowdown"]["p99"] / res["slowdown"]["p50"], 2)
     log(f"serving: {res}")
@@ -373,6 +388,9 @@ def main():
     ap.add_argument("--serve-schedule", choices=("srpt", "fifo"), default="srpt")
     ap.add_argument("--serve-pass-samples", type=int, default=0, help="pixel samples per job pass (0: the server's default)")
     ap.add_argument("--serve-min-items", type=int, default=0, help="split smaller jobs into sample sub-streams (0: the server's default)")
+    ap.add_argument("--serve-express-slots", type=int, default=0, help="path slots of the express lane (0: the server's default, -1: none)")
+    ap.add_argument("--serve-express-samples", type=int, default=0, help="largest job (samples) of the express lane (0: the server's default)")
+    ap.add_argument("--serve-express-iterations", type=int, default=0, help="bounces per express-lane run (0: as the main lane)")
     ap.add_argument("--launch", default="", help="wavefront launch configuration as key=value,... (skips the autotuner)")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (the known-size kernel that scales counter readings)")

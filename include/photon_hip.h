@@ -156,8 +156,9 @@ float ph_svgf_last_ms(const ph_svgf* f);
 
 /* Serving: several scenes stay resident; render jobs of mixed resolution, sample count and depth are
  * admitted by HBM footprint while others run, and their paths share the same wavefronts (continuous
- * batching). A job's image depends only on the job and the server's min_items, so it is identical
- * whether the job runs alone or among others; with min_items = 1 it is bit-identical to the renderer's.
+ * batching). A job's image depends only on the job and the server's configuration (how small jobs are
+ * split, which lane runs them), so it is identical whether the job runs alone or among others; with
+ * min_items = 1 it is bit-identical to the renderer's.
  * No counterpart in the reference. */
 typedef struct ph_server ph_server;
 
@@ -169,7 +170,13 @@ typedef struct ph_server_config {
     int64_t pass_samples;        /* pixel samples per pass of a job (0: 4 x path_slots) */
     int32_t schedule;            /* PH_SCHEDULE_SRPT or PH_SCHEDULE_FIFO */
     int32_t min_items;           /* a job with fewer pixels runs each pixel's samples as sub-streams so that
-                                    pixels x sub-streams reaches this (0: path_slots / 8; 1: never split) */
+                                    pixels x sub-streams reaches this (0: an eighth of the main lane's slots,
+                                    all of the express lane's; -1 or 1: never split) */
+    int64_t express_samples;     /* jobs of at most this many samples run in the express lane
+                                    (0: 8 x express_slots) */
+    int32_t express_slots;       /* path slots of the express lane, a second engine on its own stream that
+                                    runs concurrently with the main lane (0: path_slots / 8; -1: no lane) */
+    int32_t express_iterations;  /* bounces per run of the express lane (0: iterations_per_step) */
 } ph_server_config;
 
 #define PH_SCHEDULE_SRPT 0 /* unstarted work of the job with the least remaining work first */
@@ -205,7 +212,8 @@ int ph_server_submit(ph_server* s, const ph_job_desc* job);
 /* Admits queued jobs that fit, advances all paths in flight by iterations_per_step bounces (blocking),
  * queues the next pass of every job whose pass finished, and returns the number of jobs finished since
  * the server started. A job runs as passes of at most pass_samples pixel samples; unstarted work is
- * handed to path slots in the order of the schedule. */
+ * handed to path slots in the order of the schedule. While the main lane runs, the express lane runs
+ * as many times as it has work, so its jobs can finish within one step. */
 int ph_server_step(ph_server* s);
 int ph_server_status(const ph_server* s, int job, ph_job_status* out);
 /* Average radiance of a finished job (3 floats per pixel); frees its frame buffer. */

@@ -114,99 +114,112 @@ PH_DEV void stage_top_nodes(float4* top, const float4* nodes, int n) {
     __syncthreads();
 }
 
-// One ray's traversal position: walk_step visits one node at a time, so that a lane can start a new
-// ray as soon as its own finishes (the persistent kernels refill lanes between steps).
+// One ray's traversal position. walk_step does one unit of work per call: either it fetches the node
+// at `node` and slab-tests its two children, or it tests one triangle of those children's leaves. A
+// wave whose lanes stand at different points therefore runs one node fetch and one triangle test per
+// step (both loads in flight together) instead of waiting for its longest leaf list, and a lane can
+// start a new ray as soon as its own finishes (the persistent kernels refill lanes between steps).
+// Every ray still performs exactly the reference's tests in the reference's order.
 struct Walker {
     f3 o, d, inv;
     float tmax;
     int node, sp;
     bool found;
+    // The fetched node whose children are pending (pos >= 0): child links and counts as stored in the
+    // node, child entry distances (kInf: missed), and how many leaf triangles have been tested so far.
+    int pos;
+    int lidx, ridx, lcount, rcount;
+    float tl, tr;
 };
 
-PH_DEV Walker walker(f3 o, f3 d, float tmax) { return Walker{o, d, guarded_reciprocal(d), tmax, 0, 0, false}; }
+PH_DEV Walker walker(f3 o, f3 d, float tmax) {
+    Walker w{};
+    w.o = o;
+    w.d = d;
+    w.inv = guarded_reciprocal(d);
+    w.tmax = tmax;
+    w.pos = -1;
+    return w;
+}
 
-// Visits the node at w.node: tests both child boxes against the current tmax, intersects leaf
-// children in place, then moves to the nearer interior child (pushing the farther) or pops the stack.
-// Returns false once the traversal is over (any-hit: at the first accepted triangle).
+// One unit of the traversal of w (see Walker). Returns false once the traversal is over (any-hit: at
+// the first accepted triangle). Testing one triangle per call also for coherent rays, whose lanes reach
+// their leaves together, measured faster than looping over the whole leaf list when no lane of the wave
+// needs a node fetch.
 template <bool AnyHit, bool Count, class Stack>
 PH_DEV bool walk_step(const TraceScene& ts, Walker& w, HitRec& hit, Stack& stack, TraceCount* cnt) {
-    float4 lmin, lmax, rmin, rmax;
-    if (w.node < ts.top_n) {
-        const float4* np = ts.top + 4 * w.node;
-        lmin = np[0];
-        lmax = np[1];
-        rmin = np[2];
-        rmax = np[3];
+    if (w.pos < 0) {
+        // Fetch the node and test both child boxes against the tmax at node entry.
+        float4 lmin, lmax, rmin, rmax;
+        if (w.node < ts.top_n) {
+            const float4* np = ts.top + 4 * w.node;
+            lmin = np[0];
+            lmax = np[1];
+            rmin = np[2];
+            rmax = np[3];
+        } else {
+            const float4* np = ts.nodes + 4 * w.node;
+            lmin = ldg(np);
+            lmax = ldg(np + 1);
+            rmin = ldg(np + 2);
+            rmax = ldg(np + 3);
+        }
+        if (Count) cnt->nodes++;
+        w.tl = slab(lmin, lmax, w.o, w.inv, w.tmax);
+        w.tr = slab(rmin, rmax, w.o, w.inv, w.tmax);
+        w.lcount = __float_as_int(lmax.w);
+        w.rcount = __float_as_int(rmax.w);
+        w.lidx = __float_as_int(lmin.w);
+        w.ridx = __float_as_int(rmin.w);
+        w.pos = 0;
     } else {
-        const float4* np = ts.nodes + 4 * w.node;
-        lmin = ldg(np);
-        lmax = ldg(np + 1);
-        rmin = ldg(np + 2);
-        rmax = ldg(np + 3);
-    }
-    if (Count) cnt->nodes++;
-    const float tl = slab(lmin, lmax, w.o, w.inv, w.tmax);
-    const float tr = slab(rmin, rmax, w.o, w.inv, w.tmax);
-    const int lcount = __float_as_int(lmax.w), rcount = __float_as_int(rmax.w);
-    const int lidx = __float_as_int(lmin.w), ridx = __float_as_int(rmin.w);
-    // Leaf children: the left leaf's triangles, then the right leaf's, as one loop (a wave runs it for
-    // the longest lane's combined count instead of the longest left plus the longest right).
-    const int nl = (tl != kInf && lcount > 0) ? lcount : 0;
-    const int nr = (tr != kInf && rcount > 0) ? rcount : 0;
-    for (int t = 0; t < nl + nr; t++) {
-        const int i = t < nl ? lidx + t : ridx + (t - nl);
+        // Test the next leaf triangle: the left leaf's in order, then the right leaf's.
+        const int nl = (w.tl != kInf && w.lcount > 0) ? w.lcount : 0;
+        const int i = w.pos < nl ? w.lidx + w.pos : w.ridx + (w.pos - nl);
+        w.pos++;
         const float4* tp = ts.tris + 3 * i;
         const float4 a = ldg(tp), b = ldg(tp + 1), c = ldg(tp + 2);
         float th, u, v;
         if (Count) cnt->tris++;
         if (mt_hit(xyz(a), xyz(b), xyz(c), w.o, w.d, w.tmax, th, u, v)) {
+            bool cut = false;
             if (__float_as_int(b.w) & PH_TRI_ALPHA_TESTED) {
                 if (Count) cnt->alpha++;
-                if (alpha_cut(*ts.full, i, __float_as_int(a.w), u, v)) continue;
+                cut = alpha_cut(*ts.full, i, __float_as_int(a.w), u, v);
             }
-            hit.t = th;
-            hit.tri = i;
-            w.found = true;
-            if (AnyHit) return false;
-            w.tmax = th;
-            hit.u = u;
-            hit.v = v;
+            if (!cut) {
+                hit.t = th;
+                hit.tri = i;
+                w.found = true;
+                if (AnyHit) return false;
+                w.tmax = th;
+                hit.u = u;
+                hit.v = v;
+            }
         }
     }
-    // Interior children, left before right.
-    int next0 = 0, next1 = 0;
-    float tn0 = 0, tn1 = 0;
-    int nn = 0;
-    if (tl != kInf && lcount <= 0 && lidx >= 0) {
-        next0 = lidx;
-        tn0 = tl;
-        nn = 1;
-    }
-    if (tr != kInf && rcount <= 0 && ridx >= 0) {
-        if (nn == 0) {
-            next0 = ridx;
-            tn0 = tr;
-        } else {
-            next1 = ridx;
-            tn1 = tr;
-        }
-        nn++;
-    }
-    if (nn == 2) {
+    const int nl = (w.tl != kInf && w.lcount > 0) ? w.lcount : 0;
+    const int nr = (w.tr != kInf && w.rcount > 0) ? w.rcount : 0;
+    if (w.pos < nl + nr) return true;
+    // Leaves done: interior children against the shortened tmax, nearest first (left on ties), the
+    // farther one pushed.
+    w.pos = -1;
+    const bool li = w.tl != kInf && w.lcount <= 0 && w.lidx >= 0;
+    const bool ri = w.tr != kInf && w.rcount <= 0 && w.ridx >= 0;
+    int next0 = li ? w.lidx : w.ridx;
+    float tn0 = li ? w.tl : w.tr;
+    if (li && ri) {
+        int next1 = w.ridx;
+        float tn1 = w.tr;
         if (tn0 > tn1) {
-            int a = next0;
-            next0 = next1;
-            next1 = a;
-            float b = tn0;
-            tn0 = tn1;
-            tn1 = b;
+            next1 = next0;
+            next0 = w.ridx;
+            tn1 = tn0;
+            tn0 = w.tr;
         }
         if (tn1 < w.tmax && w.sp < kMaxStack) stack.store(w.sp++, next1);
-        if (tn0 < w.tmax) {
-            w.node = next0;
-            return true;
-        }
-    } else if (nn == 1 && tn0 < w.tmax) {
+    }
+    if ((li || ri) && tn0 < w.tmax) {
         w.node = next0;
         return true;
     }

@@ -1,5 +1,5 @@
 // The serving loop (photon_hip.h, ph_server_*): resident scenes, FIFO admission of render jobs by HBM
-// footprint, and continuous batching of all admitted jobs' paths in one wavefront engine.
+// footprint, and continuous batching of all admitted jobs' paths in shared wavefronts.
 //
 // A step admits the queued jobs that fit (frame buffer bytes within the budget, a free job-table entry),
 // then advances every path in flight by a fixed number of bounces. A job runs as passes of at most
@@ -8,14 +8,22 @@
 // not stuck behind a long one. Jobs finish independently, and a finished job's image can be read while
 // the others keep running.
 //
+// Jobs of at most `express_samples` samples run in an express lane: a second engine with a small slot
+// pool on a stream of its own, which runs concurrently with the main lane and as often as it has work
+// during the main lane's run. Its iterations cover few paths and are short, so a short job's latency is
+// set by its own depth rather than by iterations over millions of other paths.
+//
 // A job with fewer than `min_items` pixels runs each pixel's samples as S sub-streams (S divides spp)
 // in separate partial images, which the result adds in sub-stream order: its paths can then fill more
 // slots than it has pixels. Within a sub-stream the samples are taken in order, pass after pass. An
-// image therefore depends only on the job and the server's min_items, never on what else shared the
-// wavefronts; with S = 1 it is bit-identical to the renderer's.
+// image therefore depends only on the job and the server's configuration, never on what else shared
+// the wavefronts; with S = 1 it is bit-identical to the renderer's.
+#include <atomic>
 #include <chrono>
 #include <deque>
+#include <exception>
 #include <memory>
+#include <thread>
 #include <vector>
 
 #include "device_scene.h"
@@ -49,46 +57,77 @@ struct Job {
 
 }  // namespace
 
-struct ph_server {
-    ph_server_config cfg{};
-    hipStream_t stream = nullptr;
+// One engine with its own stream and slots. The main lane runs every job by default; the express lane
+// (when configured) runs the jobs of at most `express_samples` samples on a small slot pool of its own,
+// concurrently with the main lane, so that a short job's bounces are not each as long as an iteration
+// over millions of paths.
+struct Lane {
     WavefrontEngine engine;
-    std::vector<std::unique_ptr<ServedScene>> scenes;
-    std::vector<std::unique_ptr<Job>> jobs;
+    hipStream_t stream = nullptr;
+    int slots = 0;           // 0: lane off
+    int min_items = 1;       // split threshold of this lane's jobs
+    int iterations = 1;      // bounces per engine run
+    int64_t pass_samples = 0;
     std::deque<int> queue;   // submitted, not admitted (FIFO)
     std::vector<int> running;
+};
+
+struct ph_server {
+    ph_server_config cfg{};
+    int device = 0;
+    Lane lanes[2];  // 0: main, 1: express
+    std::vector<std::unique_ptr<ServedScene>> scenes;
+    std::vector<std::unique_ptr<Job>> jobs;
     int64_t mem_used = 0;
     int finished = 0;
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
 
     double now_ms() const { return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); }
+    hipStream_t stream() const { return lanes[0].stream; }
 
     ph_server(const ph_server_config& c) : cfg(c) {
         if (cfg.memory_budget <= 0) cfg.memory_budget = int64_t(16) << 30;
         if (cfg.path_slots <= 0) cfg.path_slots = 1 << 21;
         if (cfg.iterations_per_step <= 0) cfg.iterations_per_step = 8;
         if (cfg.max_jobs <= 0 || cfg.max_jobs > WavefrontEngine::kMaxJobs) cfg.max_jobs = WavefrontEngine::kMaxJobs;
-        if (cfg.pass_samples <= 0) cfg.pass_samples = int64_t(4) * cfg.path_slots;
         if (cfg.schedule != PH_SCHEDULE_SRPT && cfg.schedule != PH_SCHEDULE_FIFO) throw std::invalid_argument("unknown schedule");
-        if (cfg.min_items == 0) cfg.min_items = std::max(1, cfg.path_slots / 8);
-        if (cfg.min_items < 0) cfg.min_items = 1;
-        PH_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-        engine.serve_begin(stream, cfg.path_slots);
-    }
-    ~ph_server() {
-        if (stream) {
-            (void)hipStreamSynchronize(stream);
-            (void)hipStreamDestroy(stream);
+        if (cfg.express_slots == 0) cfg.express_slots = std::max(1, cfg.path_slots / 8);
+        if (cfg.express_slots < 0) cfg.express_slots = 0;
+        if (cfg.express_samples <= 0) cfg.express_samples = int64_t(8) * cfg.express_slots;
+        if (cfg.express_iterations <= 0) cfg.express_iterations = cfg.iterations_per_step;
+        PH_CHECK(hipGetDevice(&device));
+        const int slots[2] = {cfg.path_slots, cfg.express_slots};
+        for (int k = 0; k < 2; k++) {
+            Lane& L = lanes[k];
+            L.slots = slots[k];
+            if (L.slots == 0) continue;
+            L.iterations = k == 0 ? cfg.iterations_per_step : cfg.express_iterations;
+            L.pass_samples = cfg.pass_samples > 0 ? cfg.pass_samples : int64_t(4) * L.slots;
+            // Small jobs are split until they fill an eighth of the main lane, or all of the express lane.
+            L.min_items = cfg.min_items > 0 ? cfg.min_items : cfg.min_items < 0 ? 1 : (k == 0 ? std::max(1, L.slots / 8) : L.slots);
+            PH_CHECK(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
+            L.engine.serve_begin(L.stream, L.slots);
         }
     }
+    ~ph_server() {
+        for (Lane& L : lanes)
+            if (L.stream) {
+                (void)hipStreamSynchronize(L.stream);
+                (void)hipStreamDestroy(L.stream);
+            }
+    }
 
-    void admit() {
-        while (!queue.empty() && int(running.size()) < cfg.max_jobs) {
-            Job& j = *jobs[size_t(queue.front())];
+    int lane_of(const Job& j) const { return lanes[1].slots > 0 && j.status.samples <= cfg.express_samples ? 1 : 0; }
+
+    bool idle() const { return lanes[0].running.empty() && lanes[1].running.empty(); }
+
+    void admit(Lane& L) {
+        while (!L.queue.empty() && int(L.running.size()) < cfg.max_jobs) {
+            Job& j = *jobs[size_t(L.queue.front())];
             // A job larger than the whole budget is admitted alone rather than never.
-            if (mem_used + int64_t(j.bytes) > cfg.memory_budget && !running.empty()) break;
+            if (mem_used + int64_t(j.bytes) > cfg.memory_budget && !idle()) break;
             j.accum.allocate(size_t(j.items));
-            j.accum.zero(stream);
+            j.accum.zero(L.stream);
             const ServedScene& sc = *scenes[size_t(j.desc.scene)];
             WavefrontJob w{};
             w.scene = sc.scene.view();
@@ -104,7 +143,7 @@ struct ph_server {
             w.nd = nullptr;
             w.mat_class = sc.mat_class.ptr;
             w.split = j.split;
-            const int entry = engine.serve_admit(stream, w);
+            const int entry = L.engine.serve_admit(L.stream, w);
             if (entry < 0) {
                 j.accum.release();
                 break;
@@ -112,21 +151,21 @@ struct ph_server {
             j.entry = entry;
             j.status.state = PH_JOB_RUNNING;
             j.status.admit_ms = now_ms();
-            const int64_t per_pass = cfg.pass_samples / std::max<int64_t>(1, j.items);
+            const int64_t per_pass = L.pass_samples / std::max<int64_t>(1, j.items);
             j.pass_spp = int(std::min<int64_t>(std::max<int64_t>(per_pass, 1), j.desc.spp / j.split));
-            queue_next_pass(j);
+            queue_next_pass(L, j);
             mem_used += int64_t(j.bytes);
-            running.push_back(queue.front());
-            queue.pop_front();
+            L.running.push_back(L.queue.front());
+            L.queue.pop_front();
         }
     }
 
-    void queue_next_pass(Job& j) {
+    void queue_next_pass(Lane& L, Job& j) {
         const int span = j.desc.spp / j.split;
         const int k0 = j.next_k, k1 = std::min(span, k0 + j.pass_spp);
         // Shortest remaining work first, or passes in the order they are queued.
         const double priority = cfg.schedule == PH_SCHEDULE_FIFO ? 0.0 : double(j.items) * double(span - k0);
-        if (!engine.serve_enqueue(j.entry, k0, k1, priority)) {
+        if (!L.engine.serve_enqueue(j.entry, k0, k1, priority)) {
             j.pass_pending = true;
             return;
         }
@@ -137,27 +176,24 @@ struct ph_server {
         j.passes_queued++;
     }
 
-    void step() {
-        admit();
-        if (running.empty()) return;
-        engine.serve_run(stream, cfg.iterations_per_step);
-        const double t = now_ms();
+    // After a run of the lane's engine: progress of its jobs, next passes, completions at time t.
+    void advance(Lane& L, double t) {
         std::vector<int> still;
-        for (int id : running) {
+        for (int id : L.running) {
             Job& j = *jobs[size_t(id)];
             const int64_t px = j.items;
-            const int64_t done = engine.serve_pixels_done(j.entry);  // finished items, all passes so far
+            const int64_t done = L.engine.serve_pixels_done(j.entry);  // finished items, all passes so far
             const int64_t in_pass = done - px * (j.passes_queued - 1);
             j.status.samples_done = px * j.pass_k0 + in_pass * (j.pass_k1 - j.pass_k0);
             if (j.pass_pending) {
-                queue_next_pass(j);
+                queue_next_pass(L, j);
             } else if (done == px * j.passes_queued) {
                 if (j.next_k < j.desc.spp / j.split) {
-                    queue_next_pass(j);
+                    queue_next_pass(L, j);
                 } else {
                     j.status.state = PH_JOB_DONE;
                     j.status.done_ms = t;
-                    engine.serve_release(j.entry);
+                    L.engine.serve_release(j.entry);
                     j.entry = -1;
                     finished++;
                     continue;
@@ -165,7 +201,48 @@ struct ph_server {
             }
             still.push_back(id);
         }
-        running.swap(still);
+        L.running.swap(still);
+    }
+
+    void step() {
+        Lane& M = lanes[0];
+        Lane& X = lanes[1];
+        admit(M);
+        if (X.slots > 0) admit(X);
+        const bool run_m = !M.running.empty(), run_x = !X.running.empty();
+        if (run_m && run_x) {
+            // The main lane's run on a thread of its own; meanwhile the express lane runs as many times
+            // as it has work, finishing its jobs as it goes. The lanes share nothing but the device.
+            std::atomic<bool> m_done{false};
+            double m_end = 0.0;
+            std::exception_ptr m_err;
+            std::thread th([&] {
+                try {
+                    PH_CHECK(hipSetDevice(device));
+                    M.engine.serve_run(M.stream, M.iterations);
+                } catch (...) {
+                    m_err = std::current_exception();
+                }
+                m_end = now_ms();
+                m_done.store(true, std::memory_order_release);
+            });
+            try {
+                do {
+                    X.engine.serve_run(X.stream, X.iterations);
+                    advance(X, now_ms());
+                } while (!m_done.load(std::memory_order_acquire) && !X.running.empty());
+            } catch (...) {
+                th.join();
+                throw;
+            }
+            th.join();
+            if (m_err) std::rethrow_exception(m_err);
+            advance(M, m_end);
+        } else if (run_m || run_x) {
+            Lane& L = run_m ? M : X;
+            L.engine.serve_run(L.stream, L.iterations);
+            advance(L, now_ms());
+        }
     }
 };
 
@@ -186,7 +263,8 @@ void ph_server_destroy(ph_server* s) { delete s; }
 int ph_server_set_wavefront_config(ph_server* s, const ph_wavefront_config* cfg) {
     if (!s || !cfg) return set_error("null argument");
     return guarded([&] {
-        s->engine.set_config(*cfg);
+        for (Lane& L : s->lanes)
+            if (L.slots > 0) L.engine.set_config(*cfg);
         return 0;
     });
 }
@@ -196,10 +274,10 @@ int ph_server_add_scene(ph_server* s, const ph_scene_desc* d) {
     return guarded([&] {
         DeviceScene::validate(d);
         auto sc = std::make_unique<ServedScene>();
-        sc->scene.upload(*d, s->stream);
-        std::vector<int> cls = material_classes(sc->scene.view(), s->stream);
-        sc->mat_class.upload(cls.data(), cls.size(), s->stream);
-        PH_CHECK(hipStreamSynchronize(s->stream));
+        sc->scene.upload(*d, s->stream());
+        std::vector<int> cls = material_classes(sc->scene.view(), s->stream());
+        sc->mat_class.upload(cls.data(), cls.size(), s->stream());
+        PH_CHECK(hipStreamSynchronize(s->stream()));
         s->scenes.push_back(std::move(sc));
         return int(s->scenes.size()) - 1;
     });
@@ -219,15 +297,16 @@ int ph_server_submit(ph_server* s, const ph_job_desc* d) {
         j->status.pixels = d->width * d->height;
         j->status.samples = int64_t(j->status.pixels) * d->spp;
         j->status.submit_ms = s->now_ms();
-        // Sub-streams: the largest divisor of spp that keeps pixels x split within min_items (or 1).
-        const int64_t want = (int64_t(s->cfg.min_items) + j->status.pixels - 1) / j->status.pixels;
+        Lane& L = s->lanes[s->lane_of(*j)];
+        // Sub-streams: the largest divisor of spp that keeps pixels x split within the lane's min_items (or 1).
+        const int64_t want = (int64_t(L.min_items) + j->status.pixels - 1) / j->status.pixels;
         for (int k = 1; k <= d->spp && k <= want; k++)
             if (d->spp % k == 0) j->split = k;
         j->items = int64_t(j->status.pixels) * j->split;
         j->bytes = size_t(j->items) * sizeof(float4);
         s->jobs.push_back(std::move(j));
         const int id = int(s->jobs.size()) - 1;
-        s->queue.push_back(id);
+        L.queue.push_back(id);
         return id;
     });
 }
@@ -255,8 +334,8 @@ int ph_server_result(ph_server* s, int job, float* rgb) {
     return guarded([&] {
         const size_t n = size_t(j.status.pixels);
         std::vector<float4> raw(size_t(j.items));
-        PH_CHECK(hipMemcpyAsync(raw.data(), j.accum.ptr, raw.size() * sizeof(float4), hipMemcpyDeviceToHost, s->stream));
-        PH_CHECK(hipStreamSynchronize(s->stream));
+        PH_CHECK(hipMemcpyAsync(raw.data(), j.accum.ptr, raw.size() * sizeof(float4), hipMemcpyDeviceToHost, s->stream()));
+        PH_CHECK(hipStreamSynchronize(s->stream()));
         const float inv = 1.0f / float(j.desc.spp);
         for (size_t i = 0; i < n; i++) {
             // The partial images in sub-stream order (one plane: the accumulation itself).
@@ -289,7 +368,12 @@ int ph_server_release(ph_server* s, int job) {
     return 0;
 }
 
-int ph_server_pending(const ph_server* s) { return s ? int(s->queue.size() + s->running.size()) : set_error("null server"); }
+int ph_server_pending(const ph_server* s) {
+    if (!s) return set_error("null server");
+    size_t n = 0;
+    for (const Lane& L : s->lanes) n += L.queue.size() + L.running.size();
+    return int(n);
+}
 
 double ph_server_now_ms(const ph_server* s) { return s ? s->now_ms() : 0.0; }
 

@@ -156,6 +156,20 @@ PH_DEV long long wave_take(unsigned long long* counter, bool pred) {
     return (long long)first + __popcll(mask & ((1ull << lane) - 1ull));
 }
 
+// Adds to counters[key] the number of lanes with `pred` and that key: one atomic per distinct key in
+// the wave instead of one per lane (lanes finishing work for one job would otherwise serialize on its
+// counter). All lanes must call.
+PH_DEV void wave_count(int* counters, int key, bool pred) {
+    uint64_t mask = __ballot(pred);
+    while (mask) {
+        const int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+        const int k = __shfl(key, leader);
+        const uint64_t same = __ballot(pred && key == k) & mask;
+        if (lane_id() == leader) atomicAdd(counters + k, __popcll(same));
+        mask &= ~same;
+    }
+}
+
 PH_DEV long long items_next(const Work& work) {
     return (long long)__hip_atomic_load(work.items, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -628,7 +642,8 @@ __global__ __launch_bounds__(256) void regen_kernel(Work work, State st, int* co
         const int i = (ws.first + b) * 64 + lane_id();
         const bool active = i < n;
         const int slot = active ? st.q_term[i] : 0;
-        bool need_work = false, ext = false;
+        bool need_work = false, ext = false, item_done = false;
+        int job = 0;
         if (active && (!Multi || (st.stage[slot] & S_CODE) == S_TERM)) {
             const JobParams& jp = job_of<Multi>(work, st, slot, single);
             const int pixel = st.pixel[slot];
@@ -646,7 +661,8 @@ __global__ __launch_bounds__(256) void regen_kernel(Work work, State st, int* co
             k++;
             need_work = k >= (Multi ? st.sample_end[slot] : jp.spp);
             if (need_work) {
-                atomicAdd(work.job_done + (Multi ? st.job[slot] : 0), 1);
+                item_done = true;
+                if (Multi) job = st.job[slot];
             } else {
                 start_sample(jp, st, slot, pixel, k);
                 ext = true;
@@ -654,6 +670,7 @@ __global__ __launch_bounds__(256) void regen_kernel(Work work, State st, int* co
         } else if (active) {
             need_work = true;  // an idle slot
         }
+        wave_count(work.job_done, job, item_done);
         // Skip the atomic when every work item has been handed out (the counter only grows).
         const bool want = need_work && items_next(work) < total;
         const long long w = wave_take(work.items, want);
@@ -1111,7 +1128,7 @@ ph_wavefront_config WavefrontEngine::autotune(hipStream_t st, const WavefrontJob
             double t = time_of(c);
             if (t < best_ms) best_ms = t, best = c;
         }
-    for (int bpc : {2, 4, 8, 16}) {
+    for (int bpc : {2, 4, 8, 16, 24, 32}) {
         ph_wavefront_config c = best;
         c.blocks_per_cu = bpc;
         double t = time_of(c);

@@ -75,7 +75,10 @@ def test_split_jobs_match_the_renderer_and_themselves(scenes):
 @pytest.mark.parametrize("cfg", [dict(path_slots=1 << 14), dict(path_slots=3000, iterations_per_step=2),
                                  dict(path_slots=1 << 16, max_jobs=2), dict(path_slots=5000, memory_budget=64 * 64 * 16),
                                  dict(path_slots=4096, pass_samples=3000), dict(path_slots=2000, pass_samples=1),
-                                 dict(path_slots=3000, pass_samples=5000, schedule=gpu.SCHEDULE_FIFO)])
+                                 dict(path_slots=3000, pass_samples=5000, schedule=gpu.SCHEDULE_FIFO),
+                                 dict(path_slots=1 << 14, express_slots=-1),
+                                 dict(path_slots=4096, express_slots=512, express_samples=40000, express_iterations=1),
+                                 dict(path_slots=2048, express_slots=2048, express_samples=12000, iterations_per_step=1)])
 def test_jobs_in_a_mix_equal_jobs_alone(scenes, cfg):
     # The same splitting of small jobs in both runs (it is part of what defines a job's image).
     alone = {k: _serve(scenes, JOBS, [k], path_slots=1 << 14, min_items=2048)[k] for k in range(len(JOBS))}
@@ -104,6 +107,28 @@ def test_jobs_submitted_while_others_run(scenes):
         n, w, h, spp, depth, seed = JOBS[k]
         ref = _alone(scenes[n], w, h, spp, depth, seed)
         assert srv.result(j).tobytes() == ref.tobytes(), f"job {k}"
+    srv.close()
+
+
+def test_express_lane_runs_beside_the_main_lane(scenes):
+    """Small jobs in the express lane finish while a large job holds the main lane."""
+    big = ("materials", 320, 240, 64, 8, 7)
+    jobs = JOBS + [big]
+    srv = gpu.Server(path_slots=1 << 14, express_slots=4096, express_samples=40000, min_items=1)
+    ids = {n: srv.add_scene(s) for n, s in scenes.items()}
+    n, w, h, spp, depth, seed = big
+    jb = srv.submit(ids[n], w, h, spp, scenes[n].camera_view(w / h), max_depth=depth, seed=seed)
+    js = {}
+    for k, (n, w, h, spp, depth, seed) in enumerate(JOBS):
+        js[k] = srv.submit(ids[n], w, h, spp, scenes[n].camera_view(w / h), max_depth=depth, seed=seed)
+    srv.run_until_idle()
+    small_done = max(srv.status(j).done_ms for j in js.values())
+    assert small_done < srv.status(jb).done_ms
+    for k, j in js.items():
+        n, w, h, spp, depth, seed = jobs[k]
+        assert srv.result(j).tobytes() == _alone(scenes[n], w, h, spp, depth, seed).tobytes(), f"job {k}"
+    n, w, h, spp, depth, seed = big
+    assert srv.result(jb).tobytes() == _alone(scenes[n], w, h, spp, depth, seed).tobytes()
     srv.close()
 
 
