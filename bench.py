@@ -32,6 +32,13 @@ RAY_IO, NODE_B, TRI_B, ALPHA_B = 28 + 16, 64, 48, 80 + 12 + 24 + 16
 # filtered, illumination, moments, normal, position in, 4 out (9 x 16); remodulate: filtered, colour,
 # albedo, emission, position in, 1 out (6 x 16). Total 816 bytes.
 SVGF_BYTES_PER_PIXEL = 11 * 16 + 5 * 16 + 5 * 4 * 16 + 9 * 16 + 6 * 16
+# The same count for the workload's compact G-buffer (12-byte colour, albedo and emission, 4-byte depth,
+# normal code and material; float4 internal planes; the previous frame's geometry kept as depth, normal
+# code and material). temporal: 48 current + 44 history in, 32 out (124); variance: illumination,
+# moments, geometry in, 1 out (60); a-trous: filtered + geometry in, 1 out (44 each); history: filtered,
+# illumination, moments, geometry in, 44 out (104); remodulate: material, filtered, albedo, emission,
+# colour in, 1 out (72). Total 580 bytes.
+SVGF_COMPACT_BYTES_PER_PIXEL = 124 + 60 + 5 * 44 + 104 + 72
 
 
 def log(msg):
@@ -158,7 +165,9 @@ def bench_traversal(gpu, host):
 
 
 def bench_svgf(args, gpu, host, field):
-    """Real-time sequence: 1 spp G-buffer trace + SVGF per frame at 1600x900, camera orbiting 0.5 deg/frame."""
+    """Real-time sequence at 1600x900, camera orbiting 0.5 deg/frame: a 1-spp sample into the workload's
+    compact G-buffer, then SVGF on it, per frame; then the reference's float4 G-buffer path
+    (render_denoised) on the same sequence for comparison."""
     import math
     from photon_amd.types import RenderSettings, SvgfSettings
     W, H = 1600, 900
@@ -179,24 +188,38 @@ def bench_svgf(args, gpu, host, field):
         field.set_camera(p, target, cam["up"], cam["vfov"])
         return field.camera_view(W / H)
 
+    f = gpu.Svgf()
+    f.resize(W, H)
+    gb = gpu.CompactGBuffer(W, H)
+    out = gpu.DeviceArray(W * H * 16)
     prev = view(0)
     trace_ms, svgf_ms, wall = [], [], []
-    for f in range(args.svgf_frames):
-        cur = view(f)
+    for k in range(args.svgf_frames):
+        cur = view(k)
         t0 = time.perf_counter()
-        r.render_denoised(cur, prev, rs, sv)
+        r.sample_compact(cur, rs, gb)
+        trace_ms.append(r.last_frame_ms())
+        f.run_compact(gb, cur, prev, sv, out.ptr)
+        svgf_ms.append(f.last_ms())
         wall.append((time.perf_counter() - t0) * 1e3)
-        st = r.stats()
-        trace_ms.append(st.ms_shade)
-        svgf_ms.append(st.ms_other)
+        prev = cur
+    # The reference's format: render_denoised (float4 G-buffer) over the same orbit.
+    ref_ms = []
+    prev = view(0)
+    for k in range(min(16, args.svgf_frames)):
+        cur = view(k)
+        r.render_denoised(cur, prev, rs, sv)
+        ref_ms.append(r.stats().ms_other)
         prev = cur
     field.set_camera(cam["position"], target, cam["up"], cam["vfov"])
     r.close()
     k = slice(1, None)  # the first frame also allocates
-    res = dict(frames=args.svgf_frames, width=W, height=H, svgf_ms=round(sum(svgf_ms[k]) / len(svgf_ms[k]), 4),
-               gbuffer_trace_ms=round(sum(trace_ms[k]) / len(trace_ms[k]), 3), frame_ms=round(sum(wall[k]) / len(wall[k]), 3),
-               svgf_bytes_per_pixel=SVGF_BYTES_PER_PIXEL)
-    res["svgf_achieved_gbs"] = round(W * H * SVGF_BYTES_PER_PIXEL / (res["svgf_ms"] * 1e-3) / 1e9, 1)
+    mean = lambda v: sum(v[k]) / len(v[k])
+    res = dict(frames=args.svgf_frames, width=W, height=H, gbuffer="compact (fp32 RGB, fp32 depth, fp16 octahedral normal, int32 material)",
+               svgf_ms=round(mean(svgf_ms), 4), gbuffer_trace_ms=round(mean(trace_ms), 3), frame_ms=round(mean(wall), 3),
+               svgf_bytes_per_pixel=SVGF_COMPACT_BYTES_PER_PIXEL, float4_svgf_ms=round(mean(ref_ms), 4), float4_bytes_per_pixel=SVGF_BYTES_PER_PIXEL)
+    res["svgf_achieved_gbs"] = round(W * H * SVGF_COMPACT_BYTES_PER_PIXEL / (res["svgf_ms"] * 1e-3) / 1e9, 1)
+    res["float4_achieved_gbs"] = round(W * H * SVGF_BYTES_PER_PIXEL / (res["float4_svgf_ms"] * 1e-3) / 1e9, 1)
     log(f"svgf: {res}")
     return res
 

@@ -90,6 +90,26 @@ typedef struct ph_svgf_input {
     const float* emission_dev;
 } ph_svgf_input;
 
+/* The compact G-buffer of the real-time workload, 48 bytes per pixel against the 80 of ph_svgf_input.
+ * No counterpart in the reference, whose denoiser reads only the five float4 planes; a frame in this
+ * format is denoised exactly as its decoded planes would be (ph_svgf_run_compact). Device pointers:
+ *   color, albedo, emission: 3 floats per pixel (the sample's radiance, the primary albedo, the
+ *                            primary surface's emission);
+ *   depth:    float per pixel, the primary hit distance (0 where the primary ray missed);
+ *   normal:   uint32 per pixel, the primary shading normal in octahedral form, two IEEE half floats
+ *             (x in the low 16 bits), rounded to nearest even;
+ *   material: int32 per pixel, the primary material id (-1 where the primary ray missed).
+ * Positions are not stored: the denoiser rebuilds them at `depth` along each pixel's centre ray from
+ * the camera's origin. */
+typedef struct ph_compact_gbuffer {
+    void* color_dev;
+    void* albedo_dev;
+    void* emission_dev;
+    void* depth_dev;
+    void* normal_dev;
+    void* material_dev;
+} ph_compact_gbuffer;
+
 const char* ph_last_error(void);
 int ph_device_count(void);
 int ph_device_info(int device, ph_gpu_info* out);
@@ -120,6 +140,10 @@ int ph_renderer_height(const ph_renderer* r);
 int ph_renderer_samples(const ph_renderer* r);
 float ph_renderer_last_frame_ms(const ph_renderer* r);
 uint32_t* ph_renderer_display_buffer(const ph_renderer* r);
+/* One sample per pixel into a caller-owned compact G-buffer: the sample render_denoised traces (the
+ * reference's sample_kernel, gpu_renderer.cu:83-95), written in the compact format. Uses and advances
+ * the renderer's frame counter like render_denoised; blocks until done (last_frame_ms is its time). */
+int ph_renderer_sample_compact(ph_renderer* r, const ph_camera_view* camera, const ph_render_settings* settings, const ph_compact_gbuffer* out);
 
 /* Engine selection, tuning and statistics. */
 int ph_renderer_set_engine(ph_renderer* r, int engine);
@@ -151,6 +175,11 @@ void ph_svgf_destroy(ph_svgf* f);
 int ph_svgf_resize(ph_svgf* f, int width, int height);
 int ph_svgf_reset(ph_svgf* f);
 int ph_svgf_run(ph_svgf* f, const ph_svgf_input* in, const ph_camera_view* previous_camera, const ph_svgf_settings* settings, float* output_dev);
+/* The denoiser on a compact G-buffer traced from view `camera` (no counterpart in the reference). The
+ * previous frame is kept in the format of the call that stored it; switching formats starts a new
+ * history. */
+int ph_svgf_run_compact(ph_svgf* f, const ph_compact_gbuffer* in, const ph_camera_view* camera, const ph_camera_view* previous_camera,
+                        const ph_svgf_settings* settings, float* output_dev);
 const float* ph_svgf_variance_buffer(const ph_svgf* f);
 float ph_svgf_last_ms(const ph_svgf* f);
 

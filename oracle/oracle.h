@@ -74,6 +74,17 @@ int or_svgf_run(or_svgf* f, const float* color4, const float* albedo4, const flo
  * 6 filter0, 7 filter1. */
 const float* or_svgf_plane(const or_svgf* f, int which);
 
+/* The compact G-buffer of the real-time workload: fp16 conversion, octahedral normals, conversion from
+ * the float4 planes, and the denoiser on a compact frame traced from `cam`. */
+uint32_t or_f32_to_f16(float f);
+float or_f16_to_f32(uint32_t h);
+uint32_t or_oct_encode(const float* n);
+void or_oct_decode(uint32_t code, float* n);
+int or_gbuffer_compact(int n, const float* color4, const float* albedo4, const float* nd4, const float* pos4, const float* emission4, float* color3,
+                       float* albedo3, float* emission3, float* depth, uint32_t* normal, int32_t* material);
+int or_svgf_run_compact(or_svgf* f, const float* color3, const float* albedo3, const float* emission3, const float* depth, const uint32_t* normal,
+                        const int32_t* material, const ph_camera_view* cam, const ph_camera_view* prev_cam, const ph_svgf_settings* st, float* out4);
+
 #ifdef __cplusplus
 }
 #endif

@@ -64,6 +64,16 @@ def oracle():
         L.or_svgf_run.argtypes = [P, FP, FP, FP, FP, FP, CV, C.POINTER(SvgfSettings), FP]
         L.or_svgf_plane.argtypes = [P, C.c_int]
         L.or_svgf_plane.restype = FP
+        L.or_f32_to_f16.argtypes = [C.c_float]
+        L.or_f32_to_f16.restype = C.c_uint32
+        L.or_f16_to_f32.argtypes = [C.c_uint32]
+        L.or_f16_to_f32.restype = C.c_float
+        L.or_oct_encode.argtypes = [FP]
+        L.or_oct_encode.restype = C.c_uint32
+        L.or_oct_decode.argtypes = [C.c_uint32, FP]
+        L.or_oct_decode.restype = None
+        L.or_gbuffer_compact.argtypes = [C.c_int, FP, FP, FP, FP, FP, FP, FP, FP, FP, P, P]
+        L.or_svgf_run_compact.argtypes = [P, FP, FP, FP, FP, P, P, CV, CV, C.POINTER(SvgfSettings), FP]
         _libs["oracle"] = L
     return L
 
@@ -151,6 +161,39 @@ def trace(desc, rays, anyhit=False, counters=False, threads=0):
     return (hits, cnt) if counters else hits
 
 
+def f32_to_f16(x):
+    return int(oracle().or_f32_to_f16(float(x)))
+
+
+def f16_to_f32(h):
+    return float(oracle().or_f16_to_f32(int(h)))
+
+
+def oct_encode(n):
+    v = np.ascontiguousarray(n, np.float32)
+    return int(oracle().or_oct_encode(fptr(v)))
+
+
+def oct_decode(code):
+    out = np.zeros(3, np.float32)
+    oracle().or_oct_decode(int(code), fptr(out))
+    return out
+
+
+def compact_gbuffer(color, albedo, normal_depth, position, emission):
+    """The compact G-buffer (3-float colour/albedo/emission, depth, octahedral fp16 normal, material)
+    of the reference's five float4 planes."""
+    planes = [np.ascontiguousarray(a, np.float32) for a in (color, albedo, normal_depth, position, emission)]
+    h, w = planes[0].shape[:2]
+    n = w * h
+    out = dict(color=np.zeros((h, w, 3), np.float32), albedo=np.zeros((h, w, 3), np.float32), emission=np.zeros((h, w, 3), np.float32),
+               depth=np.zeros((h, w), np.float32), normal=np.zeros((h, w), np.uint32), material=np.zeros((h, w), np.int32))
+    rc = oracle().or_gbuffer_compact(n, *[fptr(p) for p in planes], fptr(out["color"]), fptr(out["albedo"]), fptr(out["emission"]),
+                                     fptr(out["depth"]), out["normal"].ctypes.data, out["material"].ctypes.data)
+    assert rc == 0
+    return out
+
+
 class SvgfOracle:
     def __init__(self, width, height):
         self.w, self.h = width, height
@@ -169,6 +212,16 @@ class SvgfOracle:
         planes = [np.ascontiguousarray(a, np.float32).reshape(-1) for a in (color, albedo, normal_depth, position, emission)]
         out = np.zeros(self.w * self.h * 4, np.float32)
         rc = oracle().or_svgf_run(self._h, *[fptr(p) for p in planes], C.byref(prev_cam), C.byref(settings), fptr(out))
+        assert rc == 0
+        return out.reshape(self.h, self.w, 4)
+
+    def run_compact(self, g, cam, prev_cam, settings):
+        """The denoiser on a compact G-buffer (a dict as compact_gbuffer returns) traced from `cam`."""
+        c = {k: np.ascontiguousarray(g[k]).reshape(-1) for k in ("color", "albedo", "emission", "depth", "normal", "material")}
+        out = np.zeros(self.w * self.h * 4, np.float32)
+        rc = oracle().or_svgf_run_compact(self._h, fptr(c["color"]), fptr(c["albedo"]), fptr(c["emission"]), fptr(c["depth"]),
+                                          c["normal"].ctypes.data, c["material"].ctypes.data, C.byref(cam), C.byref(prev_cam),
+                                          C.byref(settings), fptr(out))
         assert rc == 0
         return out.reshape(self.h, self.w, 4)
 

@@ -10,6 +10,7 @@
 //   history / remodulate ....... svgf.cu:188-210
 //   pass order ................. svgf.cu:238-255 (history taken after the first a-trous pass)
 #include <algorithm>
+#include <cstdint>
 #include <cstring>
 #include <vector>
 
@@ -232,9 +233,128 @@ void remodulate(const or_svgf& f, const Inputs& in, const V4* filtered, V4* out)
     }
 }
 
+// ---- the compact G-buffer of the real-time workload (restating photon_amd/csrc/hip/dgbuf.h) ----------
+// Integer fp16 conversion with round-to-nearest-even, octahedral normals, centre-ray positions. Not a
+// reference routine: the reference's denoiser only reads float4 planes. Pinned by the tests against
+// numpy's float16 conversion and by the geometry of the octahedral map.
+
+uint32_t f2u(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return u;
+}
+
+float u2f(uint32_t u) {
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+uint32_t f32_to_f16(float f) {
+    const uint32_t x = f2u(f);
+    const uint32_t sign = (x >> 16) & 0x8000u;
+    const uint32_t e = (x >> 23) & 0xffu;
+    uint32_t m = x & 0x7fffffu;
+    if (e == 0xffu) return sign | 0x7c00u | (m ? 0x200u : 0u);
+    const int ex = int(e) - 112;
+    if (ex >= 31) return sign | 0x7c00u;
+    if (ex <= 0) {
+        if (ex < -10) return sign;
+        m |= 0x800000u;
+        const int shift = 14 - ex;
+        uint32_t h = m >> shift;
+        const uint32_t rem = m & ((1u << shift) - 1u), half = 1u << (shift - 1);
+        if (rem > half || (rem == half && (h & 1u))) h++;
+        return sign | h;
+    }
+    uint32_t h = sign | (uint32_t(ex) << 10) | (m >> 13);
+    const uint32_t rem = m & 0x1fffu;
+    if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) h++;
+    return h;
+}
+
+float f16_to_f32(uint32_t h) {
+    const uint32_t sign = (h & 0x8000u) << 16, e = (h >> 10) & 0x1fu;
+    uint32_t m = h & 0x3ffu;
+    if (e == 0) {
+        if (m == 0) return u2f(sign);
+        int k = 0;
+        while (!(m & 0x400u)) {
+            m <<= 1;
+            k++;
+        }
+        return u2f(sign | (uint32_t(113 - k) << 23) | ((m & 0x3ffu) << 13));
+    }
+    if (e == 31) return u2f(sign | 0x7f800000u | (m << 13));
+    return u2f(sign | ((e + 112u) << 23) | (m << 13));
+}
+
+uint32_t oct_encode(V3 n) {
+    const float s = fabsf(n.x) + fabsf(n.y) + fabsf(n.z);
+    if (!(s > 0.0f)) return 0u;
+    float x = n.x / s, y = n.y / s;
+    if (n.z < 0.0f) {
+        const float ox = (1.0f - fabsf(y)) * (x >= 0.0f ? 1.0f : -1.0f);
+        const float oy = (1.0f - fabsf(x)) * (y >= 0.0f ? 1.0f : -1.0f);
+        x = ox;
+        y = oy;
+    }
+    return f32_to_f16(x) | (f32_to_f16(y) << 16);
+}
+
+V3 oct_decode(uint32_t code) {
+    float x = f16_to_f32(code & 0xffffu), y = f16_to_f32(code >> 16);
+    const float z = 1.0f - fabsf(x) - fabsf(y);
+    if (z < 0.0f) {
+        const float ox = (1.0f - fabsf(y)) * (x >= 0.0f ? 1.0f : -1.0f);
+        const float oy = (1.0f - fabsf(x)) * (y >= 0.0f ? 1.0f : -1.0f);
+        x = ox;
+        y = oy;
+    }
+    const float len = sqrtf(x * x + y * y + z * z);
+    return v3(x / len, y / len, z / len);
+}
+
+V3 centre_position(const ph_camera_view& c, int x, int y, int width, int height, float depth) {
+    const float px = float(x) + 0.5f, py = float(y) + 0.5f;
+    const float sx = (2.0f * px / float(width) - 1.0f) * c.tan_half_fov * c.aspect;
+    const float sy = (1.0f - 2.0f * py / float(height)) * c.tan_half_fov;
+    const V3 d = unit(v3p(c.forward) + v3p(c.right) * sx + v3p(c.up) * sy);
+    return v3p(c.position) + d * depth;
+}
+
 }  // namespace
 
 extern "C" {
+
+uint32_t or_f32_to_f16(float f) { return f32_to_f16(f); }
+float or_f16_to_f32(uint32_t h) { return f16_to_f32(h); }
+uint32_t or_oct_encode(const float* n) { return oct_encode(v3(n[0], n[1], n[2])); }
+void or_oct_decode(uint32_t code, float* n) {
+    V3 v = oct_decode(code);
+    n[0] = v.x;
+    n[1] = v.y;
+    n[2] = v.z;
+}
+
+// The compact G-buffer of n pixels from the reference's float4 planes (colour, albedo, normal+depth,
+// position+material, emission).
+int or_gbuffer_compact(int n, const float* color4, const float* albedo4, const float* nd4, const float* pos4, const float* emission4, float* color3,
+                       float* albedo3, float* emission3, float* depth, uint32_t* normal, int32_t* material) {
+    if (n < 0 || !color4 || !albedo4 || !nd4 || !pos4 || !emission4 || !color3 || !albedo3 || !emission3 || !depth || !normal || !material)
+        return -1;
+    for (int i = 0; i < n; i++) {
+        for (int k = 0; k < 3; k++) {
+            color3[3 * i + k] = color4[4 * i + k];
+            albedo3[3 * i + k] = albedo4[4 * i + k];
+            emission3[3 * i + k] = emission4[4 * i + k];
+        }
+        depth[i] = nd4[4 * i + 3];
+        normal[i] = oct_encode(v3(nd4[4 * i], nd4[4 * i + 1], nd4[4 * i + 2]));
+        material[i] = int32_t(pos4[4 * i + 3]);
+    }
+    return 0;
+}
 
 int or_svgf_create(int width, int height, or_svgf** out) {
     if (!out || width <= 0 || height <= 0) return -1;
@@ -268,6 +388,27 @@ int or_svgf_run(or_svgf* f, const float* color4, const float* albedo4, const flo
     remodulate(*f, in, f->plane[6 + src].data(), reinterpret_cast<V4*>(out4));
     f->has_history = true;
     return 0;
+}
+
+// The denoiser on a compact G-buffer traced from `cam`: the planes it decodes to, through the float4
+// pipeline above (colour, albedo and emission with w = 1; decoded normal with depth; the centre-ray
+// position with the material id).
+int or_svgf_run_compact(or_svgf* f, const float* color3, const float* albedo3, const float* emission3, const float* depth, const uint32_t* normal,
+                        const int32_t* material, const ph_camera_view* cam, const ph_camera_view* prev_cam, const ph_svgf_settings* st, float* out4) {
+    if (!f || !color3 || !albedo3 || !emission3 || !depth || !normal || !material || !cam) return -1;
+    const int w = f->w, h = f->h;
+    const size_t n = size_t(w) * size_t(h);
+    std::vector<V4> c(n), a(n), nd(n), p(n), e(n);
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            const size_t i = size_t(y) * w + x;
+            c[i] = v4(color3[3 * i], color3[3 * i + 1], color3[3 * i + 2], 1.0f);
+            a[i] = v4(albedo3[3 * i], albedo3[3 * i + 1], albedo3[3 * i + 2], 1.0f);
+            e[i] = v4(emission3[3 * i], emission3[3 * i + 1], emission3[3 * i + 2], 1.0f);
+            nd[i] = v4(oct_decode(normal[i]), depth[i]);
+            p[i] = v4(centre_position(*cam, x, y, w, h, depth[i]), float(material[i]));
+        }
+    return or_svgf_run(f, &c[0].x, &a[0].x, &nd[0].x, &p[0].x, &e[0].x, prev_cam, st, out4);
 }
 
 const float* or_svgf_plane(const or_svgf* f, int which) {

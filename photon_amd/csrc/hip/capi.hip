@@ -335,6 +335,24 @@ int ph_renderer_render_denoised(ph_renderer* r, const ph_camera_view* cam, const
     });
 }
 
+int ph_renderer_sample_compact(ph_renderer* r, const ph_camera_view* cam, const ph_render_settings* rs, const ph_compact_gbuffer* out) {
+    if (!r || !cam || !rs || !out) return fail("null argument");
+    if (!out->color_dev || !out->albedo_dev || !out->emission_dev || !out->depth_dev || !out->normal_dev || !out->material_dev)
+        return fail("missing G-buffer plane");
+    return guarded([&] {
+        ph_renderer& m = *r;
+        if (!m.uploaded || m.width == 0 || m.height == 0) return 0;
+        PH_CHECK(hipEventRecord(m.ev0, m.stream));
+        launch_sample_compact(m.stream, m.view, *cam, m.width, m.height, m.frame, rs->seed, opts_of(*rs), *out);
+        PH_CHECK(hipEventRecord(m.ev1, m.stream));
+        PH_CHECK(hipEventSynchronize(m.ev1));
+        PH_CHECK(hipEventElapsedTime(&m.last_ms, m.ev0, m.ev1));
+        m.frame++;
+        m.last_camera = *cam;
+        return 0;
+    });
+}
+
 int ph_renderer_display(ph_renderer* r, const ph_render_settings* rs, int mode, uint32_t* target) {
     if (!r || !rs) return fail("null argument");
     if (mode < PH_DISPLAY_COLOR || mode > PH_DISPLAY_HEATMAP) return fail("unknown display mode");
@@ -540,6 +558,23 @@ int ph_svgf_run(ph_svgf* f, const ph_svgf_input* in, const ph_camera_view* prev,
     return guarded([&] {
         PH_CHECK(hipEventRecord(f->ev0, f->stream));
         svgf_run(f->state, f->stream, *in, *prev, *s, reinterpret_cast<float4*>(out));
+        PH_CHECK(hipEventRecord(f->ev1, f->stream));
+        PH_CHECK(hipEventSynchronize(f->ev1));
+        PH_CHECK(hipEventElapsedTime(&f->last_ms, f->ev0, f->ev1));
+        return 0;
+    });
+}
+
+int ph_svgf_run_compact(ph_svgf* f, const ph_compact_gbuffer* in, const ph_camera_view* cam, const ph_camera_view* prev, const ph_svgf_settings* s,
+                        float* out) {
+    if (!f || !in || !cam || !prev || !s || !out) return fail("null argument");
+    if (!in->color_dev || !in->albedo_dev || !in->emission_dev || !in->depth_dev || !in->normal_dev || !in->material_dev)
+        return fail("missing G-buffer plane");
+    if (f->state.w <= 0) return fail("the denoiser is not sized");
+    if (s->atrous_iterations < 0 || s->atrous_iterations > 16) return fail("a-trous iterations out of range");
+    return guarded([&] {
+        PH_CHECK(hipEventRecord(f->ev0, f->stream));
+        svgf_run_compact(f->state, f->stream, *in, *cam, *prev, *s, reinterpret_cast<float4*>(out));
         PH_CHECK(hipEventRecord(f->ev1, f->stream));
         PH_CHECK(hipEventSynchronize(f->ev1));
         PH_CHECK(hipEventElapsedTime(&f->last_ms, f->ev0, f->ev1));

@@ -90,6 +90,9 @@ void launch_render(hipStream_t st, const phd::DScene& s, const ph_camera_view& c
                    const phd::PathOpts& opt, float4* accum, float4* albedo, float4* nd);
 void launch_sample(hipStream_t st, const phd::DScene& s, const ph_camera_view& cam, int w, int h, uint32_t frame, uint32_t seed, const phd::PathOpts& opt,
                    float4* color, float4* albedo, float4* nd, float4* position, float4* emission);
+// The same sample written as the compact G-buffer (photon_hip.h, ph_compact_gbuffer).
+void launch_sample_compact(hipStream_t st, const phd::DScene& s, const ph_camera_view& cam, int w, int h, uint32_t frame, uint32_t seed,
+                           const phd::PathOpts& opt, const ph_compact_gbuffer& out);
 void launch_heatmap(hipStream_t st, const phd::DScene& s, const ph_camera_view& cam, int w, int h, float* out);
 void launch_display(hipStream_t st, const DisplayParams& p, const float4* accum, const float4* denoised, const float4* albedo, const float4* nd,
                     const float4* variance, const float* heat, uint32_t* out);

@@ -1,6 +1,7 @@
 // Per-pixel kernels: the plain one-thread-per-pixel path tracer (render_kernel's counterpart,
 // src/render/gpu_renderer.cu:60-80), the real-time G-buffer sample (sample_kernel, :83-95), the BVH
 // heat map (:98-106), the display pass (:129-175), and the ray-buffer tracer of the traversal workload.
+#include "dgbuf.h"
 #include "internal.h"
 
 using namespace phd;
@@ -53,6 +54,32 @@ __global__ __launch_bounds__(kPixelBlock) void sample_kernel(DScene s, ph_camera
     nd[i] = make_float4(p.normal.x, p.normal.y, p.normal.z, p.depth);
     position[i] = make_float4(p.position.x, p.position.y, p.position.z, float(p.material));
     emission[i] = make_float4(p.emission.x, p.emission.y, p.emission.z, 1.0f);
+}
+
+// sample_kernel's sample, written as the compact G-buffer.
+__global__ __launch_bounds__(kPixelBlock) void sample_compact_kernel(DScene s, ph_camera_view cam, int w, int h, uint32_t frame, uint32_t seed,
+                                                                     PathOpts opt, float* color, float* albedo, float* emission, float* depth,
+                                                                     uint32_t* normal, int* material) {
+    __shared__ int lds[kMegaDepth * kPixelBlock];
+    const int tid = threadIdx.y * kBX + threadIdx.x;
+    LaneStack<kMegaDepth> stack{lds + tid, kPixelBlock};
+    const int x = blockIdx.x * kBX + threadIdx.x, y = blockIdx.y * kBY + threadIdx.y;
+    if (x >= w || y >= h) return;
+    const int i = y * w + x;
+    PrimaryOut p{mk3(0), 0.0f, mk3(0), -1, mk3(1), mk3(0)};
+    f3 L = pixel_sample<LaneStack<kMegaDepth>, false>(s, cam, x, y, w, h, frame, seed, opt, &p, stack, nullptr);
+    color[3 * i] = L.x;
+    color[3 * i + 1] = L.y;
+    color[3 * i + 2] = L.z;
+    albedo[3 * i] = p.albedo.x;
+    albedo[3 * i + 1] = p.albedo.y;
+    albedo[3 * i + 2] = p.albedo.z;
+    emission[3 * i] = p.emission.x;
+    emission[3 * i + 1] = p.emission.y;
+    emission[3 * i + 2] = p.emission.z;
+    depth[i] = p.depth;
+    normal[i] = oct_encode(p.normal);
+    material[i] = p.material;
 }
 
 __global__ __launch_bounds__(kPixelBlock) void heatmap_kernel(DScene s, ph_camera_view cam, int w, int h, float* out) {
@@ -215,6 +242,15 @@ void launch_sample(hipStream_t st, const DScene& s, const ph_camera_view& cam, i
                    float4* color, float4* albedo, float4* nd, float4* position, float4* emission) {
     dim3 block(kBX, kBY), grid((w + kBX - 1) / kBX, (h + kBY - 1) / kBY);
     hipLaunchKernelGGL(sample_kernel, grid, block, 0, st, s, cam, w, h, frame, seed, opt, color, albedo, nd, position, emission);
+    PH_CHECK(hipGetLastError());
+}
+
+void launch_sample_compact(hipStream_t st, const DScene& s, const ph_camera_view& cam, int w, int h, uint32_t frame, uint32_t seed, const PathOpts& opt,
+                           const ph_compact_gbuffer& out) {
+    dim3 block(kBX, kBY), grid((w + kBX - 1) / kBX, (h + kBY - 1) / kBY);
+    hipLaunchKernelGGL(sample_compact_kernel, grid, block, 0, st, s, cam, w, h, frame, seed, opt, static_cast<float*>(out.color_dev),
+                       static_cast<float*>(out.albedo_dev), static_cast<float*>(out.emission_dev), static_cast<float*>(out.depth_dev),
+                       static_cast<uint32_t*>(out.normal_dev), static_cast<int*>(out.material_dev));
     PH_CHECK(hipGetLastError());
 }
 

@@ -2,11 +2,20 @@
 // (src/denoise/svgf.cu:19-255): temporal accumulation -> variance -> a-trous pass 0 -> history ->
 // a-trous passes 1..4 -> remodulation.
 //
-// MI355X layout: every plane is float4 per pixel (16-byte loads). The variance estimate and the
-// a-trous passes stage their footprint through LDS: a 16x16 output tile reads a (16 + 2r)^2 halo of the
-// G-buffer planes it needs once per block instead of once per tap. The larger a-trous steps (4, 8, 16)
-// gather directly from L2, where the footprint exceeds what a tile can usefully share.
+// MI355X layout: the reference's planes are float4 per pixel (16-byte loads). The variance estimate and
+// the a-trous passes stage their footprint through LDS: a 16x16 output tile reads a (16 + 2r)^2 halo of
+// the G-buffer planes it needs once per block instead of once per tap. The larger a-trous steps (4, 8,
+// 16) gather directly through the caches: a wave's loads for one tap cover 4 rows of 16 neighbouring
+// pixels, so they stay coalesced, and staging by residue class mod the step (so that a block's taps
+// share one lattice halo) measured slower, its staging loads being strided.
+//
+// Two input formats share the kernels through an accessor policy: the reference's five float4 planes
+// (RefIn), and the real-time workload's compact G-buffer (CompactIn: 12-byte colour, albedo and emission,
+// fp32 depth, fp16 octahedral normal, int32 material; positions rebuilt from depth along each pixel's
+// centre ray). A compact frame is denoised exactly as its decoded float4 planes would be; only the
+// loads, and the format of the stored previous-frame geometry, differ.
 #include "internal.h"
+#include "dgbuf.h"
 #include "svgf_state.h"
 
 using namespace phd;
@@ -50,26 +59,83 @@ PH_DEV bool reproject(const ph_camera_view& c, f3 p, int w, int h, float& px, fl
     return true;
 }
 
-struct Planes {
+// The reference's five float4 planes (SvgfInput, src/denoise/svgf.h:24-30) and its previous-frame
+// normal+depth and position+material planes.
+struct RefIn {
     const float4 *color, *albedo, *nd, *pos, *emission;
+    PH_DEV f3 color3(int i) const { return xyz(color[i]); }
+    PH_DEV f3 albedo3(int i) const { return xyz(albedo[i]); }
+    PH_DEV f3 emission3(int i) const { return xyz(emission[i]); }
+    PH_DEV float4 ndz(int i) const { return nd[i]; }
+    PH_DEV float mat(int i) const { return pos[i].w; }
+    PH_DEV f3 position(int i, int, int) const { return xyz(pos[i]); }
+    PH_DEV float4 raw_color(int i) const { return color[i]; }
 };
 
-__global__ __launch_bounds__(TX* TY) void temporal_kernel(Planes in, ph_camera_view prev_cam, int w, int h, int has_history, float alpha_c,
-                                                           float alpha_m, const float4* prev_illum, const float4* prev_moments,
-                                                           const float4* prev_nd, const float4* prev_pos, float4* out_illum,
-                                                           float4* out_moments) {
+struct RefHist {
+    float4 *nd, *pos;
+    PH_DEV float4 ndz(int j) const { return nd[j]; }
+    PH_DEV float mat(int j) const { return pos[j].w; }
+    PH_DEV void store(const RefIn& in, int i) const {
+        nd[i] = in.nd[i];
+        pos[i] = in.pos[i];
+    }
+};
+
+// The compact G-buffer (ph_compact_gbuffer) and its previous-frame depth, normal code and material.
+struct CompactIn {
+    const float *color, *albedo, *emission, *depth;
+    const uint32_t* normal;
+    const int* material;
+    ph_camera_view cam;  // the view the G-buffer was traced from (positions are rebuilt from depth)
+    int w, h;
+    PH_DEV f3 color3(int i) const { return ld3(color + 3 * i); }
+    PH_DEV f3 albedo3(int i) const { return ld3(albedo + 3 * i); }
+    PH_DEV f3 emission3(int i) const { return ld3(emission + 3 * i); }
+    PH_DEV float4 ndz(int i) const {
+        const f3 n = oct_decode(normal[i]);
+        return make_float4(n.x, n.y, n.z, depth[i]);
+    }
+    PH_DEV float mat(int i) const { return float(material[i]); }
+    PH_DEV f3 position(int i, int x, int y) const { return centre_position(cam, x, y, w, h, depth[i]); }
+    PH_DEV float4 raw_color(int i) const {
+        const f3 c = color3(i);
+        return make_float4(c.x, c.y, c.z, 1.0f);
+    }
+};
+
+struct CompactHist {
+    float* depth;
+    uint32_t* normal;
+    int* material;
+    PH_DEV float4 ndz(int j) const {
+        const f3 n = oct_decode(normal[j]);
+        return make_float4(n.x, n.y, n.z, depth[j]);
+    }
+    PH_DEV float mat(int j) const { return float(material[j]); }
+    PH_DEV void store(const CompactIn& in, int i) const {
+        depth[i] = in.depth[i];
+        normal[i] = in.normal[i];
+        material[i] = in.material[i];
+    }
+};
+
+template <class In, class Hist>
+__global__ __launch_bounds__(TX* TY) void temporal_kernel(In in, ph_camera_view prev_cam, int w, int h, int has_history, float alpha_c,
+                                                           float alpha_m, const float4* prev_illum, const float4* prev_moments, Hist prev,
+                                                           float4* out_illum, float4* out_moments) {
     const int x = blockIdx.x * TX + threadIdx.x, y = blockIdx.y * TY + threadIdx.y;
     if (x >= w || y >= h) return;
     const int i = y * w + x;
-    const float4 pos = in.pos[i];
-    const f3 cur = demod(xyz(in.color[i]), xyz(in.emission[i]), xyz(in.albedo[i]));
+    const float mat = in.mat(i);
+    const f3 cur = demod(in.color3(i), in.emission3(i), in.albedo3(i));
     const float l = lum(cur);
     f3 hist_c = mk3(0);
     float hm0 = 0, hm1 = 0, hist_len = 0, wsum = 0;
-    if (has_history && pos.w >= 0) {
+    if (has_history && mat >= 0) {
         float px, py, dist;
-        if (reproject(prev_cam, xyz(pos), w, h, px, py, dist)) {
-            const f3 n = xyz(in.nd[i]);
+        if (reproject(prev_cam, in.position(i, x, y), w, h, px, py, dist)) {
+            const f3 n = xyz(in.ndz(i));
             const int x0 = int(floorf(px)), y0 = int(floorf(py));
             const float fx = px - float(x0), fy = py - float(y0);
 #pragma unroll
@@ -77,9 +143,8 @@ __global__ __launch_bounds__(TX* TY) void temporal_kernel(Planes in, ph_camera_v
                 const int xx = x0 + (k & 1), yy = y0 + (k >> 1);
                 if (xx < 0 || yy < 0 || xx >= w || yy >= h) continue;
                 const int j = yy * w + xx;
-                const float4 pp = prev_pos[j];
-                const float4 pn = prev_nd[j];
-                if (pp.w != pos.w) continue;
+                if (prev.mat(j) != mat) continue;
+                const float4 pn = prev.ndz(j);
                 if (fabsf(pn.w - dist) > 0.05f * dist + 0.01f) continue;
                 if (dot(xyz(pn), n) < 0.9f) continue;
                 const float bw = ((k & 1) ? fx : 1 - fx) * ((k >> 1) ? fy : 1 - fy);
@@ -119,7 +184,8 @@ __global__ __launch_bounds__(TX* TY) void temporal_kernel(Planes in, ph_camera_v
 // 7x7 bilateral variance estimate while the history is short (svgf.cu:96-132). The block stages the
 // 22x22 halo of normal+depth and position.w (material) in LDS; moments and illumination are read only
 // for accepted taps of short-history pixels.
-__global__ __launch_bounds__(TX* TY) void variance_kernel(Planes in, int w, int h, const float4* illum, const float4* moments, float phi_normal,
+template <class In>
+__global__ __launch_bounds__(TX* TY) void variance_kernel(In in, int w, int h, const float4* illum, const float4* moments, float phi_normal,
                                                            float4* out) {
     constexpr int R = 3, SX = TX + 2 * R, SY = TY + 2 * R;
     __shared__ float4 s_nd[SY][SX];
@@ -130,8 +196,8 @@ __global__ __launch_bounds__(TX* TY) void variance_kernel(Planes in, int w, int 
         const int gx = bx + sx, gy = by + sy;
         if (gx >= 0 && gy >= 0 && gx < w && gy < h) {
             const int j = gy * w + gx;
-            s_nd[sy][sx] = in.nd[j];
-            s_mat[sy][sx] = in.pos[j].w;
+            s_nd[sy][sx] = in.ndz(j);
+            s_mat[sy][sx] = in.mat(j);
         }
     }
     __syncthreads();
@@ -178,8 +244,8 @@ __global__ __launch_bounds__(TX* TY) void variance_kernel(Planes in, int w, int 
 
 // One a-trous pass (svgf.cu:134-186). For steps 1 and 2 the block stages src, normal+depth and the
 // material id of its (16 + 4*step + 2)^2 footprint in LDS; larger steps read through the caches.
-template <int Step>
-__global__ __launch_bounds__(TX* TY) void atrous_kernel(Planes in, int w, int h, int step_rt, float phi_color, float phi_normal, float phi_depth,
+template <int Step, class In>
+__global__ __launch_bounds__(TX* TY) void atrous_kernel(In in, int w, int h, int step_rt, float phi_color, float phi_normal, float phi_depth,
                                                          const float4* src, float4* dst) {
     constexpr bool Tiled = Step > 0;
     constexpr int R = Tiled ? 2 * Step + 1 : 1;  // taps reach 2*step; the 3x3 prefilter and gradient need 1
@@ -197,8 +263,8 @@ __global__ __launch_bounds__(TX* TY) void atrous_kernel(Planes in, int w, int h,
             const int gx = min(max(blockIdx.x * TX - R + sx, 0), w - 1), gy = min(max(blockIdx.y * TY - R + sy, 0), h - 1);
             const int j = gy * w + gx;
             s_src[sy][sx] = src[j];
-            s_nd[sy][sx] = in.nd[j];
-            s_mat[sy][sx] = in.pos[j].w;
+            s_nd[sy][sx] = in.ndz(j);
+            s_mat[sy][sx] = in.mat(j);
             (void)bx;
             (void)by;
         }
@@ -210,8 +276,8 @@ __global__ __launch_bounds__(TX* TY) void atrous_kernel(Planes in, int w, int h,
     const int lx = threadIdx.x + R, ly = threadIdx.y + R;
     // Loads through LDS (tiled) or global memory; (gx, gy) are image coordinates already in range.
     auto SRC = [&](int gx, int gy) { return Tiled ? s_src[ly + gy - y][lx + gx - x] : src[gy * w + gx]; };
-    auto ND = [&](int gx, int gy) { return Tiled ? s_nd[ly + gy - y][lx + gx - x] : in.nd[gy * w + gx]; };
-    auto MAT = [&](int gx, int gy) { return Tiled ? s_mat[ly + gy - y][lx + gx - x] : in.pos[gy * w + gx].w; };
+    auto ND = [&](int gx, int gy) { return Tiled ? s_nd[ly + gy - y][lx + gx - x] : in.ndz(gy * w + gx); };
+    auto MAT = [&](int gx, int gy) { return Tiled ? s_mat[ly + gy - y][lx + gx - x] : in.mat(gy * w + gx); };
     const float4 center = SRC(x, y);
     const float mat = MAT(x, y);
     if (mat < 0) {
@@ -267,27 +333,65 @@ __global__ __launch_bounds__(TX* TY) void atrous_kernel(Planes in, int w, int h,
     }
 }
 
-__global__ void history_kernel(Planes in, int n, const float4* filtered, const float4* illum, const float4* moments, float4* prev_illum,
-                               float4* prev_moments, float4* prev_nd, float4* prev_pos) {
+template <class In, class Hist>
+__global__ void history_kernel(In in, int n, const float4* filtered, const float4* illum, const float4* moments, float4* prev_illum,
+                               float4* prev_moments, Hist prev) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4 f = filtered[i];
     prev_illum[i] = make_float4(f.x, f.y, f.z, illum[i].w);
     prev_moments[i] = moments[i];
-    prev_nd[i] = in.nd[i];
-    prev_pos[i] = in.pos[i];
+    prev.store(in, i);
 }
 
-__global__ void remodulate_kernel(Planes in, int n, const float4* filtered, float4* out) {
+template <class In>
+__global__ void remodulate_kernel(In in, int n, const float4* filtered, float4* out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    if (in.pos[i].w < 0) {
-        out[i] = in.color[i];
+    if (in.mat(i) < 0) {
+        out[i] = in.raw_color(i);
         return;
     }
-    const f3 a = xyz(in.albedo[i]);
-    const f3 c = xyz(filtered[i]) * vmax3(a, mk3(1e-3f)) + xyz(in.emission[i]);
+    const f3 a = in.albedo3(i);
+    const f3 c = xyz(filtered[i]) * vmax3(a, mk3(1e-3f)) + in.emission3(i);
     out[i] = make_float4(c.x, c.y, c.z, 1.0f);
+}
+
+// The nine launches of one frame (svgf.cu:238-255) for either input format.
+template <class In, class Hist>
+void run_passes(SvgfState& f, hipStream_t st, const In& in, const Hist& prev, const ph_camera_view& prev_cam, const ph_svgf_settings& s,
+                float4* output) {
+    const int w = f.w, h = f.h, n = w * h;
+    float4 *prev_illum = f.plane[0].ptr, *prev_moments = f.plane[1].ptr;
+    float4 *illum = f.plane[4].ptr, *moments = f.plane[5].ptr, *filter[2] = {f.plane[6].ptr, f.plane[7].ptr};
+    const dim3 tile(TX, TY), grid((w + TX - 1) / TX, (h + TY - 1) / TY);
+    const int lin = 256, lgrid = (n + lin - 1) / lin;
+    hipLaunchKernelGGL((temporal_kernel<In, Hist>), grid, tile, 0, st, in, prev_cam, w, h, int(f.has_history), s.alpha_color, s.alpha_moments,
+                       prev_illum, prev_moments, prev, illum, moments);
+    hipLaunchKernelGGL((variance_kernel<In>), grid, tile, 0, st, in, w, h, illum, moments, s.phi_normal, filter[0]);
+    int src = 0;
+    for (int it = 0; it < s.atrous_iterations; it++) {
+        const int step = 1 << it;
+        if (step == 1)
+            hipLaunchKernelGGL((atrous_kernel<1, In>), grid, tile, 0, st, in, w, h, step, s.phi_color, s.phi_normal, s.phi_depth, filter[src],
+                               filter[1 - src]);
+        else if (step == 2)
+            hipLaunchKernelGGL((atrous_kernel<2, In>), grid, tile, 0, st, in, w, h, step, s.phi_color, s.phi_normal, s.phi_depth, filter[src],
+                               filter[1 - src]);
+        else
+            hipLaunchKernelGGL((atrous_kernel<0, In>), grid, tile, 0, st, in, w, h, step, s.phi_color, s.phi_normal, s.phi_depth, filter[src],
+                               filter[1 - src]);
+        src = 1 - src;
+        if (it == 0)
+            hipLaunchKernelGGL((history_kernel<In, Hist>), dim3(lgrid), dim3(lin), 0, st, in, n, filter[src], illum, moments, prev_illum, prev_moments,
+                               prev);
+    }
+    if (s.atrous_iterations == 0)
+        hipLaunchKernelGGL((history_kernel<In, Hist>), dim3(lgrid), dim3(lin), 0, st, in, n, filter[src], illum, moments, prev_illum, prev_moments,
+                           prev);
+    hipLaunchKernelGGL((remodulate_kernel<In>), dim3(lgrid), dim3(lin), 0, st, in, n, filter[src], output);
+    PH_CHECK(hipGetLastError());
+    f.has_history = true;
 }
 
 }  // namespace
@@ -305,36 +409,31 @@ void svgf_resize(SvgfState& f, int width, int height) {
 }
 
 void svgf_run(SvgfState& f, hipStream_t st, const ph_svgf_input& input, const ph_camera_view& prev_cam, const ph_svgf_settings& s, float4* output) {
-    const int w = f.w, h = f.h, n = w * h;
+    if (f.w * f.h == 0) return;
+    if (f.history_compact) f.has_history = false;  // the stored previous frame is in the other format
+    f.history_compact = false;
+    const RefIn in{reinterpret_cast<const float4*>(input.color_dev), reinterpret_cast<const float4*>(input.albedo_dev),
+                   reinterpret_cast<const float4*>(input.normal_depth_dev), reinterpret_cast<const float4*>(input.position_dev),
+                   reinterpret_cast<const float4*>(input.emission_dev)};
+    run_passes(f, st, in, RefHist{f.plane[2].ptr, f.plane[3].ptr}, prev_cam, s, output);
+}
+
+void svgf_run_compact(SvgfState& f, hipStream_t st, const ph_compact_gbuffer& g, const ph_camera_view& cam, const ph_camera_view& prev_cam,
+                      const ph_svgf_settings& s, float4* output) {
+    const int n = f.w * f.h;
     if (n == 0) return;
-    Planes in{reinterpret_cast<const float4*>(input.color_dev), reinterpret_cast<const float4*>(input.albedo_dev),
-              reinterpret_cast<const float4*>(input.normal_depth_dev), reinterpret_cast<const float4*>(input.position_dev),
-              reinterpret_cast<const float4*>(input.emission_dev)};
-    float4 *prev_illum = f.plane[0].ptr, *prev_moments = f.plane[1].ptr, *prev_nd = f.plane[2].ptr, *prev_pos = f.plane[3].ptr;
-    float4 *illum = f.plane[4].ptr, *moments = f.plane[5].ptr, *filter[2] = {f.plane[6].ptr, f.plane[7].ptr};
-    const dim3 tile(TX, TY), grid((w + TX - 1) / TX, (h + TY - 1) / TY);
-    const int lin = 256, lgrid = (n + lin - 1) / lin;
-    hipLaunchKernelGGL(temporal_kernel, grid, tile, 0, st, in, prev_cam, w, h, int(f.has_history), s.alpha_color, s.alpha_moments, prev_illum,
-                       prev_moments, prev_nd, prev_pos, illum, moments);
-    hipLaunchKernelGGL(variance_kernel, grid, tile, 0, st, in, w, h, illum, moments, s.phi_normal, filter[0]);
-    int src = 0;
-    for (int it = 0; it < s.atrous_iterations; it++) {
-        const int step = 1 << it;
-        if (step == 1)
-            hipLaunchKernelGGL(atrous_kernel<1>, grid, tile, 0, st, in, w, h, step, s.phi_color, s.phi_normal, s.phi_depth, filter[src], filter[1 - src]);
-        else if (step == 2)
-            hipLaunchKernelGGL(atrous_kernel<2>, grid, tile, 0, st, in, w, h, step, s.phi_color, s.phi_normal, s.phi_depth, filter[src], filter[1 - src]);
-        else
-            hipLaunchKernelGGL(atrous_kernel<0>, grid, tile, 0, st, in, w, h, step, s.phi_color, s.phi_normal, s.phi_depth, filter[src], filter[1 - src]);
-        src = 1 - src;
-        if (it == 0)
-            hipLaunchKernelGGL(history_kernel, dim3(lgrid), dim3(lin), 0, st, in, n, filter[src], illum, moments, prev_illum, prev_moments, prev_nd, prev_pos);
+    if (!f.prev_depth.ptr || f.prev_depth.count < size_t(n)) {
+        f.prev_depth.allocate(size_t(n));
+        f.prev_normal.allocate(size_t(n));
+        f.prev_material.allocate(size_t(n));
+        f.has_history = false;
     }
-    if (s.atrous_iterations == 0)
-        hipLaunchKernelGGL(history_kernel, dim3(lgrid), dim3(lin), 0, st, in, n, filter[src], illum, moments, prev_illum, prev_moments, prev_nd, prev_pos);
-    hipLaunchKernelGGL(remodulate_kernel, dim3(lgrid), dim3(lin), 0, st, in, n, filter[src], output);
-    PH_CHECK(hipGetLastError());
-    f.has_history = true;
+    if (!f.history_compact) f.has_history = false;
+    f.history_compact = true;
+    const CompactIn in{static_cast<const float*>(g.color_dev), static_cast<const float*>(g.albedo_dev), static_cast<const float*>(g.emission_dev),
+                       static_cast<const float*>(g.depth_dev), static_cast<const uint32_t*>(g.normal_dev), static_cast<const int*>(g.material_dev),
+                       cam, f.w, f.h};
+    run_passes(f, st, in, CompactHist{f.prev_depth.ptr, f.prev_normal.ptr, f.prev_material.ptr}, prev_cam, s, output);
 }
 
 }  // namespace phk

@@ -68,6 +68,11 @@ class SvgfInput(C.Structure):
                 ("position_dev", C.c_void_p), ("emission_dev", C.c_void_p)]
 
 
+class CompactGBufferDesc(C.Structure):
+    _fields_ = [("color_dev", C.c_void_p), ("albedo_dev", C.c_void_p), ("emission_dev", C.c_void_p),
+                ("depth_dev", C.c_void_p), ("normal_dev", C.c_void_p), ("material_dev", C.c_void_p)]
+
+
 _sig_done = False
 
 
@@ -115,6 +120,8 @@ def lib():
         L.ph_svgf_resize.argtypes = [P, C.c_int, C.c_int]
         L.ph_svgf_reset.argtypes = [P]
         L.ph_svgf_run.argtypes = [P, C.POINTER(SvgfInput), CV, C.POINTER(SvgfSettings), P]
+        L.ph_svgf_run_compact.argtypes = [P, C.POINTER(CompactGBufferDesc), CV, CV, C.POINTER(SvgfSettings), P]
+        L.ph_renderer_sample_compact.argtypes = [R, CV, RS, C.POINTER(CompactGBufferDesc)]
         L.ph_svgf_variance_buffer.argtypes = [P]
         L.ph_svgf_variance_buffer.restype = C.c_void_p
         L.ph_svgf_last_ms.argtypes = [P]
@@ -255,6 +262,10 @@ class GpuRenderer:
         svgf = svgf or SvgfSettings.make()
         _check(lib().ph_renderer_render_denoised(self._h, C.byref(camera), C.byref(previous), C.byref(settings), C.byref(svgf)))
 
+    def sample_compact(self, camera, settings, gbuffer):
+        """One sample per pixel into a CompactGBuffer (ph_renderer_sample_compact)."""
+        _check(lib().ph_renderer_sample_compact(self._h, C.byref(camera), C.byref(settings), C.byref(gbuffer.desc())))
+
     def display(self, settings, mode=DISPLAY_COLOR, target=None):
         _check(lib().ph_renderer_display(self._h, C.byref(settings), mode, target))
 
@@ -332,6 +343,36 @@ class GpuRenderer:
         return ms.value
 
 
+class CompactGBuffer:
+    """Device planes of the compact G-buffer (ph_compact_gbuffer): 3-float colour, albedo and emission,
+    float depth, uint32 octahedral fp16 normal, int32 material."""
+
+    PLANES = (("color", np.float32, 3), ("albedo", np.float32, 3), ("emission", np.float32, 3),
+              ("depth", np.float32, 1), ("normal", np.uint32, 1), ("material", np.int32, 1))
+
+    def __init__(self, width, height):
+        self.width, self.height = width, height
+        self.planes = {name: DeviceArray(width * height * k * np.dtype(dt).itemsize) for name, dt, k in self.PLANES}
+
+    @classmethod
+    def from_numpy(cls, planes, width, height):
+        g = cls(width, height)
+        for name, dt, k in cls.PLANES:
+            g.planes[name].upload(np.asarray(planes[name], dt))
+        return g
+
+    def desc(self):
+        p = self.planes
+        return CompactGBufferDesc(p["color"].ptr, p["albedo"].ptr, p["emission"].ptr, p["depth"].ptr, p["normal"].ptr, p["material"].ptr)
+
+    def numpy(self):
+        out = {}
+        for name, dt, k in self.PLANES:
+            shape = (self.height, self.width, k) if k > 1 else (self.height, self.width)
+            out[name] = self.planes[name].numpy(dt, shape)
+        return out
+
+
 class Svgf:
     def __init__(self):
         self._h = C.c_void_p()
@@ -351,6 +392,11 @@ class Svgf:
     def run(self, color_dev, albedo_dev, normal_depth_dev, position_dev, emission_dev, previous_camera, settings, output_dev):
         inp = SvgfInput(color_dev, albedo_dev, normal_depth_dev, position_dev, emission_dev)
         _check(lib().ph_svgf_run(self._h, C.byref(inp), C.byref(previous_camera), C.byref(settings), output_dev))
+
+    def run_compact(self, gbuffer, camera, previous_camera, settings, output_dev):
+        """The denoiser on a CompactGBuffer traced from `camera` (ph_svgf_run_compact)."""
+        _check(lib().ph_svgf_run_compact(self._h, C.byref(gbuffer.desc()), C.byref(camera), C.byref(previous_camera), C.byref(settings),
+                                         output_dev))
 
     def variance_buffer(self):
         return lib().ph_svgf_variance_buffer(self._h)

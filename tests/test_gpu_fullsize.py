@@ -135,3 +135,37 @@ def test_svgf_workload_gbuffers(field):
         print(f"svgf 1600x900 frame {k}: {100 * ok.mean():.4f}% within 1e-3")
         assert ok.mean() >= 0.999
         prev = cam
+
+
+def test_svgf_workload_compact_gbuffers(field):
+    """The real-time workload in its own G-buffer format: the compact sample traced on the device at
+    1600x900, denoised on the device and by the oracle from the same compact planes, two frames."""
+    W, H = 1600, 900
+    rs = RenderSettings.make(W, H, max_depth=16, rr_depth=3, seed=1)
+    sv = SvgfSettings.make()
+    cam0 = field.camera()
+    r = uploaded(field, W, H)
+    f = gpu.Svgf()
+    f.resize(W, H)
+    out = gpu.DeviceArray(W * H * 16)
+    gb = gpu.CompactGBuffer(W, H)
+    ora = O.SvgfOracle(W, H)
+    prev = None
+    for k in range(2):
+        a = np.deg2rad(0.5 * k)
+        p = np.array(cam0["position"]) - np.array(cam0["target"])
+        p = np.array([np.cos(a) * p[0] + np.sin(a) * p[2], p[1], -np.sin(a) * p[0] + np.cos(a) * p[2]]) + np.array(cam0["target"])
+        field.set_camera(p, cam0["target"], cam0["up"], cam0["vfov"])
+        cam = field.camera_view(W / H)
+        r.sample_compact(cam, rs, gb)
+        f.run_compact(gb, cam, prev or cam, sv, out.ptr)
+        got = out.numpy(np.float32, (H, W, 4))
+        want = ora.run_compact(gb.numpy(), cam, prev or cam, sv)
+        assert np.all(np.isfinite(got))
+        d = np.abs(got - want)
+        ok = np.all((d <= 1e-3 * np.abs(want)) | (d <= 1e-6), axis=-1)
+        print(f"compact svgf 1600x900 frame {k}: {100 * ok.mean():.4f}% within 1e-3")
+        assert ok.mean() >= 0.999
+        prev = cam
+    field.set_camera(cam0["position"], cam0["target"], cam0["up"], cam0["vfov"])
+    r.close()

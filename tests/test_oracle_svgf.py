@@ -135,3 +135,116 @@ def test_denoiser_handles_camera_motion():
     assert np.isfinite(out).all() and out[..., :3].min() >= 0 and lum(out).mean() > 0.02
     # Reprojection accepted history for most of the frame.
     assert (f.plane(4)[..., 3] > 4).mean() > 0.5
+
+
+# ---- the compact G-buffer of the real-time workload ----------------------------------------------
+
+def test_fp16_conversion_matches_numpy_round_to_nearest_even():
+    """The integer fp16 conversion against numpy's float16 (IEEE round to nearest even), over normals,
+    subnormals, exact ties, overflow, infinities and NaN; and the widening of every half bit pattern."""
+    g = np.random.default_rng(3)
+    vals = [0.0, -0.0, 1.0, -1.0, 65504.0, 65520.0, 65519.99, 1e9, -1e9, np.inf, -np.inf, 2.0 ** -24, 2.0 ** -25, 3 * 2.0 ** -26,
+            2.0 ** -14, 2.0 ** -14 * (1 - 2.0 ** -11), 1 + 2.0 ** -11, 1 + 3 * 2.0 ** -11, 1e-8, -1e-8]
+    vals += list(g.standard_normal(3000) * 10.0 ** g.uniform(-9, 5, 3000))
+    # Exact halfway cases between neighbouring halves (normal and subnormal range).
+    h = g.integers(1, 0x7bff, 2000).astype(np.uint16)
+    lo = h.view(np.float16).astype(np.float64)
+    hi = (h + 1).view(np.float16).astype(np.float64)
+    vals += list(((lo + hi) / 2).astype(np.float32))
+    x = np.array(vals, np.float32)
+    with np.errstate(over="ignore"):
+        want = x.astype(np.float16).view(np.uint16)
+    got = np.array([O.f32_to_f16(v) for v in x], np.uint16)
+    assert np.array_equal(got, want)
+    nan = O.f32_to_f16(np.float32(np.nan))
+    assert (nan & 0x7c00) == 0x7c00 and (nan & 0x3ff) != 0
+    bits = np.arange(1 << 16, dtype=np.uint32)
+    ref = bits.astype(np.uint16).view(np.float16).astype(np.float32)
+    got = np.array([O.f16_to_f32(b) for b in bits], np.float32)
+    finite = np.isfinite(ref)
+    assert np.array_equal(got[finite].view(np.uint32), ref[finite].view(np.uint32))
+    assert np.array_equal(np.isnan(got), np.isnan(ref)) and np.array_equal(np.isinf(got), np.isinf(ref))
+
+
+def test_octahedral_normals_round_trip():
+    # Axis directions survive exactly, in both hemispheres.
+    for n in ([0, 0, 1], [0, 0, -1], [1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0]):
+        assert np.array_equal(O.oct_decode(O.oct_encode(np.array(n, np.float32))), np.array(n, np.float32))
+    # Random unit vectors come back within the fp16 grid's angular resolution, and unit length.
+    g = np.random.default_rng(4)
+    v = g.standard_normal((4000, 3))
+    v = (v / np.linalg.norm(v, axis=1, keepdims=True)).astype(np.float32)
+    back = np.array([O.oct_decode(O.oct_encode(n)) for n in v])
+    ang = np.arccos(np.clip(np.sum(back.astype(np.float64) * v, axis=1), -1, 1))
+    assert ang.max() < 1.5e-3
+    assert np.allclose(np.linalg.norm(back, axis=1), 1.0, atol=1e-6)
+    # The lower hemisphere is folded, not mirrored: z keeps its sign.
+    far = np.abs(v[:, 2]) > 1e-3
+    assert np.array_equal(np.sign(back[far, 2]), np.sign(v[far, 2]))
+    # A zero normal (a miss) encodes as the origin.
+    assert O.oct_encode(np.zeros(3, np.float32)) == 0
+
+
+def _centre_positions(cam, w, h, depth):
+    """Pixel-centre ray positions, restated in numpy float32 with the library's operation order."""
+    f32 = np.float32
+    ys, xs = np.meshgrid(np.arange(h), np.arange(w), indexing="ij")
+    px, py = xs.astype(f32) + f32(0.5), ys.astype(f32) + f32(0.5)
+    sx = (f32(2) * px / f32(w) - f32(1)) * f32(cam.tan_half_fov) * f32(cam.aspect)
+    sy = (f32(1) - f32(2) * py / f32(h)) * f32(cam.tan_half_fov)
+    fwd, right, up, pos = (np.array(list(v), f32) for v in (cam.forward, cam.right, cam.up, cam.position))
+    d = fwd + right * sx[..., None]
+    d = d + up * sy[..., None]
+    l2 = d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]
+    l2 = l2 + d[..., 2] * d[..., 2]
+    d = d * (f32(1) / np.sqrt(l2))[..., None]
+    return pos + d * depth[..., None]
+
+
+def test_compact_frames_denoise_like_their_decoded_planes():
+    """A compact frame through or_svgf_run_compact equals the float4 pipeline on the planes it decodes
+    to (decoded normals, centre-ray positions at the stored depth, w = 1), bit for bit, over two frames
+    with a moving camera; the decoded planes here are restated independently in numpy."""
+    w, h = 48, 32
+    s = host.Scene.builtin("cornell").build()
+    cam0 = s.camera()
+    g = np.random.default_rng(6)
+    fa, fb = O.SvgfOracle(w, h), O.SvgfOracle(w, h)
+    prev = None
+    for k in range(2):
+        p = np.array(cam0["position"]) + np.array([0.03 * k, 0, 0])
+        s.set_camera(p, cam0["target"], cam0["up"], cam0["vfov"])
+        cam = s.camera_view(w / h)
+        color = g.uniform(0, 2, (h, w, 3)).astype(np.float32)
+        albedo = g.uniform(0.05, 1, (h, w, 3)).astype(np.float32)
+        emission = np.where(g.uniform(size=(h, w, 1)) < 0.05, 1.0, 0.0).astype(np.float32) * np.ones(3, np.float32)
+        depth = g.uniform(2, 4, (h, w)).astype(np.float32)
+        material = np.where(g.uniform(size=(h, w)) < 0.08, -1, g.integers(0, 3, (h, w))).astype(np.int32)
+        nrm = g.standard_normal((h, w, 3))
+        nrm = (nrm / np.linalg.norm(nrm, axis=2, keepdims=True)).astype(np.float32)
+        codes = np.array([[O.oct_encode(nrm[y, x]) for x in range(w)] for y in range(h)], np.uint32)
+        compact = dict(color=color, albedo=albedo, emission=emission, depth=depth, normal=codes, material=material)
+        one = np.ones((h, w, 1), np.float32)
+        dec = np.array([[O.oct_decode(c) for c in row] for row in codes], np.float32)
+        planes = (np.concatenate([color, one], 2), np.concatenate([albedo, one], 2), np.concatenate([dec, depth[..., None]], 2),
+                  np.concatenate([_centre_positions(cam, w, h, depth), material[..., None].astype(np.float32)], 2),
+                  np.concatenate([emission, one], 2))
+        a = fa.run_compact(compact, cam, prev or cam, SvgfSettings.make())
+        b = fb.run(*planes, prev or cam, SvgfSettings.make())
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), f"frame {k}"
+        prev = cam
+
+
+def test_compact_gbuffer_of_float4_planes():
+    w, h = 40, 30
+    s = host.Scene.builtin("cornell").build()
+    cam = s.camera_view(w / h)
+    g4 = O.render_gbuffer(s.desc(), cam, RenderSettings.make(w, h, seed=0), 0)
+    c = O.compact_gbuffer(*g4)
+    color, albedo, nd, pos, emission = g4
+    assert np.array_equal(c["color"], color[..., :3]) and np.array_equal(c["albedo"], albedo[..., :3])
+    assert np.array_equal(c["emission"], emission[..., :3]) and np.array_equal(c["depth"], nd[..., 3])
+    assert np.array_equal(c["material"], pos[..., 3].astype(np.int32))
+    hit = c["material"] >= 0
+    dec = np.array([O.oct_decode(v) for v in c["normal"][hit]])
+    assert np.max(np.abs(dec - nd[..., :3][hit])) < 2e-3
