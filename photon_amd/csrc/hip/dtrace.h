@@ -148,48 +148,52 @@ PH_DEV Walker walker(f3 o, f3 d, float tmax) {
 // needs a node fetch.
 template <bool AnyHit, bool Count, class Stack>
 PH_DEV bool walk_step(const TraceScene& ts, Walker& w, HitRec& hit, Stack& stack, TraceCount* cnt) {
-    if (w.pos < 0) {
-        // Fetch the node and test both child boxes against the tmax at node entry.
-        float4 lmin, lmax, rmin, rmax;
-        if (w.node < ts.top_n) {
-            const float4* np = ts.top + 4 * w.node;
-            lmin = np[0];
-            lmax = np[1];
-            rmin = np[2];
-            rmax = np[3];
-        } else {
-            const float4* np = ts.nodes + 4 * w.node;
-            lmin = ldg(np);
-            lmax = ldg(np + 1);
-            rmin = ldg(np + 2);
-            rmax = ldg(np + 3);
-        }
+    const bool fetch = w.pos < 0;
+    int tri = 0;
+    if (!fetch) {
+        // The next leaf triangle: the left leaf's in order, then the right leaf's.
+        const int nl = (w.tl != kInf && w.lcount > 0) ? w.lcount : 0;
+        tri = w.pos < nl ? w.lidx + w.pos : w.ridx + (w.pos - nl);
+    }
+    // One memory round trip per step for both kinds of work: every lane issues its loads (a node's four
+    // float4, or a triangle's three) before any lane uses them, so a wave whose lanes do both waits once.
+    float4 q0, q1, q2, q3 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (fetch && w.node < ts.top_n) {
+        const float4* np = ts.top + 4 * w.node;
+        q0 = np[0];
+        q1 = np[1];
+        q2 = np[2];
+        q3 = np[3];
+    } else {
+        const float4* p = fetch ? ts.nodes + 4 * w.node : ts.tris + 3 * tri;
+        q0 = ldg(p);
+        q1 = ldg(p + 1);
+        q2 = ldg(p + 2);
+        if (fetch) q3 = ldg(p + 3);
+    }
+    if (fetch) {
+        // The node: test both child boxes against the tmax at node entry.
         if (Count) cnt->nodes++;
-        w.tl = slab(lmin, lmax, w.o, w.inv, w.tmax);
-        w.tr = slab(rmin, rmax, w.o, w.inv, w.tmax);
-        w.lcount = __float_as_int(lmax.w);
-        w.rcount = __float_as_int(rmax.w);
-        w.lidx = __float_as_int(lmin.w);
-        w.ridx = __float_as_int(rmin.w);
+        w.tl = slab(q0, q1, w.o, w.inv, w.tmax);
+        w.tr = slab(q2, q3, w.o, w.inv, w.tmax);
+        w.lcount = __float_as_int(q1.w);
+        w.rcount = __float_as_int(q3.w);
+        w.lidx = __float_as_int(q0.w);
+        w.ridx = __float_as_int(q2.w);
         w.pos = 0;
     } else {
-        // Test the next leaf triangle: the left leaf's in order, then the right leaf's.
-        const int nl = (w.tl != kInf && w.lcount > 0) ? w.lcount : 0;
-        const int i = w.pos < nl ? w.lidx + w.pos : w.ridx + (w.pos - nl);
         w.pos++;
-        const float4* tp = ts.tris + 3 * i;
-        const float4 a = ldg(tp), b = ldg(tp + 1), c = ldg(tp + 2);
         float th, u, v;
         if (Count) cnt->tris++;
-        if (mt_hit(xyz(a), xyz(b), xyz(c), w.o, w.d, w.tmax, th, u, v)) {
+        if (mt_hit(xyz(q0), xyz(q1), xyz(q2), w.o, w.d, w.tmax, th, u, v)) {
             bool cut = false;
-            if (__float_as_int(b.w) & PH_TRI_ALPHA_TESTED) {
+            if (__float_as_int(q1.w) & PH_TRI_ALPHA_TESTED) {
                 if (Count) cnt->alpha++;
-                cut = alpha_cut(*ts.full, i, __float_as_int(a.w), u, v);
+                cut = alpha_cut(*ts.full, tri, __float_as_int(q0.w), u, v);
             }
             if (!cut) {
                 hit.t = th;
-                hit.tri = i;
+                hit.tri = tri;
                 w.found = true;
                 if (AnyHit) return false;
                 w.tmax = th;
