@@ -70,7 +70,7 @@ def bench_render(args, gpu, host, RenderSettings):
         log(f"launch configuration {cfg.as_dict()}")
     elif not args.no_autotune:
         t = time.time()
-        cfg = r.autotune(cam, rs, 1)
+        cfg = r.autotune(cam, rs, args.spp)
         log(f"autotune {time.time() - t:.1f} s -> {cfg.as_dict()}")
     for i in range(args.warmup):
         r.reset()

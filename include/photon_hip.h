@@ -61,6 +61,11 @@ typedef struct ph_wavefront_config {
     int32_t lds_nodes;     /* top BVH nodes (breadth-first) each traversal block stages in LDS */
     int32_t ray_bins;      /* trace camera rays, then continuing rays by direction octant (1), or all together (0) */
     int32_t xcd_parts;     /* 8: each XCD takes a contiguous eighth of every queue (L2 locality); 1: all waves interleave */
+    int32_t sample_split;  /* 1: a path slot runs all of a pixel's samples in order, and the accumulation is bit-identical
+                              to the per-pixel kernel's (the reference's order). S > 1: each pixel's samples run as S
+                              sub-streams of spp/S samples (S rounded down to a divisor of spp) that any slot may take,
+                              into partial sums that are added to the accumulation in sub-stream order; deterministic,
+                              within rounding of the in-order sum, and free of the tail of pixels with long paths */
 } ph_wavefront_config;
 
 /* Statistics of the last render call. After render_denoised, ms_shade is the one-sample G-buffer trace

@@ -56,12 +56,14 @@ enum : int {
     C_CLASS = 4,    // 4 queues: metal, opaque, glass, mixed
     C_CHUNKS = 10,  // entries of the chunk table
     C_PARTS = 11,   // 1, or 8 for per-XCD parts of the queues (wave_share)
+    C_NEED = 12,    // the last C_NEED entries of the finished-path queue take a new work item
     C_COUNT = 16
 };
 
 // Slot stages: the stage code (low 3 bits), the extension ray's bin (bits 3-6: 0 camera rays, 1-8
-// continuing rays by direction octant) and the pending-shadow-ray bit.
-enum : uint8_t { S_IDLE = 0, S_EXT = 1, S_MISS = 2, S_CLASS = 3, S_TERM = 7, S_CODE = 7, S_SHADOW = 0x80 };
+// continuing rays by direction octant) or, for a finished path, whether its work item is finished too
+// (S_NEED: the slot takes a new item), and the pending-shadow-ray bit.
+enum : uint8_t { S_IDLE = 0, S_EXT = 1, S_MISS = 2, S_CLASS = 3, S_TERM = 7, S_CODE = 7, S_NEED = 0x08, S_SHADOW = 0x80 };
 constexpr int kBins = 9;
 
 PH_DEV uint8_t ext_stage(int bin) { return uint8_t(S_EXT | (bin << 3)); }
@@ -144,18 +146,6 @@ PH_DEV void flush_counts(const TraceCount& c, TraceTotals* totals) {
     }
 }
 
-// Gives every lane with `pred` the next work item index (one atomic per wave). All lanes must call.
-PH_DEV long long wave_take(unsigned long long* counter, bool pred) {
-    const uint64_t mask = __ballot(pred);
-    if (mask == 0) return 0;
-    const int lane = lane_id();
-    const int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
-    unsigned long long first = 0;
-    if (lane == leader) first = atomicAdd(counter, (unsigned long long)__popcll(mask));
-    first = __shfl(first, leader);
-    return (long long)first + __popcll(mask & ((1ull << lane) - 1ull));
-}
-
 // Adds to counters[key] the number of lanes with `pred` and that key: one atomic per distinct key in
 // the wave instead of one per lane (lanes finishing work for one job would otherwise serialize on its
 // counter). All lanes must call.
@@ -168,10 +158,6 @@ PH_DEV void wave_count(int* counters, int key, bool pred) {
         if (lane_id() == leader) atomicAdd(counters + k, __popcll(same));
         mask &= ~same;
     }
-}
-
-PH_DEV long long items_next(const Work& work) {
-    return (long long)__hip_atomic_load(work.items, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Work item -> pixel of a w x h image: 8-row bands, 8-column tiles inside a band (the last tile and band
@@ -222,6 +208,14 @@ PH_DEV const JobParams& job_of(const Work& work, const State& st, int slot, cons
     return Multi ? work.jobs[st.job[slot]] : single;
 }
 
+// Stage of a path that ends with its current sample: S_TERM, plus S_NEED when that sample is the last of
+// the slot's work item (the pixel's samples, or the pass or sub-stream the slot runs).
+template <bool Multi>
+PH_DEV uint8_t term_stage(const JobParams& jp, const State& st, int slot) {
+    const int end = (Multi || jp.split > 1) ? st.sample_end[slot] : jp.spp;
+    return uint8_t(S_TERM | (st.sample[slot] + 1 >= end ? S_NEED : 0));
+}
+
 PH_DEV void start_sample(const JobParams& jp, const State& st, int slot, int pixel, int k) {
     const int x = pixel % jp.width, y = pixel / jp.width;
     Pcg rng = pixel_rng(uint32_t(pixel), jp.frame0 + uint32_t(k), jp.seed);
@@ -255,6 +249,13 @@ PH_DEV void start_work(const Work& work, const State& st, int chunks, int slot, 
         st.job[slot] = pass.x;
         st.part[slot] = part;
         st.sample_end[slot] = part * jp.span + pass.z;
+    } else if (jp.split > 1) {
+        // A single render in sub-streams: item = part * pixels + pixel item; the part's samples in order.
+        const int part = local / jp.pixels;
+        local -= part * jp.pixels;
+        k0 = part * jp.span;
+        st.part[slot] = part;
+        st.sample_end[slot] = k0 + jp.span;
     }
     start_sample(jp, st, slot, work_pixel(local, jp.width, jp.height), k0);
 }
@@ -290,16 +291,21 @@ enum { K_EXT = 0, K_HIT = 1, K_DONE = 2 };
 
 template <int Kind>
 struct Compact {
-    static constexpr int cats = Kind == K_EXT ? kBins : Kind == K_HIT ? 5 : 2;
-    static constexpr int counter = Kind == K_EXT ? C_EXT : Kind == K_HIT ? C_MISS : C_SHADOW;
-    // The extension queue is one list, its bins one after another; the others are one list each.
-    static constexpr bool concatenated = Kind == K_EXT;
-    // Category bits of a stage byte: extension bins; miss + 4 classes; shadow + finished (or idle with work).
+    static constexpr int cats = Kind == K_EXT ? kBins : Kind == K_HIT ? 5 : 3;
+    // Output list of a category; the categories of one list follow each other in category order. The
+    // extension queue is one list of bins; the hit queues are one list each; the finished-path queue
+    // holds the paths that continue their work item, then those that take a new one (and idle slots).
+    static PH_DEV constexpr int list(int k) { return Kind == K_EXT ? 0 : Kind == K_HIT ? k : (k == 0 ? 0 : 1); }
+    static constexpr int lists = Kind == K_EXT ? 1 : Kind == K_HIT ? 5 : 2;
+    static PH_DEV constexpr int counter(int l) { return Kind == K_EXT ? C_EXT : Kind == K_HIT ? C_MISS + l : (l == 0 ? C_SHADOW : C_TERM); }
+    // Category bits of a stage byte: extension bins; miss + 4 classes; shadow, finished path continuing
+    // its item, finished path taking a new item (or idle slot while work waits).
     static PH_DEV unsigned bits(uint8_t s, bool idle_takes_work) {
         const int code = s & S_CODE;
         if (Kind == K_EXT) return code == S_EXT ? 1u << ((s >> 3) & 15) : 0u;
         if (Kind == K_HIT) return (code >= S_MISS && code < S_CLASS + kClasses) ? 1u << (code - S_MISS) : 0u;
-        return ((s & S_SHADOW) ? 1u : 0u) | ((code == S_TERM || (idle_takes_work && code == S_IDLE)) ? 2u : 0u);
+        const unsigned term = code == S_TERM ? ((s & S_NEED) ? 4u : 2u) : (idle_takes_work && code == S_IDLE) ? 4u : 0u;
+        return ((s & S_SHADOW) ? 1u : 0u) | term;
     }
 };
 
@@ -380,8 +386,8 @@ __global__ __launch_bounds__(kCompactThreads) void compact_scatter_kernel(const 
         for (int k = 0; k < K; k++) cnt[k] += (m[j] >> k) & 1;
     }
     const int lane = lane_id(), wave = threadIdx.x >> 6;
-    int pos[K], chunk_tot[K];
-    int offset = 0;  // concatenated lists: entries of the earlier categories
+    int pos[K], cat_tot[K];
+    int offset[Compact<Kind>::lists] = {};  // entries of the list's earlier categories
     for (int k = 0; k < K; k++) {
         int before = 0, all = 0;
         for (int i = threadIdx.x; i < nchunks; i += kCompactThreads) {
@@ -389,8 +395,10 @@ __global__ __launch_bounds__(kCompactThreads) void compact_scatter_kernel(const 
             if (i < c) before += v;
             all += v;
         }
-        const int base = block_sum(before, red) + offset;
-        if (Compact<Kind>::concatenated) offset += block_sum(all, red);
+        const int l = Compact<Kind>::list(k);
+        const int base = block_sum(before, red) + offset[l];
+        cat_tot[k] = block_sum(all, red);
+        offset[l] += cat_tot[k];
         // Inclusive scan over the wave, then over the waves of the block.
         int v = cnt[k];
         for (int off = 1; off < 64; off <<= 1) {
@@ -399,22 +407,16 @@ __global__ __launch_bounds__(kCompactThreads) void compact_scatter_kernel(const 
         }
         if (lane == 63) wave_tot[k][wave] = v;
         __syncthreads();
-        int prev = 0, tot = 0;
-        for (int w = 0; w < W; w++) {
-            if (w < wave) prev += wave_tot[k][w];
-            tot += wave_tot[k][w];
-        }
+        int prev = 0;
+        for (int w = 0; w < wave; w++) prev += wave_tot[k][w];
         pos[k] = base + prev + v - cnt[k];
-        chunk_tot[k] = base + tot;
     }
     for (int j = 0; j < 16; j++)
         for (int k = 0; k < K; k++)
-            if ((m[j] >> k) & 1) out.q[Compact<Kind>::concatenated ? 0 : k][pos[k]++] = first + j;
+            if ((m[j] >> k) & 1) out.q[Compact<Kind>::list(k)][pos[k]++] = first + j;
     if (c == nchunks - 1 && threadIdx.x == 0) {
-        if (Compact<Kind>::concatenated)
-            counters[Compact<Kind>::counter] = chunk_tot[K - 1];
-        else
-            for (int k = 0; k < K; k++) counters[Compact<Kind>::counter + k] = chunk_tot[k];
+        for (int l = 0; l < Compact<Kind>::lists; l++) counters[Compact<Kind>::counter(l)] = offset[l];
+        if (Kind == K_DONE) counters[C_NEED] = cat_tot[2];
     }
 }
 
@@ -521,7 +523,7 @@ __global__ __launch_bounds__(256) void miss_kernel(Work work, State st, int* cou
                 jp.albedo[p] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
                 jp.nd[p] = make_float4(-rd.x, -rd.y, -rd.z, 0.0f);
             }
-            st.stage[slot] = S_TERM;
+            st.stage[slot] = term_stage<Multi>(jp, st, slot);
         }
     }
 }
@@ -623,7 +625,7 @@ __global__ __launch_bounds__(256) void shade_kernel(Work work, State st, int* co
                 st.rng[slot] = rng.state;
             }
             st.L[slot] = make_float4(L.x, L.y, L.z, 0.0f);
-            st.stage[slot] = uint8_t((cont ? ext_stage(bin) : S_TERM) | (shadow ? S_SHADOW : 0));
+            st.stage[slot] = uint8_t((cont ? ext_stage(bin) : term_stage<Multi>(jp, st, slot)) | (shadow ? S_SHADOW : 0));
         }
     }
 }
@@ -634,6 +636,10 @@ __global__ __launch_bounds__(256) void shade_kernel(Work work, State st, int* co
 template <bool Multi>
 __global__ __launch_bounds__(256) void regen_kernel(Work work, State st, int* counters) {
     const int n = counters[C_TERM];
+    // The queue's last C_NEED entries take new work items, in queue order: entry i gets item
+    // base + (i - first_need). No atomics: advance_items_kernel moves the counter on afterwards.
+    const int first_need = n - counters[C_NEED];
+    const long long base = (long long)work.items[0];
     const long long total = (long long)work.items[1];
     const int chunks = counters[C_CHUNKS];
     const JobParams single = single_job<Multi>(work);
@@ -650,7 +656,8 @@ __global__ __launch_bounds__(256) void regen_kernel(Work work, State st, int* co
             int k = st.sample[slot];
             f3 L = xyz(st.L[slot]);
             if (!finite3(L) || L.x < 0 || L.y < 0 || L.z < 0) L = mk3(0);
-            float4* acc = jp.accum + (Multi ? st.part[slot] * jp.pixels : 0) + pixel;
+            const bool parts = Multi || jp.split > 1;
+            float4* acc = jp.accum + (parts ? st.part[slot] * jp.pixels : 0) + pixel;
             float4 a = *acc;
             a.x += L.x;
             a.y += L.y;
@@ -659,7 +666,7 @@ __global__ __launch_bounds__(256) void regen_kernel(Work work, State st, int* co
             a.w += l * l;
             *acc = a;
             k++;
-            need_work = k >= (Multi ? st.sample_end[slot] : jp.spp);
+            need_work = k >= (parts ? st.sample_end[slot] : jp.spp);
             if (need_work) {
                 item_done = true;
                 if (Multi) job = st.job[slot];
@@ -670,14 +677,34 @@ __global__ __launch_bounds__(256) void regen_kernel(Work work, State st, int* co
         } else if (active) {
             need_work = true;  // an idle slot
         }
-        wave_count(work.job_done, job, item_done);
-        // Skip the atomic when every work item has been handed out (the counter only grows).
-        const bool want = need_work && items_next(work) < total;
-        const long long w = wave_take(work.items, want);
-        const bool got = want && w < total;
+        // Finished items per job (serving progress); a single render needs no count.
+        if (Multi) wave_count(work.job_done, job, item_done);
+        const long long w = base + (i - first_need);
+        const bool got = need_work && i >= first_need && w < total;
         if (got) start_work<Multi>(work, st, chunks, slot, w, single);
         if (active) st.stage[slot] = (ext || got) ? ext_stage(0) : S_IDLE;
     }
+}
+
+// After regeneration: the items handed out by rank move the hand-out counter on.
+__global__ void advance_items_kernel(unsigned long long* items, const int* counters) {
+    if (threadIdx.x == 0) items[0] += (unsigned long long)counters[C_NEED];
+}
+
+// The end of a single render in sub-streams: adds the partial sums to the accumulation, sub-stream 0
+// first.
+__global__ void add_parts_kernel(const float4* parts, int pixels, int count, float4* accum) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= pixels) return;
+    float4 a = accum[i];
+    for (int p = 0; p < count; p++) {
+        const float4 s = parts[size_t(p) * size_t(pixels) + size_t(i)];
+        a.x += s.x;
+        a.y += s.y;
+        a.z += s.z;
+        a.w += s.w;
+    }
+    accum[i] = a;
 }
 
 int device_cus() {
@@ -760,7 +787,7 @@ std::vector<int> material_classes(const DScene& scene, hipStream_t st) {
 }
 
 struct WavefrontEngine::Impl {
-    ph_wavefront_config cfg{128, 24, 0, 16, 1, 1, 16, 16, 0, 1, 1};
+    ph_wavefront_config cfg{128, 24, 0, 16, 1, 1, 16, 16, 0, 1, 1, 1};
     int slots = 0;
     int cus = 0;
     DevBuf<int> job, pixel, sample, sample_end, part, counters, q_ext, q_shadow, q_term, q_miss, q_cls[kClasses], mat_class, partial;
@@ -771,6 +798,7 @@ struct WavefrontEngine::Impl {
     DevBuf<int4> chunk_pass;
     DevBuf<uint64_t> rng;
     DevBuf<float4> ro, rd, beta, L, hit, so, sd, sc;
+    DevBuf<float4> part_sums;  // partial sums of a single render in sub-streams
     DevBuf<JobParams> jobs;
     DevBuf<TraceTotals> totals;             // [0] extension rays, [1] shadow rays
     DevBuf<unsigned long long> ray_totals;  // rays traced: [0] extension, [1] shadow
@@ -925,6 +953,7 @@ struct WavefrontEngine::Impl {
         fns.shadow(st, tgrid, w, s, cnt, 0, refill, cfg.lds_nodes, totals.ptr + 1, count);
         if (ev) PH_CHECK(hipEventRecord(ev[3], st));
         hipLaunchKernelGGL(regen_kernel<Multi>, dim3(sgrid), dim3(256), 0, st, w, s, cnt);
+        hipLaunchKernelGGL(advance_items_kernel, dim3(1), dim3(64), 0, st, items.ptr, cnt);
         compact<K_EXT, Multi>(st, s);
         PH_CHECK(hipGetLastError());
     }
@@ -1028,6 +1057,7 @@ void WavefrontEngine::set_config(const ph_wavefront_config& c) {
     if (c.graph_iterations < 1 || c.graph_iterations > 256) throw std::invalid_argument("graph_iterations must be in [1, 256]");
     if (c.refill_lanes < 1 || c.refill_lanes > 64) throw std::invalid_argument("refill_lanes must be in [1, 64]");
     if (c.ray_bins != 0 && c.ray_bins != 1) throw std::invalid_argument("ray_bins must be 0 or 1");
+    if (c.sample_split < 1 || c.sample_split > 256) throw std::invalid_argument("sample_split must be in [1, 256]");
     if (c.xcd_parts != 1 && c.xcd_parts != 8) throw std::invalid_argument("xcd_parts must be 1 or 8");
     if (c.lds_nodes < 0 || c.lds_nodes > DeviceScene::kTopNodes || size_t(c.block) * size_t(c.stack_depth) * 4 + size_t(c.lds_nodes) * 64 > 65536)
         throw std::invalid_argument("lds_nodes does not fit the block's LDS with its stacks");
@@ -1044,20 +1074,33 @@ void WavefrontEngine::render(hipStream_t st, const WavefrontJob& job, bool count
     const int pixels = job.width * job.height;
     if (pixels <= 0 || job.spp <= 0) return;
     if (m.serving) throw std::runtime_error("the engine is serving; a progressive render needs its own engine");
-    // Paths in flight: one per pixel up to a cap of 4M slots (~900 MB of state).
-    const int slots = m.cfg.path_slots > 0 ? std::min(m.cfg.path_slots, pixels) : std::min(pixels, 1 << 22);
+    // Sub-streams per pixel: the largest divisor of spp not above the configured split.
+    int split = 1;
+    for (int k = 1; k <= std::min(m.cfg.sample_split, job.spp); k++)
+        if (job.spp % k == 0) split = k;
+    const int64_t items = int64_t(pixels) * split;
+    if (items > int64_t(INT32_MAX)) throw std::invalid_argument("pixels x sub-streams must stay below 2^31");
+    // Paths in flight: one per pixel up to a cap of 4M slots (~900 MB of state), never more than items.
+    const int slots = int(std::min<int64_t>(m.cfg.path_slots > 0 ? m.cfg.path_slots : std::min(pixels, 1 << 22), items));
     m.ensure_slots(slots);
     if (!job.mat_class) {
         std::vector<int> cls = material_classes(job.scene, st);
         m.mat_class.upload(cls.data(), cls.size(), st);
     }
-    const JobParams jp = m.params_of(job, job.mat_class ? job.mat_class : m.mat_class.ptr);
+    WavefrontJob sub = job;
+    sub.split = split;
+    if (split > 1) {
+        m.part_sums.allocate(size_t(items));
+        m.part_sums.zero(st);
+        sub.accum = m.part_sums.ptr;
+    }
+    const JobParams jp = m.params_of(sub, job.mat_class ? job.mat_class : m.mat_class.ptr);
     PH_CHECK(hipMemcpyAsync(m.jobs.ptr, &jp, sizeof(jp), hipMemcpyHostToDevice, st));
     m.reset_counters(st);
     // Work items 0..slots-1 go to the slots directly; regeneration hands out the rest.
     PH_CHECK(hipStreamSynchronize(st));  // the staging buffer may still feed an earlier copy
-    m.host_stage[0] = (unsigned long long)std::min(slots, pixels);
-    m.host_stage[1] = (unsigned long long)pixels;
+    m.host_stage[0] = (unsigned long long)slots;
+    m.host_stage[1] = (unsigned long long)items;
     PH_CHECK(hipMemcpyAsync(m.items.ptr, m.host_stage, 2 * sizeof(unsigned long long), hipMemcpyHostToDevice, st));
     PH_CHECK(hipMemsetAsync(m.job_done.ptr, 0, sizeof(int), st));
     hipLaunchKernelGGL(init_kernel, dim3((slots + 255) / 256), dim3(256), 0, st, m.work(), m.state(), slots);
@@ -1065,8 +1108,8 @@ void WavefrontEngine::render(hipStream_t st, const WavefrontJob& job, bool count
     PH_CHECK(hipGetLastError());
 
     const int G = m.cfg.graph_iterations;
-    // A sample takes at most max_depth bounces and a slot runs ceil(pixels / slots) pixels.
-    const int64_t bound = int64_t(job.spp) * int64_t(std::max(1, job.opt.max_depth)) * int64_t((pixels + slots - 1) / slots) + 2 * G;
+    // A sample takes at most max_depth bounces and a slot runs ceil(items / slots) items of spp/split samples.
+    const int64_t bound = int64_t(job.spp / split) * int64_t(std::max(1, job.opt.max_depth)) * ((items + slots - 1) / slots) + 2 * G;
     double ms_extend = 0, ms_shadow = 0;
     int64_t iterations = 0;
     for (;;) {
@@ -1076,6 +1119,10 @@ void WavefrontEngine::render(hipStream_t st, const WavefrontJob& job, bool count
         iterations += G;
         if (m.host_counters[C_EXT] == 0) break;
         if (iterations > bound) throw std::runtime_error("wavefront loop did not drain");
+    }
+    if (split > 1) {
+        hipLaunchKernelGGL(add_parts_kernel, dim3((pixels + 255) / 256), dim3(256), 0, st, m.part_sums.ptr, pixels, split, job.accum);
+        PH_CHECK(hipGetLastError());
     }
     TraceTotals tot[2] = {};
     if (counting) PH_CHECK(hipMemcpy(tot, m.totals.ptr, sizeof(tot), hipMemcpyDeviceToHost));
@@ -1164,6 +1211,13 @@ ph_wavefront_config WavefrontEngine::autotune(hipStream_t st, const WavefrontJob
         if (slots > 0 && slots < 65536) continue;
         ph_wavefront_config c = best;
         c.path_slots = slots;
+        double t = time_of(c);
+        if (t < best_ms) best_ms = t, best = c;
+    }
+    for (int split : {1, 2, 4, 8, 16}) {
+        if (split > job.spp || job.spp % split) continue;
+        ph_wavefront_config c = best;
+        c.sample_split = split;
         double t = time_of(c);
         if (t < best_ms) best_ms = t, best = c;
     }

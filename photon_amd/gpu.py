@@ -25,7 +25,8 @@ class GpuInfo(C.Structure):
 class WavefrontConfig(C.Structure):
     _fields_ = [("block", C.c_int32), ("stack_depth", C.c_int32), ("path_slots", C.c_int32), ("blocks_per_cu", C.c_int32),
                 ("sort_materials", C.c_int32), ("use_graph", C.c_int32), ("graph_iterations", C.c_int32),
-                ("refill_lanes", C.c_int32), ("lds_nodes", C.c_int32), ("ray_bins", C.c_int32), ("xcd_parts", C.c_int32)]
+                ("refill_lanes", C.c_int32), ("lds_nodes", C.c_int32), ("ray_bins", C.c_int32), ("xcd_parts", C.c_int32),
+                ("sample_split", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

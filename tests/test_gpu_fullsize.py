@@ -90,6 +90,12 @@ def test_field_workload_sampled_pixels(field):
         r.close()
     # The two engines accumulate every pixel in the same order.
     assert accs[gpu.ENGINE_WAVEFRONT].tobytes() == accs[gpu.ENGINE_MEGAKERNEL].tobytes()
+    # With 8 sub-streams per pixel: the in-order sums of each eighth, added in order.
+    r = uploaded(field, W, H, engine=gpu.ENGINE_WAVEFRONT)
+    r.set_wavefront_config(sample_split=8)
+    r.render(cam, rs, SPP)
+    img8 = r.download_image()
+    r.close()
     g = np.random.default_rng(5)
     n = 384
     xs, ys = g.integers(0, W, n), g.integers(0, H, n)
@@ -99,10 +105,18 @@ def test_field_workload_sampled_pixels(field):
     for k in range(SPP):
         sums += rgb[k::SPP]
     ref = sums * (np.float32(1.0) / np.float32(SPP))
-    got = img[ys, xs]
-    agree = radiance_agreement(got[:, None, :], ref[:, None, :])
-    print(f"field 1080p x 64, {n} pixels: {100 * agree:.3f}% within 0.1%, {100 * bitwise_fraction(got, ref):.2f}% bitwise")
-    assert agree == 1.0
+    parts = np.zeros((8, n, 3), np.float32)
+    for k in range(SPP):
+        parts[k // 8] += rgb[k::SPP]
+    split = np.zeros((n, 3), np.float32)
+    for p in range(8):
+        split += parts[p]
+    ref8 = split * (np.float32(1.0) / np.float32(SPP))
+    for name, image, want in (("in order", img, ref), ("8 sub-streams", img8, ref8)):
+        got = image[ys, xs]
+        agree = radiance_agreement(got[:, None, :], want[:, None, :])
+        print(f"field 1080p x 64 {name}, {n} pixels: {100 * agree:.3f}% within 0.1%, {100 * bitwise_fraction(got, want):.2f}% bitwise")
+        assert agree == 1.0
 
 
 def test_svgf_workload_gbuffers(field):

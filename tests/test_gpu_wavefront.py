@@ -2,7 +2,8 @@
 
 A slot runs one pixel's samples in order, so the wavefront image must be bit-identical to the
 one-thread-per-pixel kernel's for every launch shape, queue size and graph setting, and within the
-workload tolerance (0.1% on 99.9% of pixels) of the CPU oracle.
+workload tolerance (0.1% on 99.9% of pixels) of the CPU oracle. With the samples of each pixel split
+into sub-streams, the image is deterministic for a given split and within rounding of the in-order sum.
 """
 import numpy as np
 import pytest
@@ -52,6 +53,33 @@ def test_wavefront_config_does_not_change_the_image(cfg):
     _, ref_acc, _, _ = _render(scene, 64, 64, 4, gpu.ENGINE_MEGAKERNEL)
     _, acc, _, _ = _render(scene, 64, 64, 4, gpu.ENGINE_WAVEFRONT, cfg=cfg)
     assert acc.tobytes() == ref_acc.tobytes()
+
+
+@pytest.mark.parametrize("cfg", [dict(sample_split=2), dict(sample_split=4, path_slots=1500), dict(sample_split=8, use_graph=0),
+                                 dict(sample_split=3, refill_lanes=1), dict(sample_split=8, path_slots=20000, block=256, stack_depth=16)])
+def test_sample_split_is_deterministic_and_within_tolerance(cfg):
+    """Sub-streams change only how each pixel's in-order sample sums are grouped: the image is within
+    rounding of the in-order accumulation, and the same for any launch shape with the same split."""
+    scene = host.Scene.builtin("cornell-c1").build()
+    w = h = 64
+    _, ref_acc, ref_g, _ = _render(scene, w, h, 8, gpu.ENGINE_MEGAKERNEL)
+    _, acc, g, st = _render(scene, w, h, 8, gpu.ENGINE_WAVEFRONT, cfg=cfg)
+    other = dict(cfg, path_slots=333, block=64, stack_depth=24, use_graph=1, graph_iterations=5)
+    _, acc2, _, _ = _render(scene, w, h, 8, gpu.ENGINE_WAVEFRONT, cfg=other)
+    assert acc.tobytes() == acc2.tobytes()
+    agree = radiance_agreement(acc[..., :3], ref_acc[..., :3])
+    print(f"{cfg}: {100 * agree:.4f}% within 0.1%, {100 * bitwise_fraction(acc, ref_acc):.2f}% bitwise, {st.iterations} iterations")
+    assert agree >= 0.999
+    assert np.allclose(acc[..., 3], ref_acc[..., 3], rtol=1e-5, atol=1e-6)
+    # The first sample's G-buffer is unchanged.
+    assert g[0].tobytes() == ref_g[0].tobytes() and g[1].tobytes() == ref_g[1].tobytes()
+
+
+def test_sample_split_accumulates_progressively():
+    scene = host.Scene.builtin("materials").build()
+    _, a, _, _ = _render(scene, 48, 32, 8, gpu.ENGINE_MEGAKERNEL, calls=2)
+    _, b, _, _ = _render(scene, 48, 32, 8, gpu.ENGINE_WAVEFRONT, calls=2, cfg=dict(sample_split=4))
+    assert radiance_agreement(b[..., :3], a[..., :3]) >= 0.999
 
 
 def test_wavefront_matches_oracle_on_the_field_scene():
