@@ -65,7 +65,9 @@ typedef struct ph_wavefront_config {
                               to the per-pixel kernel's (the reference's order). S > 1: each pixel's samples run as S
                               sub-streams of spp/S samples (S rounded down to a divisor of spp) that any slot may take,
                               into partial sums that are added to the accumulation in sub-stream order; deterministic,
-                              within rounding of the in-order sum, and free of the tail of pixels with long paths */
+                              within rounding of the in-order sum, and free of the tail of pixels with long paths.
+                              With S = spp every partial sum is one sample and the accumulation is the in-order one,
+                              bit for bit */
 } ph_wavefront_config;
 
 /* Statistics of the last render call. After render_denoised, ms_shade is the one-sample G-buffer trace

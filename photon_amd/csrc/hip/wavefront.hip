@@ -810,7 +810,7 @@ struct WavefrontEngine::Impl {
     std::vector<hipEvent_t> events;
     hipGraphExec_t graph_exec = nullptr;
     std::tuple<int, int, int, int, int, bool> graph_key{};
-    std::map<std::tuple<const void*, int, int, int>, ph_wavefront_config> tuned;
+    std::map<std::tuple<const void*, int, int, int, int>, ph_wavefront_config> tuned;
     // Serving bookkeeping (host side).
     bool serving = false;
     std::vector<bool> entry_used;
@@ -1148,7 +1148,7 @@ void WavefrontEngine::render(hipStream_t st, const WavefrontJob& job, bool count
 
 ph_wavefront_config WavefrontEngine::autotune(hipStream_t st, const WavefrontJob& job) {
     Impl& m = *impl_;
-    auto key = std::make_tuple(static_cast<const void*>(job.scene.nodes), job.width, job.height, job.opt.max_depth);
+    auto key = std::make_tuple(static_cast<const void*>(job.scene.nodes), job.width, job.height, job.opt.max_depth, job.spp);
     auto found = m.tuned.find(key);
     if (found != m.tuned.end()) {
         set_config(found->second);
@@ -1166,67 +1166,73 @@ ph_wavefront_config WavefrontEngine::autotune(hipStream_t st, const WavefrontJob
     };
     ph_wavefront_config best = m.cfg;
     double best_ms = time_of(best);
-    // Coordinate descent: traversal shape, persistent grid size, paths in flight, graph replay.
+    auto consider = [&](const ph_wavefront_config& c) {
+        const double t = time_of(c);
+        if (t < best_ms) best_ms = t, best = c;
+    };
+    // Coordinate descent, largest effects first. Sub-streams per pixel (the caller's spp decides which
+    // divide it) and paths in flight: with sub-streams there are more work items than pixels, more slots
+    // than pixels pay, and every traversal launch gets more rays to hide latency behind.
+    const int pixels = job.width * job.height;
+    for (int split : {2, 4, 8, 16, 32, 64}) {
+        if (split > job.spp || job.spp % split) continue;
+        ph_wavefront_config c = best;
+        c.sample_split = split;
+        consider(c);
+    }
+    for (int64_t slots : {int64_t(pixels) / 4, int64_t(pixels) / 2, int64_t(0), int64_t(pixels) * 2, int64_t(pixels) * 4, int64_t(pixels) * 8}) {
+        if (slots > 0 && slots < 65536) continue;
+        if (slots > pixels && (best.sample_split < 2 || slots > int64_t(pixels) * best.sample_split || slots > (int64_t(1) << 25))) continue;
+        ph_wavefront_config c = best;
+        c.path_slots = int(slots);
+        consider(c);
+    }
+    // Traversal shape, persistent grid size, lane refill.
     for (int block : {64, 128, 256})
         for (int depth : {8, 16, 24}) {
             ph_wavefront_config c = best;
             c.block = block;
             c.stack_depth = depth;
-            double t = time_of(c);
-            if (t < best_ms) best_ms = t, best = c;
+            consider(c);
         }
-    for (int bpc : {2, 4, 8, 16, 24, 32}) {
+    for (int bpc : {4, 8, 16, 24, 32}) {
         ph_wavefront_config c = best;
         c.blocks_per_cu = bpc;
-        double t = time_of(c);
-        if (t < best_ms) best_ms = t, best = c;
+        consider(c);
     }
     for (int refill : {4, 16, 32, 64}) {
         ph_wavefront_config c = best;
         c.refill_lanes = refill;
-        double t = time_of(c);
-        if (t < best_ms) best_ms = t, best = c;
+        consider(c);
     }
     for (int bins : {0, 1}) {
         ph_wavefront_config c = best;
         c.ray_bins = bins;
-        double t = time_of(c);
-        if (t < best_ms) best_ms = t, best = c;
+        consider(c);
     }
     for (int parts : {1, 8}) {
         ph_wavefront_config c = best;
         c.xcd_parts = parts;
-        double t = time_of(c);
-        if (t < best_ms) best_ms = t, best = c;
+        consider(c);
     }
     for (int top : {0, 128, 256, 512}) {
         ph_wavefront_config c = best;
         c.lds_nodes = top;
         if (size_t(c.block) * size_t(c.stack_depth) * 4 + size_t(top) * 64 > 65536) continue;
-        double t = time_of(c);
-        if (t < best_ms) best_ms = t, best = c;
+        consider(c);
     }
-    const int pixels = job.width * job.height;
-    for (int slots : {pixels / 4, pixels / 2, 0}) {
-        if (slots > 0 && slots < 65536) continue;
-        ph_wavefront_config c = best;
-        c.path_slots = slots;
-        double t = time_of(c);
-        if (t < best_ms) best_ms = t, best = c;
-    }
-    for (int split : {1, 2, 4, 8, 16}) {
-        if (split > job.spp || job.spp % split) continue;
-        ph_wavefront_config c = best;
-        c.sample_split = split;
-        double t = time_of(c);
-        if (t < best_ms) best_ms = t, best = c;
-    }
+    // Graph replay and its length (the loop checks for an empty queue once per replay).
     for (int g : {0, 1}) {
         ph_wavefront_config c = best;
         c.use_graph = g;
-        double t = time_of(c);
-        if (t < best_ms) best_ms = t, best = c;
+        consider(c);
     }
+    if (best.use_graph)
+        for (int gi : {4, 8, 16}) {
+            ph_wavefront_config c = best;
+            c.graph_iterations = gi;
+            consider(c);
+        }
     set_config(best);
     m.tuned[key] = best;
     PH_CHECK(hipMemsetAsync(job.accum, 0, accum_bytes, st));

@@ -4,7 +4,8 @@ The oracle checks what it can finish in seconds: a deterministic subsample of th
 image where the scene is small, and sampled pixels where it is not. Properties that do not depend on
 size check the rest:
   * any-hit occlusion of a camera ray agrees with the closest hit existing, on every ray;
-  * the wavefront engine and the one-thread-per-pixel kernel give bit-identical accumulations;
+  * the wavefront engine and the one-thread-per-pixel kernel give bit-identical accumulations, also with
+    one sample per sub-stream on more path slots than pixels;
   * the SVGF output is finite and within 1e-3 of the oracle on identical G-buffers.
 
 Workloads:
@@ -90,6 +91,14 @@ def test_field_workload_sampled_pixels(field):
         r.close()
     # The two engines accumulate every pixel in the same order.
     assert accs[gpu.ENGINE_WAVEFRONT].tobytes() == accs[gpu.ENGINE_MEGAKERNEL].tobytes()
+    # One sample per sub-stream on four path slots per pixel (the benchmark's tuned shape): the partial
+    # sums are single samples added in order, so the accumulation is still the in-order one, bit for bit.
+    r = uploaded(field, W, H, engine=gpu.ENGINE_WAVEFRONT)
+    r.set_wavefront_config(sample_split=SPP, path_slots=4 * W * H, block=128, stack_depth=16)
+    r.render(cam, rs, SPP)
+    acc64 = gpu.copy_device_to_numpy(r.buffer_ptr(gpu.BUF_ACCUM), W * H * 16, np.float32, (H, W, 4))
+    r.close()
+    assert acc64.tobytes() == accs[gpu.ENGINE_MEGAKERNEL].tobytes()
     # With 8 sub-streams per pixel: the in-order sums of each eighth, added in order.
     r = uploaded(field, W, H, engine=gpu.ENGINE_WAVEFRONT)
     r.set_wavefront_config(sample_split=8)

@@ -47,8 +47,12 @@ def test_wavefront_bitwise_equals_megakernel(name, w, h, spp):
                                  dict(path_slots=1000), dict(refill_lanes=1), dict(refill_lanes=64), dict(sort_materials=0),
                                  dict(block=256, stack_depth=16, lds_nodes=512), dict(block=128, stack_depth=24, lds_nodes=100),
                                  dict(ray_bins=0), dict(ray_bins=1, path_slots=777), dict(xcd_parts=8), dict(xcd_parts=8, path_slots=100),
-                                 dict(block=64, stack_depth=8, blocks_per_cu=2), dict(block=256, stack_depth=24, blocks_per_cu=4)])
+                                 dict(block=64, stack_depth=8, blocks_per_cu=2), dict(block=256, stack_depth=24, blocks_per_cu=4),
+                                 dict(sample_split=4, path_slots=16384), dict(sample_split=4, path_slots=5000, graph_iterations=2),
+                                 dict(sample_split=64, path_slots=12288)])
 def test_wavefront_config_does_not_change_the_image(cfg):
+    """Every launch shape gives the in-order accumulation; so does one sample per sub-stream (a split
+    equal to spp, also when asked for more), whose partial sums are single samples added in order."""
     scene = host.Scene.builtin("cornell-c1").build()
     _, ref_acc, _, _ = _render(scene, 64, 64, 4, gpu.ENGINE_MEGAKERNEL)
     _, acc, _, _ = _render(scene, 64, 64, 4, gpu.ENGINE_WAVEFRONT, cfg=cfg)
