@@ -407,7 +407,7 @@ def main():
     ap.add_argument("--serve-burst", type=int, default=48)
     ap.add_argument("--serve-load", type=float, default=0.8)
     ap.add_argument("--serve-slots", type=int, default=1 << 21)
-    ap.add_argument("--serve-iterations", type=int, default=8)
+    ap.add_argument("--serve-iterations", type=int, default=0, help="bounces per server step (0: the server's default)")
     ap.add_argument("--serve-schedule", choices=("srpt", "fifo"), default="srpt")
     ap.add_argument("--serve-pass-samples", type=int, default=0, help="pixel samples per job pass (0: the server's default)")
     ap.add_argument("--serve-min-items", type=int, default=0, help="split smaller jobs into sample sub-streams (0: the server's default)")

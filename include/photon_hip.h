@@ -201,7 +201,7 @@ typedef struct ph_server ph_server;
 typedef struct ph_server_config {
     int64_t memory_budget;       /* bytes of job frame buffers admitted at once (0: 16 GiB) */
     int32_t path_slots;          /* paths in flight shared by all jobs (0: 2^21) */
-    int32_t iterations_per_step; /* bounces per ph_server_step (0: 8) */
+    int32_t iterations_per_step; /* bounces per ph_server_step (0: 4) */
     int32_t max_jobs;            /* jobs admitted at once, at most 1024 (0: 1024) */
     int64_t pass_samples;        /* pixel samples per pass of a job (0: 4 x path_slots) */
     int32_t schedule;            /* PH_SCHEDULE_SRPT or PH_SCHEDULE_FIFO */
@@ -209,9 +209,9 @@ typedef struct ph_server_config {
                                     pixels x sub-streams reaches this (0: an eighth of the main lane's slots,
                                     all of the express lane's; -1 or 1: never split) */
     int64_t express_samples;     /* jobs of at most this many samples run in the express lane
-                                    (0: 8 x express_slots) */
+                                    (0: 16 x express_slots) */
     int32_t express_slots;       /* path slots of the express lane, a second engine on its own stream that
-                                    runs concurrently with the main lane (0: path_slots / 8; -1: no lane) */
+                                    runs concurrently with the main lane (0: path_slots / 2; -1: no lane) */
     int32_t express_iterations;  /* bounces per run of the express lane (0: iterations_per_step) */
 } ph_server_config;
 

@@ -88,12 +88,12 @@ struct ph_server {
     ph_server(const ph_server_config& c) : cfg(c) {
         if (cfg.memory_budget <= 0) cfg.memory_budget = int64_t(16) << 30;
         if (cfg.path_slots <= 0) cfg.path_slots = 1 << 21;
-        if (cfg.iterations_per_step <= 0) cfg.iterations_per_step = 8;
+        if (cfg.iterations_per_step <= 0) cfg.iterations_per_step = 4;
         if (cfg.max_jobs <= 0 || cfg.max_jobs > WavefrontEngine::kMaxJobs) cfg.max_jobs = WavefrontEngine::kMaxJobs;
         if (cfg.schedule != PH_SCHEDULE_SRPT && cfg.schedule != PH_SCHEDULE_FIFO) throw std::invalid_argument("unknown schedule");
-        if (cfg.express_slots == 0) cfg.express_slots = std::max(1, cfg.path_slots / 8);
+        if (cfg.express_slots == 0) cfg.express_slots = std::max(1, cfg.path_slots / 2);
         if (cfg.express_slots < 0) cfg.express_slots = 0;
-        if (cfg.express_samples <= 0) cfg.express_samples = int64_t(8) * cfg.express_slots;
+        if (cfg.express_samples <= 0) cfg.express_samples = int64_t(16) * cfg.express_slots;
         if (cfg.express_iterations <= 0) cfg.express_iterations = cfg.iterations_per_step;
         PH_CHECK(hipGetDevice(&device));
         const int slots[2] = {cfg.path_slots, cfg.express_slots};
