@@ -59,6 +59,7 @@ public:
         b += nodes_.bytes() + tris_.bytes() + materials_.bytes() + indices_.bytes() + tri_emissive_.bytes();
         b += normals_.bytes() + uvs_.bytes() + tangents_.bytes() + srgb_lut_.bytes() + env_texels_.bytes();
         b += env_marginal_.bytes() + env_conditional_.bytes() + textures_.bytes() + texels_.bytes();
+        b += guide_marginal_.bytes() + guide_conditional_.bytes();
         b += emissive_.bytes() + alias_.bytes() + punctual_.bytes();
         return b;
     }
@@ -136,6 +137,25 @@ public:
         v.env_conditional = or_null(env_conditional_);
         v.env_w = d.env_width;
         v.env_h = d.env_height;
+        // Guide tables for the sampling searches, one entry per power-of-two step of u.
+        v.env_guide_marginal = v.env_guide_conditional = nullptr;
+        v.env_guide_mh = v.env_guide_mw = 0;
+        if (env_n) {
+            auto pow2 = [](int n) {
+                int m = 1;
+                while (m < n && m < (1 << 16)) m <<= 1;
+                return m;
+            };
+            const int mh = pow2(d.env_height), mw = pow2(d.env_width);
+            guide_marginal_.allocate(size_t(mh) + 1);
+            guide_conditional_.allocate(size_t(d.env_height) * size_t(mw + 1));
+            build_cdf_guides(st, env_marginal_.ptr, 1, d.env_height, 0, mh, guide_marginal_.ptr);
+            build_cdf_guides(st, env_conditional_.ptr, d.env_height, d.env_width, size_t(d.env_width + 1), mw, guide_conditional_.ptr);
+            v.env_guide_marginal = guide_marginal_.ptr;
+            v.env_guide_conditional = guide_conditional_.ptr;
+            v.env_guide_mh = mh;
+            v.env_guide_mw = mw;
+        }
         v.env_intensity = d.env_intensity;
         v.env_rotation = d.env_rotation;
         v.env_integral = d.env_marginal_integral;
@@ -150,6 +170,7 @@ private:
     DevBuf<float4> nodes_, tris_, materials_;
     DevBuf<int> indices_, tri_emissive_;
     DevBuf<float> normals_, uvs_, tangents_, srgb_lut_, env_texels_, env_marginal_, env_conditional_;
+    DevBuf<int> guide_marginal_, guide_conditional_;
     DevBuf<ph_texture_info> textures_;
     DevBuf<uint32_t> texels_;
     DevBuf<ph_emissive_tri> emissive_;

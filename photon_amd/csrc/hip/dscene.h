@@ -33,6 +33,12 @@ struct DScene {
     const float* env_texels;
     const float* env_marginal;
     const float* env_conditional;
+    // Guide tables of the environment CDFs: entry k of a table with m + 1 entries (m a power of two) is
+    // the index the CDF search returns for u = k / m, so a search for u starts between entries
+    // floor(u m) and floor(u m) + 1. Null when the scene has no environment map.
+    const int* env_guide_marginal;     // env_guide_mh + 1 entries
+    const int* env_guide_conditional;  // env_h rows of env_guide_mw + 1 entries
+    int env_guide_mh, env_guide_mw;
     int num_tris, num_nodes, num_emissive, num_punctual;
     int top_nodes;  // nodes [0, top_nodes) are the top of the tree in breadth-first order
     int env_w, env_h;

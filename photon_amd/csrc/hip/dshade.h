@@ -272,6 +272,22 @@ PH_DEV int cdf_search(const float* cdf, int n, float u) {
     return lo;
 }
 
+// The same index as cdf_search, starting from a guide table (see DScene): with m a power of two, u m is
+// exact, so for k = floor(u m) the answer lies in [guide[k], guide[k + 1]] and the bisection runs over
+// that range only (a few entries instead of log2(n) dependent loads).
+PH_DEV int cdf_search_guided(const float* cdf, int n, const int* guide, int m, float u) {
+    const int k = min(int(u * float(m)), m - 1);
+    int lo = ldg(guide + k), hi = min(ldg(guide + k + 1) + 1, n);
+    while (lo + 1 < hi) {
+        int mid = (lo + hi) / 2;
+        if (ldg(cdf + mid) <= u)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
 PH_DEV float sky_pdf_dir(const DScene& s, f3 dir) {
     if (!s.env_texels) return 1.0f / (4.0f * kPi);
     f3 d = yaw(dir, -s.env_rotation);
@@ -294,11 +310,13 @@ PH_DEV bool sky_sample(const DScene& s, f2 u, f3& dir, float& pdf) {
         return true;
     }
     const int w = s.env_w, h = s.env_h;
-    int y = cdf_search(s.env_marginal, h, u.y);
+    const bool guided = s.env_guide_marginal != nullptr;
+    int y = guided ? cdf_search_guided(s.env_marginal, h, s.env_guide_marginal, s.env_guide_mh, u.y) : cdf_search(s.env_marginal, h, u.y);
     float c0 = ldg(s.env_marginal + y), c1 = ldg(s.env_marginal + y + 1);
     float dv = c1 > c0 ? (u.y - c0) / (c1 - c0) : 0.5f;
     const float* row = s.env_conditional + size_t(y) * size_t(w + 1);
-    int x = cdf_search(row, w, u.x);
+    int x = guided ? cdf_search_guided(row, w, s.env_guide_conditional + size_t(y) * size_t(s.env_guide_mw + 1), s.env_guide_mw, u.x)
+                   : cdf_search(row, w, u.x);
     float r0 = ldg(row + x), r1 = ldg(row + x + 1);
     float du = r1 > r0 ? (u.x - r0) / (r1 - r0) : 0.5f;
     f2 uv = mk2((float(x) + du) / float(w), (float(y) + dv) / float(h));

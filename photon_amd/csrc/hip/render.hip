@@ -82,6 +82,13 @@ __global__ __launch_bounds__(kPixelBlock) void sample_compact_kernel(DScene s, p
     material[i] = p.material;
 }
 
+__global__ void cdf_guide_kernel(const float* cdf, int rows, int n, size_t stride, int m, int* guide) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)rows * (m + 1)) return;
+    const int r = int(t / (m + 1)), k = int(t - (long long)r * (m + 1));
+    guide[t] = cdf_search(cdf + size_t(r) * stride, n, float(k) / float(m));
+}
+
 __global__ __launch_bounds__(kPixelBlock) void heatmap_kernel(DScene s, ph_camera_view cam, int w, int h, float* out) {
     __shared__ int lds[kMegaDepth * kPixelBlock];
     const int tid = threadIdx.y * kBX + threadIdx.x;
@@ -242,6 +249,12 @@ void launch_sample(hipStream_t st, const DScene& s, const ph_camera_view& cam, i
                    float4* color, float4* albedo, float4* nd, float4* position, float4* emission) {
     dim3 block(kBX, kBY), grid((w + kBX - 1) / kBX, (h + kBY - 1) / kBY);
     hipLaunchKernelGGL(sample_kernel, grid, block, 0, st, s, cam, w, h, frame, seed, opt, color, albedo, nd, position, emission);
+    PH_CHECK(hipGetLastError());
+}
+
+void build_cdf_guides(hipStream_t st, const float* cdf, int rows, int n, size_t stride, int m, int* guide) {
+    const long long total = (long long)rows * (m + 1);
+    hipLaunchKernelGGL(cdf_guide_kernel, dim3(unsigned((total + 255) / 256)), dim3(256), 0, st, cdf, rows, n, stride, m, guide);
     PH_CHECK(hipGetLastError());
 }
 
