@@ -249,7 +249,7 @@ def bench_serving(args, gpu, host, field, launch):
               host.Scene.spheres(256, box=100.0, alpha_fraction=0.1, seed=1).build(), field]
     log(f"serving scenes: {[s.num_tris for s in scenes]} triangles, setup {time.time() - t:.1f} s")
     srv = gpu.Server(path_slots=args.serve_slots, iterations_per_step=args.serve_iterations,
-                     schedule=gpu.SCHEDULE_FIFO if args.serve_schedule == "fifo" else gpu.SCHEDULE_SRPT,
+                     schedule={"fifo": gpu.SCHEDULE_FIFO, "srpt": gpu.SCHEDULE_SRPT, "hrrn": gpu.SCHEDULE_HRRN}[args.serve_schedule],
                      pass_samples=args.serve_pass_samples, min_items=args.serve_min_items,
                      express_slots=args.serve_express_slots, express_samples=args.serve_express_samples,
                      express_iterations=args.serve_express_iterations)
@@ -408,7 +408,7 @@ def main():
     ap.add_argument("--serve-load", type=float, default=0.8)
     ap.add_argument("--serve-slots", type=int, default=1 << 21)
     ap.add_argument("--serve-iterations", type=int, default=0, help="bounces per server step (0: the server's default)")
-    ap.add_argument("--serve-schedule", choices=("srpt", "fifo"), default="srpt")
+    ap.add_argument("--serve-schedule", choices=("srpt", "fifo", "hrrn"), default="srpt")
     ap.add_argument("--serve-pass-samples", type=int, default=0, help="pixel samples per job pass (0: the server's default)")
     ap.add_argument("--serve-min-items", type=int, default=0, help="split smaller jobs into sample sub-streams (0: the server's default)")
     ap.add_argument("--serve-express-slots", type=int, default=0, help="path slots of the express lane (0: the server's default, -1: none)")

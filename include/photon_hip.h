@@ -217,6 +217,8 @@ typedef struct ph_server_config {
 
 #define PH_SCHEDULE_SRPT 0 /* unstarted work of the job with the least remaining work first */
 #define PH_SCHEDULE_FIFO 1 /* passes in the order they were queued (round robin over running jobs) */
+#define PH_SCHEDULE_HRRN 2 /* highest response ratio next: (waiting time + estimated remaining service) / estimated
+                              remaining service, re-evaluated every step; evens out slowdown across job sizes */
 
 typedef struct ph_job_desc {
     int32_t scene; /* id returned by ph_server_add_scene */

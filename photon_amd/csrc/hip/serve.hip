@@ -70,6 +70,7 @@ struct Lane {
     int64_t pass_samples = 0;
     std::deque<int> queue;   // submitted, not admitted (FIFO)
     std::vector<int> running;
+    double rate = 0.0;       // samples per ms of the lane's recent runs (response-ratio estimates)
 };
 
 struct ph_server {
@@ -90,7 +91,8 @@ struct ph_server {
         if (cfg.path_slots <= 0) cfg.path_slots = 1 << 21;
         if (cfg.iterations_per_step <= 0) cfg.iterations_per_step = 4;
         if (cfg.max_jobs <= 0 || cfg.max_jobs > WavefrontEngine::kMaxJobs) cfg.max_jobs = WavefrontEngine::kMaxJobs;
-        if (cfg.schedule != PH_SCHEDULE_SRPT && cfg.schedule != PH_SCHEDULE_FIFO) throw std::invalid_argument("unknown schedule");
+        if (cfg.schedule != PH_SCHEDULE_SRPT && cfg.schedule != PH_SCHEDULE_FIFO && cfg.schedule != PH_SCHEDULE_HRRN)
+            throw std::invalid_argument("unknown schedule");
         if (cfg.express_slots == 0) cfg.express_slots = std::max(1, cfg.path_slots / 2);
         if (cfg.express_slots < 0) cfg.express_slots = 0;
         if (cfg.express_samples <= 0) cfg.express_samples = int64_t(16) * cfg.express_slots;
@@ -160,11 +162,29 @@ struct ph_server {
         }
     }
 
+    // Lower runs first. SRPT: the job's remaining samples; FIFO: queue order; HRRN: minus the response
+    // ratio (waiting time + estimated remaining service) / estimated remaining service.
+    double priority_of(const Lane& L, const Job& j, int k0) const {
+        const double remaining = double(j.items) * double(j.desc.spp / j.split - k0);
+        if (cfg.schedule == PH_SCHEDULE_FIFO) return 0.0;
+        if (cfg.schedule == PH_SCHEDULE_SRPT) return remaining;
+        const double service = remaining / std::max(L.rate, 1e-3);
+        const double waited = now_ms() - j.status.submit_ms;
+        return -(waited + service) / std::max(service, 1e-6);
+    }
+
+    void reprioritize(Lane& L) {
+        if (cfg.schedule != PH_SCHEDULE_HRRN) return;
+        for (int id : L.running) {
+            const Job& j = *jobs[size_t(id)];
+            if (!j.pass_pending) L.engine.serve_reprioritize(j.entry, priority_of(L, j, j.pass_k0));
+        }
+    }
+
     void queue_next_pass(Lane& L, Job& j) {
         const int span = j.desc.spp / j.split;
         const int k0 = j.next_k, k1 = std::min(span, k0 + j.pass_spp);
-        // Shortest remaining work first, or passes in the order they are queued.
-        const double priority = cfg.schedule == PH_SCHEDULE_FIFO ? 0.0 : double(j.items) * double(span - k0);
+        const double priority = priority_of(L, j, k0);
         if (!L.engine.serve_enqueue(j.entry, k0, k1, priority)) {
             j.pass_pending = true;
             return;
@@ -177,14 +197,17 @@ struct ph_server {
     }
 
     // After a run of the lane's engine: progress of its jobs, next passes, completions at time t.
-    void advance(Lane& L, double t) {
+    void advance(Lane& L, double t, double run_ms) {
         std::vector<int> still;
+        int64_t progress = 0;
         for (int id : L.running) {
             Job& j = *jobs[size_t(id)];
             const int64_t px = j.items;
             const int64_t done = L.engine.serve_pixels_done(j.entry);  // finished items, all passes so far
             const int64_t in_pass = done - px * (j.passes_queued - 1);
+            const int64_t before = j.status.samples_done;
             j.status.samples_done = px * j.pass_k0 + in_pass * (j.pass_k1 - j.pass_k0);
+            progress += j.status.samples_done - before;
             if (j.pass_pending) {
                 queue_next_pass(L, j);
             } else if (done == px * j.passes_queued) {
@@ -202,6 +225,7 @@ struct ph_server {
             still.push_back(id);
         }
         L.running.swap(still);
+        if (run_ms > 0 && progress > 0) L.rate = L.rate > 0 ? 0.8 * L.rate + 0.2 * (double(progress) / run_ms) : double(progress) / run_ms;
     }
 
     void step() {
@@ -216,6 +240,8 @@ struct ph_server {
             std::atomic<bool> m_done{false};
             double m_end = 0.0;
             std::exception_ptr m_err;
+            reprioritize(M);
+            const double m_start = now_ms();
             std::thread th([&] {
                 try {
                     PH_CHECK(hipSetDevice(device));
@@ -228,8 +254,11 @@ struct ph_server {
             });
             try {
                 do {
+                    reprioritize(X);
+                    const double t0 = now_ms();
                     X.engine.serve_run(X.stream, X.iterations);
-                    advance(X, now_ms());
+                    const double t1 = now_ms();
+                    advance(X, t1, t1 - t0);
                 } while (!m_done.load(std::memory_order_acquire) && !X.running.empty());
             } catch (...) {
                 th.join();
@@ -237,11 +266,14 @@ struct ph_server {
             }
             th.join();
             if (m_err) std::rethrow_exception(m_err);
-            advance(M, m_end);
+            advance(M, m_end, m_end - m_start);
         } else if (run_m || run_x) {
             Lane& L = run_m ? M : X;
+            reprioritize(L);
+            const double t0 = now_ms();
             L.engine.serve_run(L.stream, L.iterations);
-            advance(L, now_ms());
+            const double t1 = now_ms();
+            advance(L, t1, t1 - t0);
         }
     }
 };

@@ -57,6 +57,8 @@ public:
     void serve_begin(hipStream_t st, int slots);
     int serve_admit(hipStream_t st, const WavefrontJob& job);
     bool serve_enqueue(int entry, int k0, int k1, double priority);
+    // New priority for the queued, not yet started work of an entry (the next serve_run lays it out again).
+    void serve_reprioritize(int entry, double priority);
     void serve_run(hipStream_t st, int iterations);
     int serve_pixels_done(int entry) const;
     void serve_release(int entry);

@@ -1293,6 +1293,15 @@ bool WavefrontEngine::serve_enqueue(int entry, int k0, int k1, double priority) 
     return true;
 }
 
+void WavefrontEngine::serve_reprioritize(int entry, double priority) {
+    Impl& m = *impl_;
+    for (Impl::Pass& p : m.passes)
+        if (p.entry == entry && p.priority != priority) {
+            p.priority = priority;
+            m.chunks_dirty = true;
+        }
+}
+
 void WavefrontEngine::serve_run(hipStream_t st, int iterations) {
     Impl& m = *impl_;
     if (!m.serving) throw std::runtime_error("serve_begin was not called");

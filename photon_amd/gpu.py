@@ -61,7 +61,7 @@ class JobStatus(C.Structure):
 
 
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_RELEASED = range(4)
-SCHEDULE_SRPT, SCHEDULE_FIFO = 0, 1
+SCHEDULE_SRPT, SCHEDULE_FIFO, SCHEDULE_HRRN = 0, 1, 2
 
 
 class SvgfInput(C.Structure):
