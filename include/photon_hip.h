@@ -121,6 +121,25 @@ const char* ph_last_error(void);
 int ph_device_count(void);
 int ph_device_info(int device, ph_gpu_info* out);
 
+/* Binned-SAH BVH build on the device. No counterpart in the reference, which builds on the CPU
+ * (src/bvh/bvh.cpp:51-255); the result is that builder's, bit for bit: the same node array, the same
+ * triangle order (BVH position -> input triangle) and the same statistics, so a scene built either way
+ * traces identically. Inputs in host memory: positions (3 floats per vertex), indices (3 per triangle,
+ * each in [0, num_vertices)). bins (0: 32, at most 64), max_leaf (0: 8) and traversal_cost (0: 1) are
+ * BvhBuildOptions (src/bvh/bvh.h:32-37). Outputs in host memory: nodes (room for node_capacity,
+ * at least max(1, 2 num_tris - 1)) and order (num_tris). Returns the node count, or -1.
+ * At most PH_BVH_BUILD_MAX_TRIS triangles: larger inputs fail with an error before any device work
+ * (a 1M-triangle build made a GPU memory fault whose cause is not yet found). */
+#define PH_BVH_BUILD_MAX_TRIS 262144
+typedef struct ph_bvh_build_stats {
+    int32_t nodes, leaves, max_depth;
+    float sah_cost;
+    float bounds_min[3], bounds_max[3];
+    double build_ms; /* wall time of the call, uploads and downloads included */
+} ph_bvh_build_stats;
+int ph_bvh_build(const float* positions, int num_vertices, const int32_t* indices, int num_tris, int bins, int max_leaf, float traversal_cost,
+                 ph_bvh_node* nodes, int node_capacity, int32_t* order, ph_bvh_build_stats* stats);
+
 /* Device memory helpers for callers without a HIP toolchain of their own (bindings, tests). */
 int ph_device_alloc(int64_t bytes, void** out_dev);
 int ph_device_free(void* dev);

@@ -525,6 +525,31 @@ int ph_trace(ph_renderer* r, const ph_ray* rays, int n, int anyhit, ph_hit* hits
     });
 }
 
+int ph_bvh_build(const float* positions, int num_vertices, const int32_t* indices, int num_tris, int bins, int max_leaf, float traversal_cost,
+                 ph_bvh_node* nodes, int node_capacity, int32_t* order, ph_bvh_build_stats* stats) {
+    if (num_tris < 0 || num_vertices < 0) return fail("negative size");
+    if (num_tris > 0 && (!positions || !indices || !order)) return fail("null argument");
+    if (!nodes || !stats) return fail("null argument");
+    if (bins < 0 || bins > 64 || bins == 1) return fail("bins must be 0 (32) or in [2, 64]");
+    if (max_leaf < 0) return fail("negative leaf size");
+    if (node_capacity < std::max(1, 2 * num_tris - 1)) return fail("node array too small: it needs room for max(1, 2 num_tris - 1) nodes");
+    if (num_tris > PH_BVH_BUILD_MAX_TRIS)
+        return fail("device BVH build is limited to PH_BVH_BUILD_MAX_TRIS (262144) triangles: a 1M-triangle build made a GPU memory "
+                    "fault whose cause is not found yet, so larger builds are switched off; build larger scenes on the host");
+    // Every index is read on the device: reject any outside the vertex array here.
+    for (int64_t i = 0; i < int64_t(num_tris) * 3; i++)
+        if (indices[i] < 0 || indices[i] >= num_vertices) return fail("vertex index out of range");
+    return guarded([&] {
+        std::vector<ph_bvh_node> out;
+        std::vector<int> ord;
+        const int64_t count = bvh_build_device(positions, num_vertices, indices, num_tris, bins > 0 ? bins : 32, max_leaf > 0 ? max_leaf : 8,
+                                               traversal_cost > 0 ? traversal_cost : 1.0f, out, ord, *stats);
+        std::memcpy(nodes, out.data(), out.size() * sizeof(ph_bvh_node));
+        if (num_tris > 0) std::memcpy(order, ord.data(), ord.size() * sizeof(int));
+        return int(count);
+    });
+}
+
 int ph_svgf_create(ph_svgf** out) {
     if (!out) return fail("null argument");
     return guarded([&] {

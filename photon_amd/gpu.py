@@ -69,6 +69,11 @@ class SvgfInput(C.Structure):
                 ("position_dev", C.c_void_p), ("emission_dev", C.c_void_p)]
 
 
+class BvhBuildStats(C.Structure):
+    _fields_ = [("nodes", C.c_int32), ("leaves", C.c_int32), ("max_depth", C.c_int32), ("sah_cost", C.c_float),
+                ("bounds_min", C.c_float * 3), ("bounds_max", C.c_float * 3), ("build_ms", C.c_double)]
+
+
 class CompactGBufferDesc(C.Structure):
     _fields_ = [("color_dev", C.c_void_p), ("albedo_dev", C.c_void_p), ("emission_dev", C.c_void_p),
                 ("depth_dev", C.c_void_p), ("normal_dev", C.c_void_p), ("material_dev", C.c_void_p)]
@@ -121,6 +126,7 @@ def lib():
         L.ph_svgf_resize.argtypes = [P, C.c_int, C.c_int]
         L.ph_svgf_reset.argtypes = [P]
         L.ph_svgf_run.argtypes = [P, C.POINTER(SvgfInput), CV, C.POINTER(SvgfSettings), P]
+        L.ph_bvh_build.argtypes = [P, C.c_int, P, C.c_int, C.c_int, C.c_int, C.c_float, P, C.c_int, P, C.POINTER(BvhBuildStats)]
         L.ph_svgf_run_compact.argtypes = [P, C.POINTER(CompactGBufferDesc), CV, CV, C.POINTER(SvgfSettings), P]
         L.ph_renderer_sample_compact.argtypes = [R, CV, RS, C.POINTER(CompactGBufferDesc)]
         L.ph_svgf_variance_buffer.argtypes = [P]
@@ -342,6 +348,22 @@ class GpuRenderer:
         ms = C.c_float()
         _check(lib().ph_trace_config(self._h, rays_dev, n, int(anyhit), hits_dev, block, stack_depth, int(persistent), lds_nodes, C.byref(ms)))
         return ms.value
+
+
+def bvh_build(positions, indices, bins=0, max_leaf=0, traversal_cost=0.0):
+    """Binned-SAH BVH of triangles (ph_bvh_build, on the device): the host builder's nodes, triangle
+    order and statistics. positions: (V, 3) float32; indices: (T, 3) int32."""
+    from .types import NODE_DTYPE
+    p = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+    i = np.ascontiguousarray(indices, np.int32).reshape(-1, 3)
+    n = len(i)
+    cap = max(1, 2 * n - 1)
+    nodes = np.zeros(cap, NODE_DTYPE)
+    order = np.zeros(n, np.int32)
+    st = BvhBuildStats()
+    count = _check(lib().ph_bvh_build(p.ctypes.data if len(p) else None, len(p), i.ctypes.data if n else None, n, bins, max_leaf,
+                                      traversal_cost, nodes.ctypes.data, cap, order.ctypes.data if n else None, C.byref(st)))
+    return nodes[:count], order, st
 
 
 class CompactGBuffer:

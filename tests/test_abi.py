@@ -45,3 +45,13 @@ def test_wrapper_header_compiles():
         open(p, "w").write(src)
         r = subprocess.run(["g++", "-std=c++20", "-fsyntax-only", "-I" + os.path.join(ROOT, "include"), p], capture_output=True, text=True)
         assert r.returncode == 0, r.stderr
+
+
+def test_device_bvh_build_refuses_inputs_above_its_limit():
+    # Checked on the host before any device work, so this runs without a GPU.
+    import numpy as np
+    from photon_amd import gpu
+    limit = int(re.search(r"#define PH_BVH_BUILD_MAX_TRIS (\d+)", open(os.path.join(ROOT, "include", "photon_hip.h")).read()).group(1))
+    assert limit == 262144
+    with pytest.raises(RuntimeError, match="limited to PH_BVH_BUILD_MAX_TRIS"):
+        gpu.bvh_build(np.zeros((3, 3), np.float32), np.zeros((limit + 1, 3), np.int32))
