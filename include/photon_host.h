@@ -101,9 +101,41 @@ typedef struct phs_raw_scene {
     float env_intensity, env_rotation;
     float cam_position[3], cam_target[3], cam_up[3];
     float cam_vfov, cam_aperture, cam_focus;
+    const float* tangents; /* 4 floats per vertex (xyz, handedness), NULL when absent */
 } phs_raw_scene;
 
 int phs_scene_raw(phs_scene* s, phs_raw_scene* out);
+
+/* ---- scene files and images ---------------------------------------------------------------------- */
+
+/* Appends a glTF 2.0 file (.gltf or .glb) to the scene: meshes of the default scene under the node
+ * hierarchy, PBR materials with the transmission / ior / emissive-strength extensions, PNG textures,
+ * perspective cameras and KHR_lights_punctual lights. The first camera becomes the scene camera.
+ * <- load_gltf (src/scene/gltf.h:12, gltf.cpp:72-497). */
+int phs_scene_load_gltf(phs_scene* s, const char* path);
+/* Writes meshes, materials, camera and punctual lights as a .gltf with an embedded buffer.
+ * <- save_gltf (src/scene/gltf.h:16, gltf.cpp:501-640). */
+int phs_scene_save_gltf(const phs_scene* s, const char* path);
+/* Cameras found in loaded files (Scene::cameras, src/scene/scene.h:55). */
+int phs_scene_num_cameras(const phs_scene* s);
+int phs_scene_camera_at(const phs_scene* s, int index, float position[3], float target[3], float up[3], float* vfov_degrees);
+/* Equirectangular environment from a Radiance .hdr, a PFM or an 8-bit PNG read as sRGB.
+ * <- load_environment (src/scene/image.h:18, image.cpp:53-81). */
+int phs_scene_load_environment(phs_scene* s, const char* path, float intensity, float rotation);
+
+/* Decodes a PNG to RGBA8 (R in the low byte, row 0 at the top). With rgba NULL only the size is
+ * returned. <- decode_image (src/scene/image.h:14, image.cpp:23-37). */
+int phs_image_decode(const uint8_t* data, size_t size, int* width, int* height, uint32_t* rgba, size_t capacity);
+/* <- save_png (src/scene/image.h:20, image.cpp:84-86). */
+int phs_image_save_png(const char* path, int width, int height, const uint32_t* rgba);
+/* Linear RGB float rows, top to bottom in memory, bottom to top in the file.
+ * <- save_pfm / load_pfm (src/scene/image.h:21-22, image.cpp:88-116). */
+int phs_image_save_pfm(const char* path, int width, int height, const float* rgb);
+int phs_image_load_pfm(const char* path, int* width, int* height, float* rgb, size_t capacity);
+/* Parses JSON text and writes it canonically (members sorted, numbers as %.17g, no white space) into
+ * `out` when `capacity` exceeds its length; returns the length, or a negative code on a parse error.
+ * <- json::parse (src/scene/json.h:55). */
+int64_t phs_json_canonical(const char* text, char* out, size_t capacity);
 
 /* Pinhole rays through pixel centres (camera_ray with jitter 0.5, integrator.h:266-280), row-major. */
 int phs_camera_rays(const ph_camera_view* cam, int width, int height, ph_ray* out);

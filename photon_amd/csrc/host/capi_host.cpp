@@ -4,6 +4,9 @@
 #include <new>
 #include <string>
 
+#include "gltf.hpp"
+#include "image_io.hpp"
+#include "json.hpp"
 #include "photon_host.h"
 #include "scene.hpp"
 
@@ -300,7 +303,102 @@ int phs_scene_raw(phs_scene* s, phs_raw_scene* out) {
     out->cam_vfov = c.camera.vfov;
     out->cam_aperture = c.camera.aperture;
     out->cam_focus = c.camera.focus_distance;
+    out->tangents = c.tangents.empty() ? nullptr : &c.tangents[0].x;
     return 0;
+}
+
+int phs_scene_load_gltf(phs_scene* s, const char* path) {
+    if (!s || !path) return fail("null argument");
+    return guarded([&] {
+        std::string error;
+        return load_gltf(path, s->scene, error) ? 0 : fail(error);
+    });
+}
+
+int phs_scene_save_gltf(const phs_scene* s, const char* path) {
+    if (!s || !path) return fail("null argument");
+    return guarded([&] {
+        std::string error;
+        return save_gltf(path, s->scene, error) ? 0 : fail(error);
+    });
+}
+
+int phs_scene_num_cameras(const phs_scene* s) { return s ? int(s->scene.cameras.size()) : fail("null argument"); }
+
+int phs_scene_camera_at(const phs_scene* s, int index, float position[3], float target[3], float up[3], float* vfov) {
+    if (!s || !position || !target || !up || !vfov) return fail("null argument");
+    if (index < 0 || size_t(index) >= s->scene.cameras.size()) return fail("camera index out of range");
+    const Camera& c = s->scene.cameras[size_t(index)];
+    std::memcpy(position, &c.position, 12);
+    std::memcpy(target, &c.target, 12);
+    std::memcpy(up, &c.up, 12);
+    *vfov = c.vfov;
+    return 0;
+}
+
+int phs_scene_load_environment(phs_scene* s, const char* path, float intensity, float rotation) {
+    if (!s || !path) return fail("null argument");
+    return guarded([&] {
+        Environment env;
+        std::string error;
+        if (!load_environment(path, env, error)) return fail(error);
+        env.intensity = intensity;
+        env.rotation = rotation;
+        s->scene.env = std::move(env);
+        return 0;
+    });
+}
+
+int phs_image_decode(const uint8_t* data, size_t size, int* width, int* height, uint32_t* rgba, size_t capacity) {
+    if (!data || !width || !height) return fail("null argument");
+    return guarded([&] {
+        Texture t;
+        std::string error;
+        if (!decode_image(data, size, t, error)) return fail(error);
+        *width = t.width;
+        *height = t.height;
+        if (rgba) {
+            if (capacity < t.texels.size()) return fail("output buffer too small");
+            std::memcpy(rgba, t.texels.data(), t.texels.size() * 4);
+        }
+        return 0;
+    });
+}
+
+int phs_image_save_png(const char* path, int width, int height, const uint32_t* rgba) {
+    if (!path || !rgba || width <= 0 || height <= 0) return fail("invalid argument");
+    return guarded([&] { return save_png(path, width, height, rgba) ? 0 : fail(std::string("cannot write ") + path); });
+}
+
+int phs_image_save_pfm(const char* path, int width, int height, const float* rgb) {
+    if (!path || !rgb || width <= 0 || height <= 0) return fail("invalid argument");
+    return guarded([&] { return save_pfm(path, width, height, rgb) ? 0 : fail(std::string("cannot write ") + path); });
+}
+
+int phs_image_load_pfm(const char* path, int* width, int* height, float* rgb, size_t capacity) {
+    if (!path || !width || !height) return fail("null argument");
+    return guarded([&] {
+        std::vector<float> px;
+        if (!load_pfm(path, *width, *height, px)) return fail(std::string("cannot read PFM file ") + path);
+        if (rgb) {
+            if (capacity < px.size()) return fail("output buffer too small");
+            std::memcpy(rgb, px.data(), px.size() * 4);
+        }
+        return 0;
+    });
+}
+
+int64_t phs_json_canonical(const char* text, char* out, size_t capacity) {
+    if (!text) return fail("null argument");
+    try {
+        const std::string s = json::write(json::parse(text));
+        if (out && capacity > s.size()) std::memcpy(out, s.c_str(), s.size() + 1);
+        return int64_t(s.size());
+    } catch (const json::ParseError& e) {
+        return fail(e.what());
+    } catch (const std::exception& e) {
+        return fail(e.what());
+    }
 }
 
 int phs_camera_rays(const ph_camera_view* cam, int width, int height, ph_ray* out) {

@@ -52,6 +52,7 @@ public:
     std::vector<ph_punctual_light> punctual;
     Environment env;
     Camera camera;
+    std::vector<Camera> cameras;  // every camera a scene file defined, in file order
     std::string name;
     std::vector<vec4> spheres;  // generator metadata: centre and radius of each procedural sphere
 

@@ -55,6 +55,18 @@ def lib():
         L.phs_scene_raw.argtypes = [P, C.POINTER(RawScene)]
         L.phs_camera_rays.argtypes = [C.POINTER(CameraView), C.c_int, C.c_int, P]
         L.phs_sphere_shadow_rays.argtypes = [P, C.c_int, C.c_uint64, P]
+        L.phs_scene_load_gltf.argtypes = [P, C.c_char_p]
+        L.phs_scene_save_gltf.argtypes = [P, C.c_char_p]
+        L.phs_scene_num_cameras.argtypes = [P]
+        L.phs_scene_camera_at.argtypes = [P, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                          C.POINTER(C.c_float)]
+        L.phs_scene_load_environment.argtypes = [P, C.c_char_p, C.c_float, C.c_float]
+        L.phs_image_decode.argtypes = [C.c_char_p, C.c_size_t, C.POINTER(C.c_int), C.POINTER(C.c_int), P, C.c_size_t]
+        L.phs_image_save_png.argtypes = [C.c_char_p, C.c_int, C.c_int, P]
+        L.phs_image_save_pfm.argtypes = [C.c_char_p, C.c_int, C.c_int, P]
+        L.phs_image_load_pfm.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), P, C.c_size_t]
+        L.phs_json_canonical.argtypes = [C.c_char_p, P, C.c_size_t]
+        L.phs_json_canonical.restype = C.c_int64
         _sig_done = True
     return L
 
@@ -151,6 +163,33 @@ class Scene:
         _check(lib().phs_scene_get_camera(self._h, p, t, u, C.byref(f)))
         return dict(position=tuple(p), target=tuple(t), up=tuple(u), vfov=f.value)
 
+    # -- scene files ---------------------------------------------------------------------------------
+    @classmethod
+    def from_gltf(cls, path):
+        s = cls()
+        s.load_gltf(path)
+        return s
+
+    def load_gltf(self, path):
+        """Appends a .gltf / .glb file (src/scene/gltf.cpp:486-497)."""
+        _check(lib().phs_scene_load_gltf(self._h, str(path).encode()))
+        self.built = False
+
+    def save_gltf(self, path):
+        _check(lib().phs_scene_save_gltf(self._h, str(path).encode()))
+
+    def cameras(self):
+        out = []
+        for i in range(_check(lib().phs_scene_num_cameras(self._h))):
+            p, t, u, f = _f3((0, 0, 0)), _f3((0, 0, 0)), _f3((0, 0, 0)), C.c_float()
+            _check(lib().phs_scene_camera_at(self._h, i, p, t, u, C.byref(f)))
+            out.append(dict(position=tuple(p), target=tuple(t), up=tuple(u), vfov=f.value))
+        return out
+
+    def load_environment(self, path, intensity=1.0, rotation=0.0):
+        _check(lib().phs_scene_load_environment(self._h, str(path).encode(), intensity, rotation))
+        self.built = False
+
     def camera_view(self, aspect):
         v = CameraView()
         _check(lib().phs_scene_camera_view(self._h, aspect, C.byref(v)))
@@ -211,3 +250,41 @@ def camera_rays(view, width, height):
     out = np.zeros(width * height, RAY_DTYPE)
     _check(lib().phs_camera_rays(C.byref(view), width, height, vptr(out)))
     return out
+
+
+# -- images and JSON -------------------------------------------------------------------------------------
+def decode_image(data):
+    """PNG bytes -> (h, w) uint32 RGBA8 texels, R in the low byte (src/scene/image.cpp:23-37)."""
+    data = bytes(data)
+    w, h = C.c_int(), C.c_int()
+    _check(lib().phs_image_decode(data, len(data), C.byref(w), C.byref(h), None, 0))
+    out = np.zeros((h.value, w.value), np.uint32)
+    _check(lib().phs_image_decode(data, len(data), C.byref(w), C.byref(h), vptr(out), out.size))
+    return out
+
+
+def save_png(path, rgba):
+    a = np.ascontiguousarray(rgba, np.uint32)
+    _check(lib().phs_image_save_png(str(path).encode(), a.shape[1], a.shape[0], vptr(a)))
+
+
+def save_pfm(path, rgb):
+    a = np.ascontiguousarray(rgb, np.float32)
+    _check(lib().phs_image_save_pfm(str(path).encode(), a.shape[1], a.shape[0], vptr(a)))
+
+
+def load_pfm(path):
+    w, h = C.c_int(), C.c_int()
+    _check(lib().phs_image_load_pfm(str(path).encode(), C.byref(w), C.byref(h), None, 0))
+    out = np.zeros((h.value, w.value, 3), np.float32)
+    _check(lib().phs_image_load_pfm(str(path).encode(), C.byref(w), C.byref(h), vptr(out), out.size))
+    return out
+
+
+def json_canonical(text):
+    """Parses JSON text with the library's reader and returns its canonical form."""
+    b = text.encode() if isinstance(text, str) else bytes(text)
+    n = _check(lib().phs_json_canonical(b, None, 0))
+    buf = C.create_string_buffer(n + 1)
+    _check(lib().phs_json_canonical(b, buf, n + 1))
+    return buf.raw[:n].decode()

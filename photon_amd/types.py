@@ -135,6 +135,7 @@ class RawScene(C.Structure):
         ("env_color", F3), ("env_intensity", C.c_float), ("env_rotation", C.c_float),
         ("cam_position", F3), ("cam_target", F3), ("cam_up", F3),
         ("cam_vfov", C.c_float), ("cam_aperture", C.c_float), ("cam_focus", C.c_float),
+        ("tangents", C.POINTER(C.c_float)),
     ]
 
 
