@@ -229,6 +229,41 @@ SPPS = (4, 16, 64, 256)
 DEPTHS = (4, 8, 16)
 
 
+def bench_scene_build(gpu, host):
+    """Scene admission costs: glTF import of the 1,048,578-triangle field (written to a temporary .gltf
+    first, untimed), the threaded host BVH build of the imported scene, and the device BVH build
+    (ph_bvh_build) against the host build on the 262,144-triangle sphere soup."""
+    import tempfile
+    import numpy as np
+    out = {}
+    field = host.Scene.field(1024, seed=1, env_width=64)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "field.gltf")
+        field.save_gltf(path)
+        size = os.path.getsize(path)
+        t = time.perf_counter()
+        s = host.Scene.from_gltf(path)
+        load_s = time.perf_counter() - t
+    t = time.perf_counter()
+    s.build()
+    build_s = time.perf_counter() - t
+    out["gltf_import"] = dict(triangles=s.num_tris, file_mb=round(size / 1e6, 1), load_ms=round(load_s * 1e3, 1),
+                              mb_s=round(size / 1e6 / load_s, 1), mtris_s=round(s.num_tris / 1e6 / load_s, 2),
+                              host_bvh_ms=round(build_s * 1e3, 1), host_threads=os.cpu_count())
+    log(f"glTF import: {out['gltf_import']}")
+    soup = host.Scene.spheres(256, box=100.0, alpha_fraction=0.1, seed=1)
+    r = soup.raw()
+    p = np.ctypeslib.as_array(r.positions, (r.num_vertices * 3,)).reshape(-1, 3).copy()
+    i = np.ctypeslib.as_array(r.indices, (r.num_tris * 3,)).reshape(-1, 3).copy()
+    gpu.bvh_build(p, i)  # warm-up
+    dev = min(gpu.bvh_build(p, i)[2].build_ms for _ in range(3))
+    soup.build()
+    host_ms = soup.stats().build_ms
+    out["bvh_build"] = dict(triangles=len(i), device_ms=round(dev, 2), host_ms=round(host_ms, 2), host_threads=os.cpu_count())
+    log(f"BVH build: {out['bvh_build']}")
+    return out
+
+
 def draw_jobs(n, seed):
     """Jobs of the serving workload: resolution, spp, depth and scene drawn uniformly, seeded per job."""
     import numpy as np
@@ -402,6 +437,7 @@ def main():
     ap.add_argument("--cpu-spp", type=int, default=4, help="samples per pixel of the CPU baseline's bounded sample")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-svgf", action="store_true")
+    ap.add_argument("--no-scene-build", action="store_true", help="skip the glTF import and BVH build leg")
     ap.add_argument("--svgf-frames", type=int, default=64)
     ap.add_argument("--serve-jobs", type=int, default=200, help="jobs of the open-loop serving run (the full stream is 10,000)")
     ap.add_argument("--serve-burst", type=int, default=48)
@@ -438,6 +474,7 @@ def main():
     trav = None if args.no_traversal else bench_traversal(gpu, host)
     svgf = None if args.no_svgf else bench_svgf(args, gpu, host, scene)
     serve = None if args.no_serve else bench_serving(args, gpu, host, scene, rnd["config"])
+    build = None if args.no_scene_build else bench_scene_build(gpu, host)
     cpu = None if args.no_cpu else cpu_baseline(scene, cam, rs, args.cpu_spp)
     line = {
         "metric": METRIC,
@@ -467,6 +504,7 @@ def main():
         "traversal": trav,
         "svgf": svgf,
         "serving": serve,
+        "scene_build": build,
     }
     print(json.dumps(line), flush=True)
 

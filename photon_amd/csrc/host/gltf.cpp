@@ -135,6 +135,7 @@ std::string percent_decode(const std::string& uri) {
 }
 
 bool base64_decode(const std::string& in, std::vector<uint8_t>& out) {
+    out.reserve(out.size() + in.size() / 4 * 3 + 3);
     uint32_t acc = 0;
     int bits = 0;
     for (const char ch : in) {
@@ -182,10 +183,13 @@ std::string base64_encode(const std::vector<uint8_t>& bin) {
 }
 
 bool read_file(const fs::path& p, std::vector<uint8_t>& out) {
-    std::ifstream f(p, std::ios::binary);
+    std::ifstream f(p, std::ios::binary | std::ios::ate);
     if (!f) return false;
-    out.assign(std::istreambuf_iterator<char>(f), std::istreambuf_iterator<char>());
-    return true;
+    const std::streamoff n = f.tellg();
+    if (n < 0) return false;
+    out.resize(size_t(n));
+    f.seekg(0);
+    return bool(f.read(reinterpret_cast<char*>(out.data()), n)) || n == 0;
 }
 
 struct Failure {
@@ -637,21 +641,58 @@ bool save_gltf(const std::string& path, const Scene& scene, std::string& error) 
     const size_t nv = scene.positions.size();
     const bool has_n = scene.normals.size() == nv && nv > 0;
     const bool has_uv = scene.uvs.size() == nv && nv > 0;
-    std::vector<uint8_t> bin;
-    const size_t pos_off = put(bin, scene.positions.data(), nv * sizeof(vec3));
-    const size_t nrm_off = has_n ? put(bin, scene.normals.data(), nv * sizeof(vec3)) : 0;
-    const size_t uv_off = has_uv ? put(bin, scene.uvs.data(), nv * sizeof(vec2)) : 0;
-    vec3 lo(kInf), hi(-kInf);
-    for (const vec3& p : scene.positions) {
-        lo = vmin(lo, p);
-        hi = vmax(hi, p);
+    // One primitive per material, in material order. Each carries the vertices it references, in order
+    // of first use, so that a file of many materials does not repeat the whole vertex array per
+    // primitive (a vertex shared by several materials is written once for each).
+    struct Group {
+        std::vector<uint32_t> vertices;  // scene vertex ids
+        std::vector<uint32_t> indices;   // into `vertices`
+        size_t first = 0;                // first of its vertices in the written arrays
+    };
+    std::map<int, Group> groups;
+    std::map<int, std::map<uint32_t, uint32_t>> shared;  // local ids of vertices used by a second group
+    std::vector<int> owner(nv, -1);
+    std::vector<uint32_t> local(nv, 0);
+    for (int t = 0; t < scene.num_tris(); t++) {
+        const int m = scene.tri_material[size_t(t)];
+        Group& g = groups[m];
+        for (int k = 0; k < 3; k++) {
+            const uint32_t v = uint32_t(scene.indices[size_t(t) * 3 + size_t(k)]);
+            if (owner[v] < 0) {
+                owner[v] = m;
+                local[v] = uint32_t(g.vertices.size());
+                g.vertices.push_back(v);
+            }
+            if (owner[v] == m) {
+                g.indices.push_back(local[v]);
+                continue;
+            }
+            auto& sh = shared[m];
+            auto it = sh.find(v);
+            if (it == sh.end()) {
+                it = sh.emplace(v, uint32_t(g.vertices.size())).first;
+                g.vertices.push_back(v);
+            }
+            g.indices.push_back(it->second);
+        }
     }
-    // One index list per material, in material order.
-    std::map<int, std::vector<uint32_t>> groups;
-    for (int t = 0; t < scene.num_tris(); t++)
-        for (int k = 0; k < 3; k++) groups[scene.tri_material[size_t(t)]].push_back(uint32_t(scene.indices[size_t(t) * 3 + size_t(k)]));
+    std::vector<vec3> pos, nrm;
+    std::vector<vec2> uvs;
+    for (auto& [m, g] : groups) {
+        g.first = pos.size();
+        for (const uint32_t v : g.vertices) {
+            pos.push_back(scene.positions[v]);
+            if (has_n) nrm.push_back(scene.normals[v]);
+            if (has_uv) uvs.push_back(scene.uvs[v]);
+        }
+    }
+    const size_t nw = pos.size();
+    std::vector<uint8_t> bin;
+    const size_t pos_off = put(bin, pos.data(), nw * sizeof(vec3));
+    const size_t nrm_off = has_n ? put(bin, nrm.data(), nw * sizeof(vec3)) : 0;
+    const size_t uv_off = has_uv ? put(bin, uvs.data(), nw * sizeof(vec2)) : 0;
     std::vector<std::pair<size_t, size_t>> index_views;  // (offset, count)
-    for (const auto& [m, idx] : groups) index_views.push_back({put(bin, idx.data(), idx.size() * 4), idx.size()});
+    for (const auto& [m, g] : groups) index_views.push_back({put(bin, g.indices.data(), g.indices.size() * 4), g.indices.size()});
 
     std::ostringstream o;
     o.precision(9);
@@ -664,22 +705,40 @@ bool save_gltf(const std::string& path, const Scene& scene, std::string& error) 
     o << "{\n\"asset\": {\"version\": \"2.0\", \"generator\": \"photon_amd\"},\n";
     o << "\"extensionsUsed\": [\"KHR_materials_emissive_strength\", \"KHR_materials_transmission\", \"KHR_materials_ior\", \"KHR_lights_punctual\"],\n";
     o << "\"buffers\": [{\"byteLength\": " << bin.size() << ", \"uri\": \"data:application/octet-stream;base64," << base64_encode(bin) << "\"}],\n";
-    // Views and accessors: positions, then normals and uvs when present, then one index list per group.
+    // Views: positions, normals and uvs when present, then one index list per group. Accessors: per
+    // group its ranges of the vertex views (positions with their bounds), then its indices.
     std::vector<std::string> views, accessors;
     auto view = [&](size_t off, size_t len) {
         views.push_back("{\"buffer\": 0, \"byteOffset\": " + std::to_string(off) + ", \"byteLength\": " + std::to_string(len) + "}");
         return int(views.size()) - 1;
     };
-    auto accessor = [&](int v, int ct, size_t count, const char* type, const std::string& extra) {
-        accessors.push_back("{\"bufferView\": " + std::to_string(v) + ", \"componentType\": " + std::to_string(ct) + ", \"count\": " + std::to_string(count) +
-                            ", \"type\": \"" + type + "\"" + extra + "}");
+    auto accessor = [&](int v, size_t byte_off, int ct, size_t count, const char* type, const std::string& extra) {
+        accessors.push_back("{\"bufferView\": " + std::to_string(v) + ", \"byteOffset\": " + std::to_string(byte_off) + ", \"componentType\": " +
+                            std::to_string(ct) + ", \"count\": " + std::to_string(count) + ", \"type\": \"" + type + "\"" + extra + "}");
         return int(accessors.size()) - 1;
     };
-    const int pos_acc = accessor(view(pos_off, nv * 12), 5126, nv, "VEC3", ", \"min\": " + v3(lo) + ", \"max\": " + v3(hi));
-    const int nrm_acc = has_n ? accessor(view(nrm_off, nv * 12), 5126, nv, "VEC3", "") : -1;
-    const int uv_acc = has_uv ? accessor(view(uv_off, nv * 8), 5126, nv, "VEC2", "") : -1;
-    std::vector<int> index_acc;
-    for (const auto& [off, cnt] : index_views) index_acc.push_back(accessor(view(off, cnt * 4), 5125, cnt, "SCALAR", ""));
+    const int pos_view = view(pos_off, nw * 12);
+    const int nrm_view = has_n ? view(nrm_off, nw * 12) : -1;
+    const int uv_view = has_uv ? view(uv_off, nw * 8) : -1;
+    struct Prim {
+        int material, pos, nrm, uv, idx;
+    };
+    std::vector<Prim> prims;
+    size_t gi = 0;
+    for (const auto& [m, g] : groups) {
+        vec3 lo(kInf), hi(-kInf);
+        for (const uint32_t v : g.vertices) {
+            lo = vmin(lo, scene.positions[v]);
+            hi = vmax(hi, scene.positions[v]);
+        }
+        const size_t n = g.vertices.size();
+        Prim pr{m, accessor(pos_view, g.first * 12, 5126, n, "VEC3", ", \"min\": " + v3(lo) + ", \"max\": " + v3(hi)), -1, -1, -1};
+        if (has_n) pr.nrm = accessor(nrm_view, g.first * 12, 5126, n, "VEC3", "");
+        if (has_uv) pr.uv = accessor(uv_view, g.first * 8, 5126, n, "VEC2", "");
+        pr.idx = accessor(view(index_views[gi].first, index_views[gi].second * 4), 0, 5125, index_views[gi].second, "SCALAR", "");
+        prims.push_back(pr);
+        gi++;
+    }
     auto list = [&](const std::vector<std::string>& items) {
         std::string s;
         for (size_t i = 0; i < items.size(); i++) s += (i ? ", " : "") + items[i];
@@ -698,13 +757,12 @@ bool save_gltf(const std::string& path, const Scene& scene, std::string& error) 
           << "}}}";
     }
     o << "],\n\"meshes\": [{\"primitives\": [";
-    size_t gi = 0;
-    for (const auto& [m, idx] : groups) {
-        o << (gi ? ", " : "") << "{\"attributes\": {\"POSITION\": " << pos_acc;
-        if (has_n) o << ", \"NORMAL\": " << nrm_acc;
-        if (has_uv) o << ", \"TEXCOORD_0\": " << uv_acc;
-        o << "}, \"indices\": " << index_acc[gi] << ", \"material\": " << m << "}";
-        gi++;
+    for (size_t i = 0; i < prims.size(); i++) {
+        const Prim& pr = prims[i];
+        o << (i ? ", " : "") << "{\"attributes\": {\"POSITION\": " << pr.pos;
+        if (pr.nrm >= 0) o << ", \"NORMAL\": " << pr.nrm;
+        if (pr.uv >= 0) o << ", \"TEXCOORD_0\": " << pr.uv;
+        o << "}, \"indices\": " << pr.idx << ", \"material\": " << pr.material << "}";
     }
     o << "]}],\n";
     const Camera& c = scene.camera;

@@ -23,10 +23,13 @@ namespace phh {
 namespace {
 
 bool read_all(const std::string& path, std::vector<uint8_t>& out) {
-    std::ifstream f(path, std::ios::binary);
+    std::ifstream f(path, std::ios::binary | std::ios::ate);
     if (!f) return false;
-    out.assign(std::istreambuf_iterator<char>(f), std::istreambuf_iterator<char>());
-    return true;
+    const std::streamoff n = f.tellg();
+    if (n < 0) return false;
+    out.resize(size_t(n));
+    f.seekg(0);
+    return bool(f.read(reinterpret_cast<char*>(out.data()), n)) || n == 0;
 }
 
 uint32_t be32(const uint8_t* p) { return (uint32_t(p[0]) << 24) | (uint32_t(p[1]) << 16) | (uint32_t(p[2]) << 8) | uint32_t(p[3]); }

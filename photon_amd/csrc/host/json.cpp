@@ -116,6 +116,11 @@ private:
         require('"');
         std::string out;
         for (;;) {
+            // Runs without quotes or escapes (such as a base64 buffer) are copied in one piece.
+            size_t run = pos_;
+            while (run < n_ && src_[run] != '"' && src_[run] != '\\') run++;
+            out.append(src_, pos_, run - pos_);
+            pos_ = run;
             if (pos_ >= n_) error("unterminated string");
             const char c = src_[pos_++];
             if (c == '"') return out;

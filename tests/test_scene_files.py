@@ -580,3 +580,24 @@ def test_load_environment_png_is_linearised(tmp_path):
     assert np.abs(got - want).max() <= 1e-6
     with pytest.raises(RuntimeError, match="cannot open"):
         s.load_environment(tmp_path / "missing.hdr")
+
+
+def test_gltf_writer_writes_each_primitive_its_own_vertices(tmp_path):
+    """Meshes of interleaved materials whose vertex arrays hold unused vertices: each primitive carries
+    only the vertices its triangles use, and the triangles come back bit for bit with their materials."""
+    from photon_amd.types import Material
+    g = np.random.default_rng(13)
+    s = host.Scene()
+    mats = [s.add_material(Material.default()) for _ in range(3)]
+    p = g.uniform(-1, 1, (40, 3)).astype(np.float32)
+    idx = g.integers(0, 40, (60, 3)).astype(np.int32)
+    for t in range(60):
+        s.add_mesh(p, idx[t], mats[t % 3])
+    s.save_gltf(tmp_path / "shared.gltf")
+    b = host.Scene.from_gltf(tmp_path / "shared.gltf")
+    A, B = _raw_arrays(s), _raw_arrays(b)
+    assert b.num_tris == 60 and len(B["pos"]) == len(np.unique(A["idx"]))
+    for m in mats:
+        ta = A["pos"][A["idx"][A["mat"] == m]].reshape(-1, 9)
+        tb = B["pos"][B["idx"][B["mat"] == m]].reshape(-1, 9)
+        assert np.array_equal(ta[np.lexsort(ta.T)], tb[np.lexsort(tb.T)])
