@@ -18,6 +18,7 @@
 // nodes by tree structure only.
 #include <algorithm>
 #include <chrono>
+#include <mutex>
 #include <vector>
 
 #include "internal.h"
@@ -276,181 +277,282 @@ __global__ void swap_kernel(int* order, const int* left_pos, const int* right_po
     order[b] = t;
 }
 
-// ---- small subtrees: one block each, depth first --------------------------------------------------------
+// ---- small subtrees: one block each, one wave per node, level by level ----------------------------------
+//
+// A block builds the subtree of one root (at most kChunk triangles) breadth first: the nodes of a level are
+// shared out to the block's waves, each wave builds its node alone (boxes, bins, split, partition) and
+// appends the children to the next level's list; a block barrier separates levels. The nodes of a level
+// cover disjoint ranges of `order`, so the waves never touch the same elements. Node indices come from a
+// global counter in completion order; the flattening numbers nodes by tree structure only.
+
+constexpr int kSubWaves = 8;  // waves per subtree block
 
 struct Pending {
     int node, first, count, depth;
 };
 
-__global__ __launch_bounds__(kThreads) void subtree_kernel(const Pending* roots, int bins, int max_leaf, float traversal_cost, int* order,
-                                                            const BBox* boxes, const float4* centres, TNode* tree, int* node_counter) {
-    __shared__ BBox scratch[kThreads / 64];
-    __shared__ int red[kThreads / 64];
-    __shared__ Pending stack[128];
-    __shared__ int cnt[3][kMaxBins];
-    __shared__ int bx[3][kMaxBins][6];
-    __shared__ int lpos[kChunk / 2], rpos[kChunk / 2];
-    __shared__ int s_sp, s_split_axis, s_split_bin, s_mid;
-    __shared__ float s_lo, s_scale;
-    __shared__ BBox s_cbox;
+// Orders the calling wave's earlier LDS and global accesses before its later ones, across lanes.
+PH_DEV void wave_sync() {
+    __threadfence_block();
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Ordered fold over the wave's lanes (lane 0 first); every lane returns the total.
+PH_DEV BBox wave_fold(BBox v) {
+    for (int off = 1; off < 64; off <<= 1) {
+        BBox o;
+        for (int k = 0; k < 3; k++) {
+            o.lo[k] = __shfl_down(v.lo[k], off);
+            o.hi[k] = __shfl_down(v.hi[k], off);
+        }
+        if ((threadIdx.x & 63) + off < 64) v = fold(v, o);
+    }
+    BBox r;
+    for (int k = 0; k < 3; k++) {
+        r.lo[k] = __shfl(v.lo[k], 0);
+        r.hi[k] = __shfl(v.hi[k], 0);
+    }
+    return r;
+}
+
+// Lanes below `lane` whose flag is set.
+PH_DEV int lanes_below(uint64_t mask, int lane) { return __popcll(mask & ((uint64_t(1) << lane) - 1)); }
+
+// One node built by one wave: record, and for an inner node the partition of its range and its two
+// children appended to `next` (count in *next_count). The decisions are the host builder's; see the
+// split-search notes below.
+PH_DEV void build_node(const Pending& nd, int nb, int max_leaf, float traversal_cost, int* order, const BBox* boxes, const float4* centres,
+                       TNode* tree, int* node_counter, int* tmp, int (*cnt)[kMaxBins], int (*bx)[kMaxBins][6], Pending* next, int* next_count) {
+    const int lane = int(threadIdx.x & 63);
+    const int a = nd.first, b = nd.first + nd.count;
+    // Node box and centroid box: each lane folds a contiguous run of positions, then the lanes in order.
+    const int per = (nd.count + 63) / 64;
+    const int p0 = min(b, a + lane * per), p1 = min(b, p0 + per);
+    BBox nbox = empty_box(), cb = empty_box();
+    for (int p = p0; p < p1; p++) {
+        const int t = order[p];
+        nbox = fold(nbox, boxes[t]);
+        const float4 c = centres[t];
+        cb = fold(cb, BBox{{c.x, c.y, c.z}, {c.x, c.y, c.z}});
+    }
+    const BBox box = wave_fold(nbox), cbox = wave_fold(cb);
+    TNode rec;
+    rec.box = box;
+    rec.first = nd.first;
+    rec.count = nd.count;
+    rec.left = rec.right = -1;
+    if (nd.count <= 1 || nd.depth > 60) {
+        if (lane == 0) tree[nd.node] = rec;
+        return;
+    }
+    // Bins of the three axes (unordered LDS atomics: only differences and areas of bin boxes are used).
+    wave_sync();  // the wave's reads of its previous node's bins come first
+    for (int i = lane; i < 3 * kMaxBins; i += 64) {
+        cnt[i / kMaxBins][i % kMaxBins] = 0;
+        for (int k = 0; k < 3; k++) {
+            bx[i / kMaxBins][i % kMaxBins][k] = ordered(kBoxInf);
+            bx[i / kMaxBins][i % kMaxBins][3 + k] = ordered(-kBoxInf);
+        }
+    }
+    wave_sync();
+    float scale[3];
+    bool valid[3];
+    for (int axis = 0; axis < 3; axis++) {
+        valid[axis] = !(cbox.hi[axis] - cbox.lo[axis] < 1e-12f);
+        scale[axis] = valid[axis] ? float(nb) / (cbox.hi[axis] - cbox.lo[axis]) : 0.0f;
+    }
+    for (int p = a + lane; p < b; p += 64) {
+        const int t = order[p];
+        const float4 ce = centres[t];
+        const float cc[3] = {ce.x, ce.y, ce.z};
+        const BBox tb = boxes[t];
+        for (int axis = 0; axis < 3; axis++) {
+            if (!valid[axis]) continue;
+            const int bin = bin_of(cc[axis], cbox.lo[axis], scale[axis], nb);
+            atomicAdd(&cnt[axis][bin], 1);
+            for (int k = 0; k < 3; k++) {
+                atomicMin(&bx[axis][bin][k], ordered(tb.lo[k]));
+                atomicMax(&bx[axis][bin][3 + k], ordered(tb.hi[k]));
+            }
+        }
+    }
+    wave_sync();
+    // Split search. Lane j holds bin j; prefix folds give the left side of the split after bin j, suffix
+    // folds the right side from bin j + 1. Min/max folds are exact in any order and the counts are
+    // integers, so every candidate's cost is the value the host's sequential sweep computes; the winner
+    // is the host's first minimum in (axis, bin) order, found as the smallest (cost, candidate index)
+    // among candidates the host would accept (both sides non-empty, cost below its 1e30 start).
+    const float node_area = surface(box);
+    float best = kBoxInf;
+    int best_key = 0x7fffffff, best_left_n = 0;
+    for (int axis = 0; axis < 3; axis++) {
+        if (!valid[axis]) continue;
+        BBox v = empty_box();
+        int n = 0;
+        if (lane < nb) {
+            n = cnt[axis][lane];
+            for (int k = 0; k < 3; k++) {
+                v.lo[k] = from_ordered(bx[axis][lane][k]);
+                v.hi[k] = from_ordered(bx[axis][lane][3 + k]);
+            }
+        }
+        BBox pre = v, suf = v;
+        int pre_n = n, suf_n = n;
+        for (int off = 1; off < 64; off <<= 1) {
+            BBox up, dn;
+            for (int k = 0; k < 3; k++) {
+                up.lo[k] = __shfl_up(pre.lo[k], off);
+                up.hi[k] = __shfl_up(pre.hi[k], off);
+                dn.lo[k] = __shfl_down(suf.lo[k], off);
+                dn.hi[k] = __shfl_down(suf.hi[k], off);
+            }
+            const int up_n = __shfl_up(pre_n, off), dn_n = __shfl_down(suf_n, off);
+            if (lane >= off) {
+                pre = fold(up, pre);
+                pre_n += up_n;
+            }
+            if (lane + off < 64) {
+                suf = fold(suf, dn);
+                suf_n += dn_n;
+            }
+        }
+        // Right side of the split after bin `lane`: the suffix starting at lane + 1.
+        BBox right;
+        for (int k = 0; k < 3; k++) {
+            right.lo[k] = __shfl_down(suf.lo[k], 1);
+            right.hi[k] = __shfl_down(suf.hi[k], 1);
+        }
+        const int right_n = __shfl_down(suf_n, 1);
+        float cost = kBoxInf;
+        bool ok = lane < nb - 1 && pre_n != 0 && right_n != 0;
+        if (ok) {
+            cost = traversal_cost + (surface(pre) * float(pre_n) + surface(right) * float(right_n)) / node_area;
+            ok = cost < kBoxInf;
+        }
+        float c = ok ? cost : kBoxInf;
+        int key = ok ? axis * 64 + lane : 0x7fffffff;
+        int ln = pre_n;
+        for (int off = 32; off > 0; off >>= 1) {
+            const float oc = __shfl_xor(c, off);
+            const int ok2 = __shfl_xor(key, off), on = __shfl_xor(ln, off);
+            if (oc < c || (oc == c && ok2 < key)) {
+                c = oc;
+                key = ok2;
+                ln = on;
+            }
+        }
+        if (key != 0x7fffffff && (c < best || (c == best && key < best_key))) {
+            best = c;
+            best_key = key;
+            best_left_n = ln;
+        }
+    }
+    const int best_axis = best_key == 0x7fffffff ? -1 : best_key / 64;
+    const int best_bin = best_key == 0x7fffffff ? -1 : best_key % 64;
+    const bool forced = nd.count > max_leaf;
+    const bool leaf = best_axis < 0 ? !forced : (best >= float(nd.count) && !forced);
+    if (leaf) {
+        if (lane == 0) tree[nd.node] = rec;
+        return;
+    }
+    int mid = a + (best_axis >= 0 ? best_left_n : nd.count / 2);
+    if (best_axis >= 0) {
+        // std::partition: the k-th element on the wrong side from the front swaps with the k-th from the
+        // back. Left misplaced positions go to tmp[a + k], right ones (front to back) to tmp[b - 1 - j].
+        const float lo = cbox.lo[best_axis], sc = scale[best_axis];
+        int nl = 0, nr = 0;
+        for (int p0w = a; p0w < b; p0w += 64) {
+            const int p = p0w + lane;
+            bool lm = false, rm = false;
+            if (p < b) {
+                const float4 ce = centres[order[p]];
+                const float cc[3] = {ce.x, ce.y, ce.z};
+                const bool pred = bin_of(cc[best_axis], lo, sc, nb) <= best_bin;
+                lm = p < mid && !pred;
+                rm = p >= mid && pred;
+            }
+            const uint64_t lmask = __ballot(lm), rmask = __ballot(rm);
+            if (lm) tmp[a + nl + lanes_below(lmask, lane)] = p;
+            if (rm) tmp[b - 1 - (nr + lanes_below(rmask, lane))] = p;
+            nl += __popcll(lmask);
+            nr += __popcll(rmask);
+        }
+        wave_sync();
+        // nl == nr (both count the elements that change sides); the k-th from the back of the right list
+        // is tmp[b - nr + k].
+        const int pairs = min(nl, nr);
+        for (int k = lane; k < pairs; k += 64) {
+            const int pa = tmp[a + k], pb = tmp[b - nr + k];
+            const int t = order[pa];
+            order[pa] = order[pb];
+            order[pb] = t;
+        }
+        wave_sync();
+        if (mid == a || mid == b) mid = a + nd.count / 2;
+    }
+    if (lane == 0) {
+        const int kids = atomicAdd(node_counter, 2);
+        rec.left = kids;
+        rec.right = kids + 1;
+        tree[nd.node] = rec;
+        const int slot = atomicAdd(next_count, 2);
+        next[slot] = Pending{kids, a, mid - a, nd.depth + 1};
+        next[slot + 1] = Pending{kids + 1, mid, b - mid, nd.depth + 1};
+    }
+}
+
+// lists: per block two level lists of kChunk entries. A level of a subtree of at most kChunk triangles
+// has at most kChunk nodes (their ranges are disjoint and non-empty).
+__global__ __launch_bounds__(kSubWaves * 64) void subtree_kernel(const Pending* roots, int bins, int max_leaf, float traversal_cost, int* order,
+                                                                  const BBox* boxes, const float4* centres, TNode* tree, int* node_counter,
+                                                                  Pending* lists, int* tmp) {
+    __shared__ int cnt[kSubWaves][3][kMaxBins];
+    __shared__ int bx[kSubWaves][3][kMaxBins][6];
+    __shared__ int s_count[2];
+    const int wave = int(threadIdx.x >> 6);
+    Pending* const lv = lists + size_t(blockIdx.x) * 2 * kChunk;  // level lists at lv and lv + kChunk
     if (threadIdx.x == 0) {
-        stack[0] = roots[blockIdx.x];
-        s_sp = 1;
+        lv[0] = roots[blockIdx.x];
+        s_count[0] = 1;
+        s_count[1] = 0;
     }
     __syncthreads();
-    const int nb = bins;
+    int cur = 0;
     while (true) {
+        const int n = s_count[cur];
+        if (n == 0) break;  // every thread reads the same count: the loop ends for the whole block at once
+        for (int i = wave; i < n; i += kSubWaves)
+            build_node(lv[cur * kChunk + i], bins, max_leaf, traversal_cost, order, boxes, centres, tree, node_counter, tmp, cnt[wave], bx[wave],
+                       lv + (cur ^ 1) * kChunk, &s_count[cur ^ 1]);
         __syncthreads();
-        if (s_sp == 0) break;
-        const Pending nd = stack[s_sp - 1];
+        if (threadIdx.x == 0) s_count[cur] = 0;
+        cur ^= 1;
         __syncthreads();
-        if (threadIdx.x == 0) s_sp--;
-        const int a = nd.first, b = nd.first + nd.count;
-        BBox box, cbox;
-        range_boxes(order, boxes, centres, a, b, scratch, box, cbox);
-        if (threadIdx.x == 0) {
-            TNode t;
-            t.box = box;
-            t.first = nd.first;
-            t.count = nd.count;
-            t.left = t.right = -1;
-            tree[nd.node] = t;
-            s_cbox = cbox;
-        }
-        if (nd.count <= 1 || nd.depth > 60) continue;
-        // Bins of the three axes.
-        for (int i = threadIdx.x; i < 3 * kMaxBins; i += kThreads) {
-            cnt[i / kMaxBins][i % kMaxBins] = 0;
-            for (int k = 0; k < 3; k++) {
-                bx[i / kMaxBins][i % kMaxBins][k] = ordered(kBoxInf);
-                bx[i / kMaxBins][i % kMaxBins][3 + k] = ordered(-kBoxInf);
-            }
-        }
-        __syncthreads();
-        for (int p = a + threadIdx.x; p < b; p += kThreads) {
-            const int t = order[p];
-            const float4 ce = centres[t];
-            const float cc[3] = {ce.x, ce.y, ce.z};
-            const BBox tb = boxes[t];
-            for (int axis = 0; axis < 3; axis++) {
-                const float lo = cbox.lo[axis], hi = cbox.hi[axis];
-                if (hi - lo < 1e-12f) continue;
-                const int bin = bin_of(cc[axis], lo, float(nb) / (hi - lo), nb);
-                atomicAdd(&cnt[axis][bin], 1);
-                for (int k = 0; k < 3; k++) {
-                    atomicMin(&bx[axis][bin][k], ordered(tb.lo[k]));
-                    atomicMax(&bx[axis][bin][3 + k], ordered(tb.hi[k]));
-                }
-            }
-        }
-        __syncthreads();
-        // The split search and leaf test, as the host builder runs them.
-        if (threadIdx.x == 0) {
-            float best = kBoxInf;
-            int best_axis = -1, best_bin = -1;
-            const float leaf_cost = float(nd.count);
-            const float node_area = surface(box);
-            for (int axis = 0; axis < 3; axis++) {
-                const float lo = s_cbox.lo[axis], hi = s_cbox.hi[axis];
-                if (hi - lo < 1e-12f) continue;
-                float suffix_area[kMaxBins];
-                int suffix_n[kMaxBins];
-                BBox acc = empty_box();
-                int n = 0;
-                for (int bb = nb - 1; bb > 0; bb--) {
-                    BBox bin_box;
-                    for (int k = 0; k < 3; k++) {
-                        bin_box.lo[k] = from_ordered(bx[axis][bb][k]);
-                        bin_box.hi[k] = from_ordered(bx[axis][bb][3 + k]);
-                    }
-                    acc = fold(acc, bin_box);
-                    n += cnt[axis][bb];
-                    suffix_area[bb] = surface(acc);
-                    suffix_n[bb] = n;
-                }
-                acc = empty_box();
-                n = 0;
-                for (int bb = 0; bb < nb - 1; bb++) {
-                    BBox bin_box;
-                    for (int k = 0; k < 3; k++) {
-                        bin_box.lo[k] = from_ordered(bx[axis][bb][k]);
-                        bin_box.hi[k] = from_ordered(bx[axis][bb][3 + k]);
-                    }
-                    acc = fold(acc, bin_box);
-                    n += cnt[axis][bb];
-                    const int nr = suffix_n[bb + 1];
-                    if (n == 0 || nr == 0) continue;
-                    const float cost = traversal_cost + (surface(acc) * float(n) + suffix_area[bb + 1] * float(nr)) / node_area;
-                    if (cost < best) {
-                        best = cost;
-                        best_axis = axis;
-                        best_bin = bb;
-                    }
-                }
-            }
-            const bool forced = nd.count > max_leaf;
-            bool leaf = false;
-            if (best_axis < 0) {
-                leaf = !forced;
-            } else if (best >= leaf_cost && !forced) {
-                leaf = true;
-            }
-            s_split_axis = leaf ? -2 : best_axis;
-            s_split_bin = best_bin;
-            int left_n = nd.count / 2;
-            if (!leaf && best_axis >= 0) {
-                left_n = 0;
-                for (int bb = 0; bb <= best_bin; bb++) left_n += cnt[best_axis][bb];
-                s_lo = s_cbox.lo[best_axis];
-                s_scale = float(nb) / (s_cbox.hi[best_axis] - s_cbox.lo[best_axis]);
-            }
-            s_mid = a + left_n;
-        }
-        __syncthreads();
-        const int axis = s_split_axis;
-        if (axis == -2) continue;
-        int mid = s_mid;
-        if (axis >= 0) {
-            // std::partition: the k-th wrong-side element from the front swaps with the k-th from the back.
-            int lbase = 0, rbase = 0;
-            for (int p0 = a; p0 < b; p0 += kThreads) {
-                const int p = p0 + int(threadIdx.x);
-                bool lm = false, rm = false;
-                if (p < b) {
-                    const float4 ce = centres[order[p]];
-                    const float cc[3] = {ce.x, ce.y, ce.z};
-                    const bool pred = bin_of(cc[axis], s_lo, s_scale, nb) <= s_split_bin;
-                    lm = p < mid && !pred;
-                    rm = p >= mid && pred;
-                }
-                int le, re;
-                const int lt = block_scan(lm ? 1 : 0, red, le);
-                const int rt = block_scan(rm ? 1 : 0, red, re);
-                if (lm) lpos[lbase + le] = p;
-                if (rm) rpos[rbase + re] = p;
-                lbase += lt;
-                rbase += rt;
-            }
-            __syncthreads();
-            // rpos holds the right ones front to back: its k-th from the back is rpos[rbase - 1 - k].
-            for (int k = threadIdx.x; k < lbase; k += kThreads) {
-                const int pa = lpos[k], pb = rpos[rbase - 1 - k];
-                const int t = order[pa];
-                order[pa] = order[pb];
-                order[pb] = t;
-            }
-            __syncthreads();
-            if (mid == a || mid == b) mid = a + nd.count / 2;
-        }
-        if (threadIdx.x == 0) {
-            const int kids = atomicAdd(node_counter, 2);
-            tree[nd.node].left = kids;
-            tree[nd.node].right = kids + 1;
-            stack[s_sp] = Pending{kids + 1, mid, b - mid, nd.depth + 1};
-            stack[s_sp + 1] = Pending{kids, a, mid - a, nd.depth + 1};
-            s_sp += 2;
-        }
     }
+}
+
+// Device buffers that only grow: per-level arrays keep their allocation across levels (a smaller request
+// reuses it), so the build does not free and reallocate, and hipFree's device synchronisation is avoided.
+template <class T>
+void grow(DevBuf<T>& b, size_t n) {
+    if (!b.ptr || b.count < n) b.allocate(std::max<size_t>(n, 1));
+}
+template <class T>
+void put(DevBuf<T>& b, const T* src, size_t n, hipStream_t st) {
+    grow(b, n);
+    if (n) PH_CHECK(hipMemcpyAsync(b.ptr, src, n * sizeof(T), hipMemcpyHostToDevice, st));
+}
+
+// One stream per device for the build, created on first use and kept for the process.
+hipStream_t build_stream() {
+    static std::mutex mu;
+    static std::vector<hipStream_t> streams;
+    int dev = 0;
+    PH_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    if (size_t(dev) >= streams.size()) streams.resize(size_t(dev) + 1, nullptr);
+    if (!streams[size_t(dev)]) PH_CHECK(hipStreamCreateWithFlags(&streams[size_t(dev)], hipStreamNonBlocking));
+    return streams[size_t(dev)];
 }
 
 }  // namespace
@@ -476,12 +578,7 @@ int64_t bvh_build_device(const float* positions, int num_vertices, const int* in
         stats.nodes = 1;
         return 1;
     }
-    hipStream_t st;
-    PH_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    struct StreamGuard {
-        hipStream_t s;
-        ~StreamGuard() { (void)hipStreamDestroy(s); }
-    } guard{st};
+    hipStream_t st = build_stream();
     DevBuf<float> d_pos;
     DevBuf<int> d_idx, d_order, d_counter;
     DevBuf<BBox> d_boxes;
@@ -524,8 +621,8 @@ int64_t bvh_build_device(const float* positions, int num_vertices, const int* in
         }
         chunk_first[level.size()] = int(chunks.size());
         const int T = int(chunks.size());
-        d_chunks.upload(chunks.data(), chunks.size(), st);
-        d_chunk_boxes.allocate(size_t(2) * size_t(T));
+        put(d_chunks, chunks.data(), chunks.size(), st);
+        grow(d_chunk_boxes, size_t(2) * size_t(T));
         hipLaunchKernelGGL(chunk_boxes_kernel, dim3(T), dim3(kThreads), 0, st, d_chunks.ptr, d_order.ptr, d_boxes.ptr, d_centres.ptr, d_chunk_boxes.ptr);
         PH_CHECK(hipGetLastError());
         std::vector<BBox> cboxes(size_t(2) * size_t(T));
@@ -551,8 +648,8 @@ int64_t bvh_build_device(const float* positions, int num_vertices, const int* in
             }
         }
         // Large nodes are above max_leaf and depth 60 is far: all split unless no axis has extent.
-        d_params.upload(params.data(), params.size(), st);
-        d_bins.allocate(size_t(T) * 3 * kMaxBins * 7);
+        put(d_params, params.data(), params.size(), st);
+        grow(d_bins, size_t(T) * 3 * kMaxBins * 7);
         hipLaunchKernelGGL(chunk_bins_kernel, dim3(T), dim3(kThreads), 0, st, d_chunks.ptr, d_params.ptr, d_order.ptr, d_boxes.ptr, d_centres.ptr,
                            d_bins.ptr);
         PH_CHECK(hipGetLastError());
@@ -646,8 +743,8 @@ int64_t bvh_build_device(const float* positions, int num_vertices, const int* in
             big_records.push_back({nd.node, rec});
         }
         if (!splitting.empty()) {
-            d_splits.upload(splits.data(), splits.size(), st);
-            d_counts.allocate(size_t(2) * size_t(T));
+            put(d_splits, splits.data(), splits.size(), st);
+            grow(d_counts, size_t(2) * size_t(T));
             hipLaunchKernelGGL(chunk_misplaced_kernel, dim3(T), dim3(kThreads), 0, st, d_chunks.ptr, d_splits.ptr, d_order.ptr, d_centres.ptr, d_counts.ptr);
             PH_CHECK(hipGetLastError());
             std::vector<int> counts(size_t(2) * size_t(T));
@@ -680,10 +777,10 @@ int64_t bvh_build_device(const float* positions, int num_vertices, const int* in
                     sel.push_back(chunks[size_t(c)]);
                     sel_base.insert(sel_base.end(), {base[size_t(3 * c)], base[size_t(3 * c + 1)], base[size_t(3 * c + 2)]});
                 }
-                d_chunks.upload(sel.data(), sel.size(), st);
-                d_base.upload(sel_base.data(), sel_base.size(), st);
-                d_lpos.allocate(size_t(offset));
-                d_rpos.allocate(size_t(offset));
+                put(d_chunks, sel.data(), sel.size(), st);
+                put(d_base, sel_base.data(), sel_base.size(), st);
+                grow(d_lpos, size_t(offset));
+                grow(d_rpos, size_t(offset));
                 hipLaunchKernelGGL(chunk_ranks_kernel, dim3(unsigned(sel.size())), dim3(kThreads), 0, st, d_chunks.ptr, d_splits.ptr, d_order.ptr,
                                    d_centres.ptr, d_base.ptr, d_lpos.ptr, d_rpos.ptr);
                 PH_CHECK(hipGetLastError());
@@ -694,14 +791,21 @@ int64_t bvh_build_device(const float* positions, int num_vertices, const int* in
         level.swap(next);
     }
     // Large-node records, then every small subtree.
-    for (auto& r : big_records) PH_CHECK(hipMemcpyAsync(d_tree.ptr + r.first, &r.second, sizeof(TNode), hipMemcpyHostToDevice, st));
+    // Indices [0, next_node) hold the large nodes and the subtree roots; the roots' records are written
+    // by the subtree kernel, so one copy of the whole prefix places the large-node records.
+    std::vector<TNode> prefix(size_t(next_node), TNode{empty_box(), 0, 0, -1, -1});
+    for (auto& r : big_records) prefix[size_t(r.first)] = r.second;
+    PH_CHECK(hipMemcpyAsync(d_tree.ptr, prefix.data(), prefix.size() * sizeof(TNode), hipMemcpyHostToDevice, st));
     PH_CHECK(hipStreamSynchronize(st));
     if (!small.empty()) {
         PH_CHECK(hipMemcpyAsync(d_counter.ptr, &next_node, sizeof(int), hipMemcpyHostToDevice, st));
-        DevBuf<Pending> d_small;
+        DevBuf<Pending> d_small, d_lists;
+        DevBuf<int> d_tmp;
         d_small.upload(small.data(), small.size(), st);
-        hipLaunchKernelGGL(subtree_kernel, dim3(unsigned(small.size())), dim3(kThreads), 0, st, d_small.ptr, bins, max_leaf, traversal_cost, d_order.ptr,
-                           d_boxes.ptr, d_centres.ptr, d_tree.ptr, d_counter.ptr);
+        d_lists.allocate(small.size() * 2 * size_t(kChunk));
+        d_tmp.allocate(size_t(n));
+        hipLaunchKernelGGL(subtree_kernel, dim3(unsigned(small.size())), dim3(kSubWaves * 64), 0, st, d_small.ptr, bins, max_leaf, traversal_cost,
+                           d_order.ptr, d_boxes.ptr, d_centres.ptr, d_tree.ptr, d_counter.ptr, d_lists.ptr, d_tmp.ptr);
         PH_CHECK(hipGetLastError());
         PH_CHECK(hipMemcpyAsync(&next_node, d_counter.ptr, sizeof(int), hipMemcpyDeviceToHost, st));
         PH_CHECK(hipStreamSynchronize(st));
