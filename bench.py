@@ -382,9 +382,10 @@ def bench_serving(args, gpu, host, field, launch):
     return res
 
 
-def measured_traffic(spp):
+def measured_traffic(spp, launch):
     """HBM bytes per closest-hit launch from the committed rocprofv3 counter summary of this workload
-    (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the copy-kernel calibration), or None."""
+    (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the copy-kernel calibration), or None when the
+    summary was taken at another spp or launch configuration (rays per launch, and so traffic, differ)."""
     path = os.path.join(ROOT, "profiles", "extend_kernel_field_1080p_pmc.md")
     if not os.path.exists(path):
         return None
@@ -395,6 +396,8 @@ def measured_traffic(spp):
             vals[k.strip()] = v.strip().split()[0]
     try:
         if int(vals.get("spp", -1)) != spp:
+            return None
+        if launch is None or vals.get("launch") != ",".join(f"{k}={v}" for k, v in launch.items()):
             return None
         return int(float(vals["traffic_bytes_per_launch"]))
     except (KeyError, ValueError):
@@ -438,6 +441,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-svgf", action="store_true")
     ap.add_argument("--no-scene-build", action="store_true", help="skip the glTF import and BVH build leg")
+    ap.add_argument("--scene-build-only", action="store_true", help="run only the glTF import and BVH build leg")
     ap.add_argument("--svgf-frames", type=int, default=64)
     ap.add_argument("--serve-jobs", type=int, default=200, help="jobs of the open-loop serving run (the full stream is 10,000)")
     ap.add_argument("--serve-burst", type=int, default=48)
@@ -460,6 +464,9 @@ def main():
     from photon_amd import gpu, host
     from photon_amd.types import RenderSettings
     assert gpu.gpu_available(), "no HIP device"
+    if args.scene_build_only:
+        print(json.dumps({"scene_build": bench_scene_build(gpu, host)}), flush=True)
+        return
     info = gpu.device_info(0)
     log(f"device {info.name.decode()} {info.arch.decode()} CUs={info.compute_units}")
     if args.calibrate_copy_mb > 0:
@@ -492,7 +499,7 @@ def main():
         "config": {"workload": "wavefront path trace, 1920x1080, 64 spp, max depth 16, 1,048,578-triangle sphere field, 2048x1024 HDR sky",
                    "engine": "wavefront", "launch": rnd["config"], "spp": args.spp},
         "roofline": {"bound": "hbm", "kernel": "extend_kernel (closest-hit traversal)", "achieved": round(rnd["achieved_gbs"], 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(rnd["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.spp),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(rnd["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.spp, rnd["config"]),
                      "bytes_per_launch": int(rnd["extend_bytes_per_launch"]), "avg_launch_ms": round(rnd["extend_avg_launch_ms"], 4)},
         "cpu_baseline": cpu,
         "mrays_s": round(rnd["mrays_s"], 1),

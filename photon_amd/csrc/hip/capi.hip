@@ -534,8 +534,8 @@ int ph_bvh_build(const float* positions, int num_vertices, const int32_t* indice
     if (max_leaf < 0) return fail("negative leaf size");
     if (node_capacity < std::max(1, 2 * num_tris - 1)) return fail("node array too small: it needs room for max(1, 2 num_tris - 1) nodes");
     if (num_tris > PH_BVH_BUILD_MAX_TRIS)
-        return fail("device BVH build is limited to PH_BVH_BUILD_MAX_TRIS (262144) triangles: a 1M-triangle build made a GPU memory "
-                    "fault whose cause is not found yet, so larger builds are switched off; build larger scenes on the host");
+        return fail("device BVH build is limited to PH_BVH_BUILD_MAX_TRIS (262144) triangles: a 1M-triangle build with an earlier subtree "
+                    "kernel made a GPU memory fault whose cause was not found, so larger builds stay switched off; build larger scenes on the host");
     // Every index is read on the device: reject any outside the vertex array here.
     for (int64_t i = 0; i < int64_t(num_tris) * 3; i++)
         if (indices[i] < 0 || indices[i] >= num_vertices) return fail("vertex index out of range");

@@ -1166,9 +1166,11 @@ ph_wavefront_config WavefrontEngine::autotune(hipStream_t st, const WavefrontJob
     };
     ph_wavefront_config best = m.cfg;
     double best_ms = time_of(best);
+    // A candidate replaces the current best only when it is faster by more than the run-to-run spread of
+    // one render (about 1%), so that near-ties keep the earlier choice and repeated tunings agree.
     auto consider = [&](const ph_wavefront_config& c) {
         const double t = time_of(c);
-        if (t < best_ms) best_ms = t, best = c;
+        if (t < best_ms * 0.98) best_ms = t, best = c;
     };
     // Coordinate descent, largest effects first. Sub-streams per pixel (the caller's spp decides which
     // divide it) and paths in flight: with sub-streams there are more work items than pixels, more slots

@@ -48,9 +48,9 @@ def test_sphere_soup_and_field():
     _compare(host.Scene.field(64, seed=3, env_width=64))
 
 
-@pytest.mark.skip(reason="a 1M-triangle device build made a GPU memory fault whose cause is not found yet (a CPU emulation of "
-                         "the build keeps every index in bounds on this scene); ph_bvh_build refuses inputs above "
-                         "PH_BVH_BUILD_MAX_TRIS = 262144 triangles until it is")
+@pytest.mark.skip(reason="a 1M-triangle device build with an earlier subtree kernel made a GPU memory fault whose cause was not "
+                         "found (a CPU emulation of that build keeps every index in bounds on this scene); ph_bvh_build refuses "
+                         "inputs above PH_BVH_BUILD_MAX_TRIS = 262144 triangles and this case is not run until a cause is shown")
 def test_million_triangle_field():
     _compare(host.Scene.field(1024, seed=1, env_width=64))
 
