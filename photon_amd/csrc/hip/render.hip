@@ -12,6 +12,12 @@ namespace {
 
 constexpr int kBX = 16, kBY = 8, kPixelBlock = kBX * kBY;
 constexpr int kMegaDepth = 32;  // LDS stack entries per lane in the per-pixel kernels
+// Waves per SIMD the one-sample G-buffer kernels are compiled for. The whole integrator is live in one
+// kernel, and left alone the compiler gives it registers for 2 waves per SIMD (228 VGPRs), too few to
+// hide the traversal's memory latency. At 4 (128 VGPRs, the rest spilled) a 1600x900 frame of the
+// 1M-triangle field takes 3.96 ms instead of 4.96 ms, with bit-identical output; 3 gave 4.08 ms and 6
+// gave 4.88 ms. The per-pixel render kernel, the reference's structure, is left as the compiler builds it.
+constexpr int kSampleWaves = 4;
 
 __global__ __launch_bounds__(kPixelBlock) void render_kernel(DScene s, ph_camera_view cam, int w, int h, uint32_t frame0, int spp, uint32_t seed,
                                                              PathOpts opt, float4* accum, float4* albedo, float4* nd) {
@@ -39,7 +45,7 @@ __global__ __launch_bounds__(kPixelBlock) void render_kernel(DScene s, ph_camera
     accum[i] = a;
 }
 
-__global__ __launch_bounds__(kPixelBlock) void sample_kernel(DScene s, ph_camera_view cam, int w, int h, uint32_t frame, uint32_t seed, PathOpts opt,
+__global__ __launch_bounds__(kPixelBlock) __attribute__((amdgpu_waves_per_eu(kSampleWaves))) void sample_kernel(DScene s, ph_camera_view cam, int w, int h, uint32_t frame, uint32_t seed, PathOpts opt,
                                                              float4* color, float4* albedo, float4* nd, float4* position, float4* emission) {
     __shared__ int lds[kMegaDepth * kPixelBlock];
     const int tid = threadIdx.y * kBX + threadIdx.x;
@@ -57,7 +63,7 @@ __global__ __launch_bounds__(kPixelBlock) void sample_kernel(DScene s, ph_camera
 }
 
 // sample_kernel's sample, written as the compact G-buffer.
-__global__ __launch_bounds__(kPixelBlock) void sample_compact_kernel(DScene s, ph_camera_view cam, int w, int h, uint32_t frame, uint32_t seed,
+__global__ __launch_bounds__(kPixelBlock) __attribute__((amdgpu_waves_per_eu(kSampleWaves))) void sample_compact_kernel(DScene s, ph_camera_view cam, int w, int h, uint32_t frame, uint32_t seed,
                                                                      PathOpts opt, float* color, float* albedo, float* emission, float* depth,
                                                                      uint32_t* normal, int* material) {
     __shared__ int lds[kMegaDepth * kPixelBlock];

@@ -932,6 +932,9 @@ struct WavefrontEngine::Impl {
     template <bool Multi>
     void iteration(hipStream_t st, const ShapeFns& fns, bool count, hipEvent_t* ev) {
         const int tgrid = cus * std::max(1, cfg.blocks_per_cu);
+        // Shading: 4 blocks of 256 threads per CU, which is what the shading kernels' registers allow (116
+        // VGPRs, 4 waves per SIMD). Compiling them for 6 or 8 waves (with a grid to match) spills and was
+        // slower: 940 and 897 against 995 Msamples/s.
         const int sgrid = cus * 4;
         const State s = state();
         const Work w = work();
