@@ -12,14 +12,15 @@ namespace {
 
 constexpr int kBX = 16, kBY = 8, kPixelBlock = kBX * kBY;
 constexpr int kMegaDepth = 32;  // LDS stack entries per lane in the per-pixel kernels
-// Waves per SIMD the one-sample G-buffer kernels are compiled for. The whole integrator is live in one
-// kernel, and left alone the compiler gives it registers for 2 waves per SIMD (228 VGPRs), too few to
-// hide the traversal's memory latency. At 4 (128 VGPRs, the rest spilled) a 1600x900 frame of the
-// 1M-triangle field takes 3.96 ms instead of 4.96 ms, with bit-identical output; 3 gave 4.08 ms and 6
-// gave 4.88 ms. The per-pixel render kernel, the reference's structure, is left as the compiler builds it.
+// Waves per SIMD the per-pixel kernels are compiled for. The whole integrator is live in one kernel,
+// and left alone the compiler gives it registers for 2 waves per SIMD (228-246 VGPRs), too few to hide
+// the traversal's memory latency. At 4 (128 VGPRs, the rest spilled), with bit-identical output:
+//   * a 1600x900 one-sample G-buffer frame of the 1M-triangle field takes 3.96 ms instead of 4.96 ms
+//     (3 waves: 4.08 ms, 6 waves: 4.88 ms);
+//   * the 1080p render of that field runs at 479 Msamples/s instead of 329 (3 waves: 416).
 constexpr int kSampleWaves = 4;
 
-__global__ __launch_bounds__(kPixelBlock) void render_kernel(DScene s, ph_camera_view cam, int w, int h, uint32_t frame0, int spp, uint32_t seed,
+__global__ __launch_bounds__(kPixelBlock) __attribute__((amdgpu_waves_per_eu(kSampleWaves))) void render_kernel(DScene s, ph_camera_view cam, int w, int h, uint32_t frame0, int spp, uint32_t seed,
                                                              PathOpts opt, float4* accum, float4* albedo, float4* nd) {
     __shared__ int lds[kMegaDepth * kPixelBlock];
     const int tid = threadIdx.y * kBX + threadIdx.x;
