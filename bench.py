@@ -130,8 +130,34 @@ def bench_render(args, gpu, host, RenderSettings):
         out["megakernel_spp_timed"] = mk_spp
         out["speedup_vs_megakernel"] = value / out["megakernel_msamples_s"]
         log(f"megakernel: {out['megakernel_msamples_s']:.1f} Msamples/s ({mk_spp} spp in {mk:.2f} s)")
+    out["display"] = bench_display(gpu, r, rs, W, H)
     r.close()
     return scene, cam, rs, out
+
+
+def bench_display(gpu, r, rs, W, H, reps=20):
+    """The display path at 1080p: tonemap into the renderer's display buffer and copy it to the host
+    (download_display, the reference's viewer path without GL interop), against tonemapping straight into
+    pinned host memory (ph_host_alloc as the display target)."""
+    import numpy as np
+    pinned = gpu.PinnedArray(W * H * 4)
+    r.display(rs)
+    r.download_display()
+    r.display(rs, target=pinned.ptr)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        r.display(rs)
+        img = r.download_display()
+    copy_ms = (time.perf_counter() - t0) / reps * 1e3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        r.display(rs, target=pinned.ptr)
+    pinned_ms = (time.perf_counter() - t0) / reps * 1e3
+    same = bool(np.array_equal(pinned.numpy(np.uint32, (H, W)), img))
+    pinned.free()
+    res = dict(width=W, height=H, display_and_copy_ms=round(copy_ms, 3), display_to_pinned_host_ms=round(pinned_ms, 3), identical=same)
+    log(f"display: {res}")
+    return res
 
 
 def bench_traversal(gpu, host):
@@ -506,6 +532,7 @@ def main():
         "rays_per_sample": round(rnd["rays_per_sample"], 3),
         "node_visits_per_ray": round(rnd["node_visits_per_ray"], 2),
         "tri_tests_per_ray": round(rnd["tri_tests_per_ray"], 2),
+        "display": rnd.get("display"),
         "megakernel_msamples_s": round(rnd.get("megakernel_msamples_s", 0.0), 3) or None,
         "speedup_vs_megakernel": round(rnd.get("speedup_vs_megakernel", 0.0), 3) or None,
         "traversal": trav,

@@ -147,6 +147,12 @@ int ph_copy_to_device(void* dst_dev, const void* src, int64_t bytes);
 int ph_copy_to_host(void* dst, const void* src_dev, int64_t bytes);
 int ph_copy_device(void* dst_dev, const void* src_dev, int64_t bytes);
 int ph_device_memset(void* dev, int value, int64_t bytes);
+/* Pinned host memory (hipHostMalloc), which kernels can write directly: ph_renderer_display may take
+ * such a buffer as its target, so that the tonemapped frame lands in host memory without a separate
+ * device-to-host copy (the reference's viewer copies the display buffer to the host each frame, or
+ * registers it with GL interop: gpu_renderer.cu:129-175). No counterpart in the reference. */
+int ph_host_alloc(int64_t bytes, void** out_host);
+int ph_host_free(void* host);
 
 typedef struct ph_renderer ph_renderer;
 int ph_renderer_create(ph_renderer** out);
@@ -158,6 +164,7 @@ int ph_renderer_reset(ph_renderer* r);
 int ph_renderer_render(ph_renderer* r, const ph_camera_view* camera, const ph_render_settings* settings, int spp);
 int ph_renderer_render_denoised(ph_renderer* r, const ph_camera_view* camera, const ph_camera_view* previous, const ph_render_settings* settings,
                                 const ph_svgf_settings* svgf);
+/* target: NULL (the renderer's display buffer), caller device memory, or pinned host memory from ph_host_alloc. */
 int ph_renderer_display(ph_renderer* r, const ph_render_settings* settings, int mode, uint32_t* target_dev);
 int ph_renderer_download_image(ph_renderer* r, float* rgb);
 int ph_renderer_download_display(ph_renderer* r, uint32_t* rgba);

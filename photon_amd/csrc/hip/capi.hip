@@ -157,6 +157,22 @@ int ph_device_free(void* p) {
     });
 }
 
+int ph_host_alloc(int64_t bytes, void** out) {
+    if (!out || bytes < 0) return fail("invalid argument");
+    return guarded([&] {
+        *out = nullptr;
+        if (bytes > 0) PH_CHECK(hipHostMalloc(out, size_t(bytes), hipHostMallocDefault));
+        return 0;
+    });
+}
+
+int ph_host_free(void* p) {
+    return guarded([&] {
+        if (p) PH_CHECK(hipHostFree(p));
+        return 0;
+    });
+}
+
 int ph_copy_to_device(void* dst, const void* src, int64_t bytes) {
     if (bytes < 0 || (bytes > 0 && (!dst || !src))) return fail("invalid argument");
     return guarded([&] {

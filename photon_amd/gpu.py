@@ -135,6 +135,8 @@ def lib():
         L.ph_svgf_last_ms.restype = C.c_float
         L.ph_device_alloc.argtypes = [C.c_int64, C.POINTER(P)]
         L.ph_device_free.argtypes = [P]
+        L.ph_host_alloc.argtypes = [C.c_int64, C.POINTER(P)]
+        L.ph_host_free.argtypes = [P]
         L.ph_copy_to_device.argtypes = [P, P, C.c_int64]
         L.ph_copy_to_host.argtypes = [P, P, C.c_int64]
         L.ph_device_memset.argtypes = [P, C.c_int, C.c_int64]
@@ -154,6 +156,33 @@ def lib():
         L.ph_server_now_ms.restype = C.c_double
         _sig_done = True
     return L
+
+
+class PinnedArray:
+    """Pinned host memory allocated by the library (ph_host_alloc): kernels write it directly, and numpy
+    reads it in place."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        self.ptr = C.c_void_p()
+        _check(lib().ph_host_alloc(self.nbytes, C.byref(self.ptr)))
+
+    def numpy(self, dtype, shape=None):
+        dtype = np.dtype(dtype)
+        buf = (C.c_uint8 * self.nbytes).from_address(self.ptr.value)
+        out = np.frombuffer(buf, dtype=dtype, count=self.nbytes // dtype.itemsize)
+        return out if shape is None else out.reshape(shape)
+
+    def free(self):
+        if self.ptr:
+            lib().ph_host_free(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class DeviceArray:
