@@ -109,7 +109,7 @@ int phs_scene_raw(phs_scene* s, phs_raw_scene* out);
 /* ---- scene files and images ---------------------------------------------------------------------- */
 
 /* Appends a glTF 2.0 file (.gltf or .glb) to the scene: meshes of the default scene under the node
- * hierarchy, PBR materials with the transmission / ior / emissive-strength extensions, PNG textures,
+ * hierarchy, PBR materials with the transmission / ior / emissive-strength extensions, PNG and JPEG textures,
  * perspective cameras and KHR_lights_punctual lights. The first camera becomes the scene camera.
  * <- load_gltf (src/scene/gltf.h:12, gltf.cpp:72-497). */
 int phs_scene_load_gltf(phs_scene* s, const char* path);
@@ -123,8 +123,8 @@ int phs_scene_camera_at(const phs_scene* s, int index, float position[3], float 
  * <- load_environment (src/scene/image.h:18, image.cpp:53-81). */
 int phs_scene_load_environment(phs_scene* s, const char* path, float intensity, float rotation);
 
-/* Decodes a PNG to RGBA8 (R in the low byte, row 0 at the top). With rgba NULL only the size is
- * returned. <- decode_image (src/scene/image.h:14, image.cpp:23-37). */
+/* Decodes a PNG or a baseline JPEG to RGBA8 (R in the low byte, row 0 at the top). With rgba NULL only
+ * the size is returned. <- decode_image (src/scene/image.h:14, image.cpp:23-37). */
 int phs_image_decode(const uint8_t* data, size_t size, int* width, int* height, uint32_t* rgba, size_t capacity);
 /* <- save_png (src/scene/image.h:20, image.cpp:84-86). */
 int phs_image_save_png(const char* path, int width, int height, const uint32_t* rgba);

@@ -1,4 +1,5 @@
-// Image codecs of the host library (image_io.hpp): PNG over zlib, Radiance RGBE, PFM.
+// Image codecs of the host library (image_io.hpp): PNG over zlib, Radiance RGBE, PFM (baseline JPEG is in
+// image_jpeg.cpp).
 //
 // Behaviour kept from the reference (src/scene/image.cpp:23-116, which calls stb_image):
 //   * decoded textures are RGBA8 with row 0 at the top; 16-bit PNG samples keep their high byte;
@@ -353,11 +354,8 @@ void put_chunk(std::vector<uint8_t>& o, const char* type, const uint8_t* body, s
 
 bool decode_image(const uint8_t* data, size_t size, Texture& out, std::string& error) {
     if (size >= 8 && data[0] == 0x89 && data[1] == 'P') return decode_png(data, size, out, error);
-    if (size >= 3 && data[0] == 0xFF && data[1] == 0xD8 && data[2] == 0xFF) {
-        error = "JPEG images are not supported (no JPEG decoder in this build)";
-        return false;
-    }
-    error = "unsupported image format (PNG only)";
+    if (size >= 3 && data[0] == 0xFF && data[1] == 0xD8 && data[2] == 0xFF) return decode_jpeg(data, size, out, error);
+    error = "unsupported image format (PNG and baseline JPEG only)";
     return false;
 }
 

@@ -303,7 +303,9 @@ def test_gltf_textures_samplers_and_undecodable_images(tmp_path, capfd):
     png = io.BytesIO()
     Image.fromarray(img, "RGBA").save(png, "PNG")
     jpg = io.BytesIO()
-    Image.fromarray(img[..., :3], "RGB").save(jpg, "JPEG")
+    Image.fromarray(img[..., :3], "RGB").save(jpg, "JPEG", quality=95)
+    progressive = io.BytesIO()
+    Image.fromarray(img[..., :3], "RGB").save(progressive, "JPEG", progressive=True)
     pos = np.array([0, 0, 0, 1, 0, 0, 0, 1, 0], np.float32).tobytes()
     uv = np.array([0, 0, 1, 0, 0, 1], np.float32).tobytes()
     doc = _doc_with_buffer(pos + uv, {
@@ -311,24 +313,31 @@ def test_gltf_textures_samplers_and_undecodable_images(tmp_path, capfd):
         "accessors": [{"bufferView": 0, "componentType": 5126, "count": 3, "type": "VEC3"},
                       {"bufferView": 1, "componentType": 5126, "count": 3, "type": "VEC2"}],
         "images": [{"uri": "data:image/png;base64," + base64.b64encode(png.getvalue()).decode()},
-                   {"uri": "data:image/jpeg;base64," + base64.b64encode(jpg.getvalue()).decode()}],
+                   {"uri": "data:image/jpeg;base64," + base64.b64encode(jpg.getvalue()).decode()},
+                   {"uri": "data:image/jpeg;base64," + base64.b64encode(progressive.getvalue()).decode()}],
         "samplers": [{"wrapS": 33071, "wrapT": 33071}],
-        "textures": [{"source": 0, "sampler": 0}, {"source": 0}, {"source": 1}],
+        "textures": [{"source": 0, "sampler": 0}, {"source": 0}, {"source": 1}, {"source": 2}],
         "materials": [{"pbrMetallicRoughness": {"baseColorTexture": {"index": 0}, "metallicRoughnessTexture": {"index": 0}},
                        "emissiveTexture": {"index": 0}, "normalTexture": {"index": 1, "scale": 0.5},
                        "alphaMode": "MASK", "alphaCutoff": 0.25, "doubleSided": True},
-                      {"pbrMetallicRoughness": {"baseColorTexture": {"index": 2}}, "alphaMode": "BLEND", "alphaCutoff": 0.9}],
+                      {"pbrMetallicRoughness": {"baseColorTexture": {"index": 2}}, "alphaMode": "BLEND", "alphaCutoff": 0.9},
+                      {"pbrMetallicRoughness": {"baseColorTexture": {"index": 3}}}],
         "meshes": [{"primitives": [{"attributes": {"POSITION": 0, "TEXCOORD_0": 1}, "material": 0},
-                                   {"attributes": {"POSITION": 0, "TEXCOORD_0": 1}, "material": 1}]}],
+                                   {"attributes": {"POSITION": 0, "TEXCOORD_0": 1}, "material": 1},
+                                   {"attributes": {"POSITION": 0, "TEXCOORD_0": 1}, "material": 2}]}],
         "nodes": [{"mesh": 0}], "scenes": [{"nodes": [0]}]})
     s = host.Scene.from_gltf(_write(tmp_path / "tex.gltf", doc))
     r = s.raw()
     # base colour (sRGB) and emissive (sRGB) share one texture; metal-roughness (linear) gets its own;
-    # the normal texture uses the other sampler (repeat); the JPEG one is skipped.
-    m0, m1 = s.material(0), s.material(1)
+    # the normal texture uses the other sampler (repeat); the baseline JPEG decodes; the progressive one
+    # is skipped with a warning.
+    m0, m1, m2 = s.material(0), s.material(1), s.material(2)
     assert (m0.base_color_tex, m0.emissive_tex, m0.metallic_roughness_tex, m0.normal_tex) == (0, 0, 1, 2)
-    assert m1.base_color_tex == -1
-    assert r.num_textures == 3
+    assert m1.base_color_tex == 3 and m2.base_color_tex == -1
+    assert r.num_textures == 4
+    jt = np.ctypeslib.as_array(r.tex_texels[3], (35,)).reshape(5, 7)
+    want = np.asarray(Image.open(io.BytesIO(jpg.getvalue())).convert("RGBA")).astype(np.int32)
+    assert np.abs(_channels(jt) - want).max() <= 4
     assert [r.tex_srgb[i] for i in range(3)] == [1, 0, 0] and [r.tex_repeat[i] for i in range(3)] == [0, 0, 1]
     for i in range(3):
         assert (r.tex_width[i], r.tex_height[i]) == (7, 5)
@@ -478,11 +487,58 @@ def test_png_writer_round_trips_through_pillow(tmp_path):
         assert np.array_equal(host.decode_image((tmp_path / "o.png").read_bytes()), px)
 
 
+def _channels(texels):
+    t = np.asarray(texels, np.uint32)
+    return np.stack([(t >> s) & 255 for s in (0, 8, 16, 24)], -1).astype(np.int32)
+
+
+def _jpeg_test_image(h, w, seed):
+    g = np.random.default_rng(seed)
+    yy, xx = np.mgrid[0:h, 0:w]
+    smooth = np.stack([(xx * 2) % 256, (yy * 3) % 256, ((xx + yy) * 2) % 256], -1)
+    return np.clip(smooth * 0.7 + g.integers(0, 70, (h, w, 3)), 0, 255).astype(np.uint8)
+
+
+@pytest.mark.parametrize("subsampling", [0, 1, 2])  # 4:4:4, 4:2:2, 4:2:0
+@pytest.mark.parametrize("quality", [30, 75, 95, 100])
+def test_jpeg_decoder_against_pillow(subsampling, quality):
+    """Baseline JPEG against Pillow's decoder (libjpeg): inverse DCTs and colour conversions differ in
+    their last bits, so agreement is within a few code values, nearly everywhere exact."""
+    img = _jpeg_test_image(77, 131, subsampling * 7 + quality)
+    b = io.BytesIO()
+    Image.fromarray(img).save(b, "JPEG", quality=quality, subsampling=subsampling)
+    got = _channels(host.decode_image(b.getvalue()))
+    want = np.asarray(Image.open(io.BytesIO(b.getvalue())).convert("RGBA")).astype(np.int32)
+    d = np.abs(got - want)
+    assert d.max() <= 4 and d.mean() <= 0.1, (d.max(), d.mean())
+
+
+@pytest.mark.parametrize("kind", ["grey", "restart", "tiny", "odd"])
+def test_jpeg_variants(kind):
+    b = io.BytesIO()
+    if kind == "grey":
+        Image.fromarray(_jpeg_test_image(40, 57, 1)[..., 0]).save(b, "JPEG", quality=85)
+    elif kind == "restart":
+        Image.fromarray(_jpeg_test_image(64, 96, 2)).save(b, "JPEG", quality=80, restart_marker_blocks=3)
+    elif kind == "tiny":
+        Image.fromarray(_jpeg_test_image(1, 1, 3)).save(b, "JPEG", quality=90, subsampling=2)
+    else:
+        Image.fromarray(_jpeg_test_image(17, 33, 4)).save(b, "JPEG", quality=90, subsampling=2)
+    got = _channels(host.decode_image(b.getvalue()))
+    want = np.asarray(Image.open(io.BytesIO(b.getvalue())).convert("RGBA")).astype(np.int32)
+    assert got.shape == want.shape
+    assert np.abs(got - want).max() <= 4
+
+
 def test_image_decode_rejects_other_formats():
     jpg = io.BytesIO()
-    Image.fromarray(np.zeros((4, 4, 3), np.uint8)).save(jpg, "JPEG")
-    with pytest.raises(RuntimeError, match="JPEG"):
+    Image.fromarray(np.zeros((16, 16, 3), np.uint8)).save(jpg, "JPEG", progressive=True)
+    with pytest.raises(RuntimeError, match="progressive JPEG"):
         host.decode_image(jpg.getvalue())
+    cmyk = io.BytesIO()
+    Image.fromarray(np.zeros((16, 16, 4), np.uint8), "CMYK").save(cmyk, "JPEG")
+    with pytest.raises(RuntimeError, match="JPEG"):
+        host.decode_image(cmyk.getvalue())
     with pytest.raises(RuntimeError, match="unsupported image format"):
         host.decode_image(b"GIF89a....")
     png = _encode_png(np.zeros((4, 4, 3), np.int64), 2, 8, 0, np.random.default_rng(0))
