@@ -198,6 +198,14 @@ struct Failure {
 
 [[noreturn]] void fail(const std::string& msg) { throw Failure{msg}; }
 
+// A byte offset, length or stride of the file: a whole number in [0, 2^40), so that sums and products
+// of them cannot wrap.
+size_t byte_size(const json::Value& v, double fallback, const char* what) {
+    const double d = v.number(fallback);
+    if (!(d >= 0.0) || d >= 1099511627776.0 || d != std::floor(d)) fail(std::string("invalid ") + what);
+    return size_t(d);
+}
+
 class GltfImport {
 public:
     GltfImport(Scene& s, const std::string& path) : scene_(s), path_(path) {}
@@ -300,13 +308,19 @@ private:
         return buffers_[size_t(b)];
     }
 
+    static int accessor_count(const Value& a) {
+        const size_t n = byte_size(a["count"], 0, "accessor count");
+        if (n > size_t(1) << 30) fail("accessor count out of range");
+        return int(n);
+    }
+
     static int component_size(int ct) { return (ct == 5126 || ct == 5125) ? 4 : (ct == 5122 || ct == 5123) ? 2 : 1; }
 
     // Accessor -> count x `want` floats (missing components stay 0).
     std::vector<float> floats(int accessor, int want) {
         const Value& a = doc_["accessors"][size_t(accessor)];
         if (a.is_null()) fail("missing accessor");
-        const int count = a["count"].integer(0);
+        const int count = accessor_count(a);
         static const std::map<std::string, int> widths = {{"SCALAR", 1}, {"VEC2", 2}, {"VEC3", 3}, {"VEC4", 4}, {"MAT4", 16}};
         const auto wit = widths.find(a["type"].string());
         if (wit == widths.end()) fail("unsupported accessor type");
@@ -318,8 +332,8 @@ private:
         if (!a.has("bufferView")) return out;
         const Value& view = doc_["bufferViews"][size_t(a["bufferView"].integer())];
         const std::vector<uint8_t>& buf = buffer_of(view);
-        const size_t base = size_t(view["byteOffset"].integer(0)) + size_t(a["byteOffset"].integer(0));
-        const size_t stride = view.has("byteStride") ? size_t(view["byteStride"].integer()) : size_t(nc * cs);
+        const size_t base = byte_size(view["byteOffset"], 0, "byteOffset") + byte_size(a["byteOffset"], 0, "byteOffset");
+        const size_t stride = view.has("byteStride") ? byte_size(view["byteStride"], 0, "byteStride") : size_t(nc * cs);
         if (count > 0 && base + stride * size_t(count - 1) + size_t(nc * cs) > buf.size()) fail("accessor out of the buffer's range");
         const int take = std::min(nc, want);
         for (int i = 0; i < count; i++) {
@@ -359,13 +373,13 @@ private:
 
     std::vector<int> indices(int accessor) {
         const Value& a = doc_["accessors"][size_t(accessor)];
-        const int count = a["count"].integer(0);
+        const int count = accessor_count(a);
         const int cs = a["componentType"].integer() == 5125 ? 4 : a["componentType"].integer() == 5123 ? 2 : 1;
         if (!a.has("bufferView")) fail("index accessor without data");
         const Value& view = doc_["bufferViews"][size_t(a["bufferView"].integer())];
         const std::vector<uint8_t>& buf = buffer_of(view);
-        const size_t base = size_t(view["byteOffset"].integer(0)) + size_t(a["byteOffset"].integer(0));
-        const size_t stride = view.has("byteStride") ? size_t(view["byteStride"].integer()) : size_t(cs);
+        const size_t base = byte_size(view["byteOffset"], 0, "byteOffset") + byte_size(a["byteOffset"], 0, "byteOffset");
+        const size_t stride = view.has("byteStride") ? byte_size(view["byteStride"], 0, "byteStride") : size_t(cs);
         if (count > 0 && base + stride * size_t(count - 1) + size_t(cs) > buf.size()) fail("index accessor out of range");
         std::vector<int> out(size_t(std::max(count, 0)));
         for (int i = 0; i < count; i++) {
@@ -408,12 +422,12 @@ private:
                 return false;
             }
             const std::vector<uint8_t>& buf = buffers_[size_t(b)];
-            const size_t off = size_t(view["byteOffset"].integer(0)), len = size_t(view["byteLength"].integer(0));
-            if (off + len > buf.size()) {
+            const double od = view["byteOffset"].number(0), ld = view["byteLength"].number(0);
+            if (!(od >= 0 && ld >= 0 && od + ld <= double(buf.size()))) {
                 why = "image buffer view out of range";
                 return false;
             }
-            return decode_image(buf.data() + off, len, tex, why);
+            return decode_image(buf.data() + size_t(od), size_t(ld), tex, why);
         }
         why = "image without data";
         return false;

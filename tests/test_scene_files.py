@@ -657,3 +657,63 @@ def test_gltf_writer_writes_each_primitive_its_own_vertices(tmp_path):
         ta = A["pos"][A["idx"][A["mat"] == m]].reshape(-1, 9)
         tb = B["pos"][B["idx"][B["mat"] == m]].reshape(-1, 9)
         assert np.array_equal(ta[np.lexsort(ta.T)], tb[np.lexsort(tb.T)])
+
+
+def test_gltf_rejects_negative_and_huge_offsets(tmp_path):
+    """Offsets, strides and counts are checked before any read: negative or out-of-range values fail with
+    a message instead of wrapping around the buffer bounds."""
+    pos = np.zeros(9, np.float32).tobytes()
+    base = {"meshes": [{"primitives": [{"attributes": {"POSITION": 0}}]}], "nodes": [{"mesh": 0}], "scenes": [{"nodes": [0]}]}
+    bad = {
+        "byteOffset": dict(bufferViews=[{"buffer": 0, "byteOffset": -8, "byteLength": 36}],
+                           accessors=[{"bufferView": 0, "componentType": 5126, "count": 3, "type": "VEC3"}]),
+        "byteStride": dict(bufferViews=[{"buffer": 0, "byteLength": 36, "byteStride": -12}],
+                           accessors=[{"bufferView": 0, "componentType": 5126, "count": 3, "type": "VEC3"}]),
+        "accessor count": dict(bufferViews=[{"buffer": 0, "byteLength": 36}],
+                               accessors=[{"bufferView": 0, "componentType": 5126, "count": -3, "type": "VEC3"}]),
+    }
+    for i, (msg, body) in enumerate(bad.items()):
+        with pytest.raises(RuntimeError, match=msg):
+            host.Scene.from_gltf(_write(tmp_path / f"w{i}.gltf", _doc_with_buffer(pos, dict(base, **body))))
+    huge = dict(base, bufferViews=[{"buffer": 0, "byteLength": 36}],
+                accessors=[{"bufferView": 0, "componentType": 5126, "count": 3, "type": "VEC3", "byteOffset": 2**62}])
+    with pytest.raises(RuntimeError, match="byteOffset"):
+        host.Scene.from_gltf(_write(tmp_path / "huge.gltf", _doc_with_buffer(pos, huge)))
+
+
+def test_decoders_survive_corrupted_files(tmp_path):
+    """Seeded corruptions (bytes changed, cut out or inserted) of valid PNG and JPEG files and of a glTF
+    document either decode or fail with a message; none may crash the process."""
+    g = np.random.default_rng(21)
+    img = g.integers(0, 256, (23, 37, 3), dtype=np.uint8)
+    seeds = []
+    for kw in (dict(format="PNG"), dict(format="JPEG", quality=80), dict(format="JPEG", quality=60, subsampling=2),
+               dict(format="JPEG", restart_marker_blocks=2)):
+        b = io.BytesIO()
+        Image.fromarray(img).save(b, **kw)
+        seeds.append(b.getvalue())
+    for it in range(400):
+        s = bytearray(seeds[it % len(seeds)])
+        for _ in range(int(g.integers(1, 8))):
+            k, p = int(g.integers(0, 3)), int(g.integers(0, len(s)))
+            if k == 0:
+                s[p] = int(g.integers(0, 256))
+            elif k == 1:
+                del s[p:p + int(g.integers(1, 16))]
+            else:
+                s[p:p] = bytes(g.integers(0, 256, int(g.integers(1, 8))).astype(np.uint8))
+        try:
+            host.decode_image(bytes(s))
+        except RuntimeError:
+            pass
+    host.Scene.builtin("cornell").save_gltf(tmp_path / "c.gltf")
+    text = (tmp_path / "c.gltf").read_text()
+    for it in range(100):
+        t = list(text)
+        for _ in range(int(g.integers(1, 6))):
+            t[int(g.integers(0, len(t)))] = "0123456789-,:[]{}\"e. "[int(g.integers(0, 21))]
+        (tmp_path / "m.gltf").write_text("".join(t))
+        try:
+            host.Scene.from_gltf(tmp_path / "m.gltf")
+        except RuntimeError:
+            pass
