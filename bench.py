@@ -485,7 +485,7 @@ def main():
                     help="also run one device-to-device copy of this many MiB (the known-size kernel that scales counter readings)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU: --gpus must be 1", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "the renderer runs on one GPU: --gpus must be 1", "n_gpus": args.gpus}))
         sys.exit(2)
     from photon_amd import gpu, host
     from photon_amd.types import RenderSettings
