@@ -357,6 +357,7 @@ def bench_serving(args, gpu, host, field, launch):
     arrival = start + np.cumsum(gaps) * 1e3
     live, lat, t_arr = {}, {}, {}
     nxt = 0
+    reported = 0
     while nxt < len(jobs) or srv.pending:
         now = srv.now_ms()
         while nxt < len(jobs) and arrival[nxt] <= now:
@@ -368,6 +369,9 @@ def bench_serving(args, gpu, host, field, launch):
             drain(live, lat, t_arr)
         elif nxt < len(jobs):
             time.sleep(max(0.0, (arrival[nxt] - srv.now_ms()) * 1e-3))
+        if len(lat) >= reported + 500:
+            reported = len(lat) // 500 * 500
+            log(f"serving: {len(lat)} of {len(jobs)} jobs done")
     end = srv.now_ms()
     # Isolated service time: each job again, alone on the idle server (submission to completion).
     iso = np.zeros(len(jobs))
@@ -378,6 +382,8 @@ def bench_serving(args, gpu, host, field, launch):
         st = srv.status(j)
         iso[k] = st.done_ms - st.submit_ms
         srv.release(j)
+        if (k + 1) % 500 == 0:
+            log(f"serving: {k + 1} of {len(jobs)} jobs timed alone")
     srv.close()
     L = np.array([lat[k] for k in range(len(jobs))])
     samples = np.array([j["w"] * j["h"] * j["spp"] for j in jobs], np.float64)
