@@ -41,16 +41,21 @@ def test_cornell_through_gltf(tmp_path, engine):
     assert agree >= 0.999 and img.mean() > 0.01
 
 
-def test_textured_alpha_masked_gltf(tmp_path):
-    """A floor, a back wall and an alpha-masked leaf quad with a PNG base-colour texture, lit by an
-    emissive quad: primary and shadow rays both run the alpha test against the decoded texels."""
+@pytest.mark.parametrize("fmt", ["PNG", "JPEG"])
+def test_textured_alpha_masked_gltf(tmp_path, fmt):
+    """A floor, a back wall and an alpha-masked leaf quad with a base-colour texture (PNG with a disc of
+    alpha, or a baseline JPEG, which is opaque), lit by an emissive quad: primary and shadow rays run the
+    alpha test against the texels the host library decoded."""
     g = np.random.default_rng(12)
     tex = np.zeros((64, 64, 4), np.uint8)
     tex[..., :3] = g.integers(40, 255, (64, 64, 3))
     yy, xx = np.mgrid[0:64, 0:64]
     tex[..., 3] = np.where((xx - 32) ** 2 + (yy - 32) ** 2 < 26 ** 2, 255, 0)  # a disc
     png = io.BytesIO()
-    Image.fromarray(tex, "RGBA").save(png, "PNG")
+    if fmt == "PNG":
+        Image.fromarray(tex, "RGBA").save(png, "PNG")
+    else:
+        Image.fromarray(tex[..., :3], "RGB").save(png, "JPEG", quality=90)
     quads = [  # corner, edge u, edge v, material
         ((-4, 0, -4), (8, 0, 0), (0, 0, 8), 0),
         ((-4, 0, -4), (0, 8, 0), (8, 0, 0), 0),
@@ -76,7 +81,7 @@ def test_textured_alpha_masked_gltf(tmp_path):
     doc = {"asset": {"version": "2.0"},
            "buffers": [{"byteLength": len(bin_bytes), "uri": "data:application/octet-stream;base64," + base64.b64encode(bin_bytes).decode()}],
            "bufferViews": views, "accessors": accessors,
-           "images": [{"uri": "data:image/png;base64," + base64.b64encode(png.getvalue()).decode()}],
+           "images": [{"uri": f"data:image/{fmt.lower()};base64," + base64.b64encode(png.getvalue()).decode()}],
            "textures": [{"source": 0}],
            "materials": [{"pbrMetallicRoughness": {"baseColorFactor": [0.7, 0.7, 0.7, 1], "metallicFactor": 0, "roughnessFactor": 0.8}},
                          {"pbrMetallicRoughness": {"baseColorTexture": {"index": 0}, "metallicFactor": 0, "roughnessFactor": 0.5},
@@ -92,5 +97,5 @@ def test_textured_alpha_masked_gltf(tmp_path):
     assert s.raw().num_textures == 1 and s.material(1).base_color_tex == 0
     img, ref = _render_both(s, 80, 64, 8, gpu.ENGINE_WAVEFRONT)
     agree = radiance_agreement(img, ref)
-    print(f"textured glTF: {100 * agree:.3f}% within 0.1%, {100 * bitwise_fraction(img, ref):.2f}% bitwise, mean {img.mean():.4f}")
+    print(f"textured glTF ({fmt}): {100 * agree:.3f}% within 0.1%, {100 * bitwise_fraction(img, ref):.2f}% bitwise, mean {img.mean():.4f}")
     assert agree >= 0.999 and img.mean() > 0.01
