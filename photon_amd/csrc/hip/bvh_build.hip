@@ -9,13 +9,14 @@
 //   * centroid and bin boxes enter only through differences and areas, where the sign of a zero makes
 //     no difference, so bins use unordered LDS atomics on the order-preserving integer image of floats.
 //   * the split search (first minimum over axis 0..2, then bin 0..nb-2) and every leaf test are the
-//     host's own code, run on the host for large nodes and by one thread for small ones.
+//     host's own code on the host for large nodes; for small ones a wave scans the bins and reduces
+//     (cost, candidate) pairs to the same first minimum.
 //   * the partition is std::partition's: the k-th element on the wrong side counted from the front
 //     swaps with the k-th counted from the back; ranks come from scans, so it runs in parallel.
 // Large nodes (more than kChunk triangles) are split level by level by multi-block passes with the
-// per-node decisions on the host; every subtree of at most kChunk triangles is then built depth first
-// by one block. The node indices of the build tree differ from the host's, but the flattening numbers
-// nodes by tree structure only.
+// per-node decisions on the host; every subtree of at most kChunk triangles is then built level by
+// level by one block, a wave per node. The node indices of the build tree differ from the host's, but
+// the flattening, also on the device, numbers nodes by tree structure only.
 #include <algorithm>
 #include <chrono>
 #include <mutex>
@@ -531,6 +532,132 @@ __global__ __launch_bounds__(kSubWaves * 64) void subtree_kernel(const Pending* 
     }
 }
 
+// Flattening on the device. The host builder numbers the flat nodes by a stack walk (reference
+// bvh.cpp:214-255): popping node X gives X's internal children the next free slots, left then right, and
+// pushes them in that order, so the right subtree is walked before the left. Let I(n) be the number of
+// internal nodes in n's subtree (0 for a leaf), c(X) the walk's slot counter when X is popped and p(X) X's
+// pop index; at the root c = 1, p = 0 and slot = 0. With k(P) the internal children of X's parent P and
+// R = P's right child:
+//   X = right child of P:  p = p(P) + 1,          c = c(P) + k(P),                      slot = c(P) + [P's left child internal]
+//   X = left child of P:   p = p(P) + 1 + I(R),   c = c(P) + k(P) + max(I(R) - 1, 0),   slot = c(P)
+// and X's internal children get slots c(X) (left) and c(X) + [left internal] (right). Every node sums
+// these terms over its path to the root, so all nodes are placed at once with no ordered walk.
+// Kernels, in order: parent links, leaf-start flags, their prefix sum (two passes), I(n), placement.
+
+// Parent links; leaf-start flags [0, n] and the depth maximum start at zero. The grid covers
+// max(m, n + 1) threads.
+__global__ void flat_link_kernel(const TNode* tree, int m, int n, int* parent, int* flag, int* max_depth) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= n) flag[i] = 0;
+    if (i >= m) return;
+    if (i == 0) {
+        parent[0] = -1;
+        *max_depth = 0;
+    }
+    const TNode t = tree[i];
+    if (t.left >= 0) {
+        parent[t.left] = i;
+        parent[t.right] = i;
+    }
+}
+
+// Leaf ranges are non-empty and tile [0, n), and a node's range is the union of its subtree's leaves,
+// so I(n) is the number of leaf starts in n's range, less one: a prefix sum over these flags.
+__global__ void flat_mark_kernel(const TNode* tree, int m, int* flag) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const TNode t = tree[i];
+    if (t.left < 0) flag[t.first] = 1;
+}
+
+constexpr int kTile = 4 * kThreads;  // flags per block of the prefix sum
+
+// Exclusive prefix of flag[0, len) within each tile of kTile flags; the tile's total goes to sums.
+__global__ __launch_bounds__(kThreads) void tile_scan_kernel(const int* flag, int len, int* pre, int* sums) {
+    __shared__ int red[kThreads / 64];
+    const int base = blockIdx.x * kTile + threadIdx.x * 4;
+    int v[4], s = 0;
+    for (int k = 0; k < 4; k++) {
+        v[k] = base + k < len ? flag[base + k] : 0;
+        s += v[k];
+    }
+    int excl;
+    const int total = block_scan(s, red, excl);
+    for (int k = 0; k < 4; k++) {
+        if (base + k < len) pre[base + k] = excl;
+        excl += v[k];
+    }
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// Exclusive scan of the tile totals in place, by one block.
+__global__ __launch_bounds__(kThreads) void tile_sums_kernel(int* sums, int tiles) {
+    __shared__ int red[kThreads / 64];
+    int carry = 0;
+    for (int a = 0; a < tiles; a += kThreads) {  // the same trip count for every thread
+        const int i = a + int(threadIdx.x);
+        const int v = i < tiles ? sums[i] : 0;
+        int excl;
+        const int total = block_scan(v, red, excl);
+        if (i < tiles) sums[i] = carry + excl;
+        carry += total;
+    }
+}
+
+__global__ void flat_inner_kernel(const TNode* tree, int m, const int* pre, const int* sums, int* inner) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const TNode t = tree[i];
+    if (t.left < 0) {
+        inner[i] = 0;
+        return;
+    }
+    const int a = t.first, b = t.first + t.count;
+    inner[i] = (pre[b] + sums[b / kTile]) - (pre[a] + sums[a / kTile]) - 1;
+}
+
+// One flat node per internal build node, at its slot; terms[p] holds the two SAH terms the walk adds when
+// it pops the node (pop index p), so the host can sum them in the walk's order.
+__global__ void flat_place_kernel(const TNode* tree, int m, int internal, const int* parent, const int* inner, float traversal_cost,
+                                  float root_area, ph_bvh_node* nodes, float2* terms, int* max_depth) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= m) return;
+    const TNode t = tree[x];
+    if (t.left < 0) return;
+    int p = 0, c = 1, depth = 0, first_dc = 0, slot_add = 0, n = x;
+    for (int pp = parent[n]; pp >= 0; pp = parent[n]) {
+        const TNode pt = tree[pp];
+        const int ir = inner[pt.right], k = (inner[pt.left] > 0) + (ir > 0);
+        const bool right = n == pt.right;
+        const int dc = right ? k : k + max(ir - 1, 0);
+        p += right ? 1 : 1 + ir;
+        c += dc;
+        if (depth == 0) {
+            first_dc = dc;
+            slot_add = right ? (inner[pt.left] > 0) : 0;
+        }
+        depth++;
+        n = pp;
+    }
+    const int slot = x == 0 ? 0 : c - first_dc + slot_add;
+    if (slot >= internal || p >= internal) return;  // only a malformed tree; the host's count check reports it
+    const TNode l = tree[t.left], r = tree[t.right];
+    ph_bvh_node nd;
+    for (int k = 0; k < 3; k++) {
+        nd.lmin[k] = l.box.lo[k], nd.lmax[k] = l.box.hi[k];
+        nd.rmin[k] = r.box.lo[k], nd.rmax[k] = r.box.hi[k];
+    }
+    const bool li = l.left >= 0, ri = r.left >= 0;
+    nd.left = li ? c : l.first;
+    nd.lcount = li ? 0 : l.count;
+    nd.right = ri ? c + (li ? 1 : 0) : r.first;
+    nd.rcount = ri ? 0 : r.count;
+    nodes[slot] = nd;
+    terms[p] = make_float2((li ? traversal_cost : float(l.count)) * surface(l.box) / root_area,
+                           (ri ? traversal_cost : float(r.count)) * surface(r.box) / root_area);
+    atomicMax(max_depth, depth);
+}
+
 // Device buffers that only grow: per-level arrays keep their allocation across levels (a smaller request
 // reuses it), so the build does not free and reallocate, and hipFree's device synchronisation is avoided.
 template <class T>
@@ -558,11 +685,9 @@ hipStream_t build_stream() {
 }  // namespace
 
 int64_t bvh_build_device(const float* positions, int num_vertices, const int* indices, int n, int bins, int max_leaf, float traversal_cost,
-                         std::vector<ph_bvh_node>& nodes_out, std::vector<int>& order_out, ph_bvh_build_stats& stats) {
+                         ph_bvh_node* nodes_out, int* order_out, ph_bvh_build_stats& stats) {
     const auto t0 = std::chrono::steady_clock::now();
     stats = ph_bvh_build_stats{};
-    order_out.assign(size_t(n), 0);
-    nodes_out.clear();
     auto clear_side = [](ph_bvh_node& nd, int side) {
         float* lo = side == 0 ? nd.lmin : nd.rmin;
         float* hi = side == 0 ? nd.lmax : nd.rmax;
@@ -570,11 +695,18 @@ int64_t bvh_build_device(const float* positions, int num_vertices, const int* in
         (side == 0 ? nd.left : nd.right) = -1;
         (side == 0 ? nd.lcount : nd.rcount) = 0;
     };
+    // A root that is a leaf: its range on the left, an empty right side.
+    auto write_leaf_root = [&](ph_bvh_node& nd, const TNode& root) {
+        for (int k = 0; k < 3; k++) nd.lmin[k] = root.box.lo[k], nd.lmax[k] = root.box.hi[k];
+        nd.left = root.first;
+        nd.lcount = root.count;
+        clear_side(nd, 1);
+    };
     if (n == 0) {
         ph_bvh_node root{};
         clear_side(root, 0);
         clear_side(root, 1);
-        nodes_out.push_back(root);
+        nodes_out[0] = root;
         stats.nodes = 1;
         return 1;
     }
@@ -810,62 +942,67 @@ int64_t bvh_build_device(const float* positions, int num_vertices, const int* in
         PH_CHECK(hipMemcpyAsync(&next_node, d_counter.ptr, sizeof(int), hipMemcpyDeviceToHost, st));
         PH_CHECK(hipStreamSynchronize(st));
     }
-    std::vector<TNode> tree(static_cast<size_t>(next_node));
-    PH_CHECK(hipMemcpyAsync(tree.data(), d_tree.ptr, tree.size() * sizeof(TNode), hipMemcpyDeviceToHost, st));
-    PH_CHECK(hipMemcpyAsync(order_out.data(), d_order.ptr, size_t(n) * sizeof(int), hipMemcpyDeviceToHost, st));
+    PH_CHECK(hipMemcpyAsync(order_out, d_order.ptr, size_t(n) * sizeof(int), hipMemcpyDeviceToHost, st));
+    TNode root;
+    PH_CHECK(hipMemcpyAsync(&root, d_tree.ptr, sizeof(TNode), hipMemcpyDeviceToHost, st));
     PH_CHECK(hipStreamSynchronize(st));
-
-    // Flattening, as the host builder does it: a node's two children numbered consecutively, the right
-    // child's subtree emitted before the left child's.
-    auto write_side = [&](ph_bvh_node& nd, int side, const TNode& c, int child) {
-        float* lo = side == 0 ? nd.lmin : nd.rmin;
-        float* hi = side == 0 ? nd.lmax : nd.rmax;
-        for (int k = 0; k < 3; k++) lo[k] = c.box.lo[k], hi[k] = c.box.hi[k];
-        const bool leaf = c.left < 0;
-        (side == 0 ? nd.left : nd.right) = leaf ? c.first : child;
-        (side == 0 ? nd.lcount : nd.rcount) = leaf ? c.count : 0;
-    };
-    const TNode& root = tree[0];
     for (int k = 0; k < 3; k++) stats.bounds_min[k] = root.box.lo[k], stats.bounds_max[k] = root.box.hi[k];
-    const float root_area = std::max(surface(root.box), 1e-20f);
-    nodes_out.push_back(ph_bvh_node{});
     if (root.left < 0) {
-        write_side(nodes_out[0], 0, root, 0);
-        clear_side(nodes_out[0], 1);
+        write_leaf_root(nodes_out[0], root);
         stats.nodes = 1;
         stats.leaves = 1;
         stats.sah_cost = float(root.count);
     } else {
-        struct Item {
-            int tree, slot, depth;
-        };
-        std::vector<Item> stack{{0, 0, 0}};
-        stats.sah_cost = traversal_cost;
-        while (!stack.empty()) {
-            const Item it = stack.back();
-            stack.pop_back();
-            stats.nodes++;
-            stats.max_depth = std::max(stats.max_depth, it.depth);
-            const TNode& tn = tree[size_t(it.tree)];
-            for (int side = 0; side < 2; side++) {
-                const int ci = side == 0 ? tn.left : tn.right;
-                const TNode& c = tree[size_t(ci)];
-                int slot = -1;
-                if (c.left >= 0) {
-                    slot = int(nodes_out.size());
-                    nodes_out.push_back(ph_bvh_node{});
-                    stack.push_back({ci, slot, it.depth + 1});
-                    stats.sah_cost += traversal_cost * surface(c.box) / root_area;
-                } else {
-                    stats.leaves++;
-                    stats.sah_cost += float(c.count) * surface(c.box) / root_area;
-                }
-                write_side(nodes_out[size_t(it.slot)], side, c, slot);
-            }
+        // Flattening on the device (kernels above); the host only sums the SAH terms in the walk's order.
+        const float root_area = std::max(surface(root.box), 1e-20f);
+        const int m = next_node, grid = (m + 255) / 256;
+        DevBuf<int> d_parent, d_flag, d_pre, d_sums, d_inner, d_depth;
+        DevBuf<ph_bvh_node> d_nodes;
+        DevBuf<float2> d_terms;
+        const int len = n + 1, tiles = (len + kTile - 1) / kTile;
+        d_parent.allocate(size_t(m));
+        d_flag.allocate(size_t(len));
+        d_pre.allocate(size_t(len));
+        d_sums.allocate(size_t(tiles));
+        d_inner.allocate(size_t(m));
+        d_depth.allocate(1);
+        // A full binary tree of m nodes has (m - 1) / 2 internal ones.
+        const int internal = (m - 1) / 2;
+        d_nodes.allocate(size_t(internal));
+        d_terms.allocate(size_t(internal));
+        hipLaunchKernelGGL(flat_link_kernel, dim3((std::max(m, len) + 255) / 256), dim3(256), 0, st, d_tree.ptr, m, n, d_parent.ptr, d_flag.ptr,
+                           d_depth.ptr);
+        PH_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(flat_mark_kernel, dim3(grid), dim3(256), 0, st, d_tree.ptr, m, d_flag.ptr);
+        PH_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(tile_scan_kernel, dim3(tiles), dim3(kThreads), 0, st, d_flag.ptr, len, d_pre.ptr, d_sums.ptr);
+        PH_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(tile_sums_kernel, dim3(1), dim3(kThreads), 0, st, d_sums.ptr, tiles);
+        PH_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(flat_inner_kernel, dim3(grid), dim3(256), 0, st, d_tree.ptr, m, d_pre.ptr, d_sums.ptr, d_inner.ptr);
+        PH_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(flat_place_kernel, dim3(grid), dim3(256), 0, st, d_tree.ptr, m, internal, d_parent.ptr, d_inner.ptr, traversal_cost,
+                           root_area, d_nodes.ptr, d_terms.ptr, d_depth.ptr);
+        PH_CHECK(hipGetLastError());
+        int root_inner = 0;
+        std::vector<float2> terms(static_cast<size_t>(internal));
+        PH_CHECK(hipMemcpyAsync(&root_inner, d_inner.ptr, sizeof(int), hipMemcpyDeviceToHost, st));
+        PH_CHECK(hipMemcpyAsync(&stats.max_depth, d_depth.ptr, sizeof(int), hipMemcpyDeviceToHost, st));
+        PH_CHECK(hipMemcpyAsync(nodes_out, d_nodes.ptr, size_t(internal) * sizeof(ph_bvh_node), hipMemcpyDeviceToHost, st));
+        PH_CHECK(hipMemcpyAsync(terms.data(), d_terms.ptr, terms.size() * sizeof(float2), hipMemcpyDeviceToHost, st));
+        PH_CHECK(hipStreamSynchronize(st));
+        if (root_inner != internal) throw std::runtime_error("device BVH build: the tree is not a full binary tree");
+        stats.nodes = internal;
+        stats.leaves = internal + 1;
+        float sah = traversal_cost;
+        for (const float2& t : terms) {
+            sah += t.x;
+            sah += t.y;
         }
+        stats.sah_cost = sah;
     }
     stats.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    return int64_t(nodes_out.size());
+    return int64_t(stats.nodes);
 }
 
 }  // namespace phk

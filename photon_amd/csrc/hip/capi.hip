@@ -556,12 +556,8 @@ int ph_bvh_build(const float* positions, int num_vertices, const int32_t* indice
     for (int64_t i = 0; i < int64_t(num_tris) * 3; i++)
         if (indices[i] < 0 || indices[i] >= num_vertices) return fail("vertex index out of range");
     return guarded([&] {
-        std::vector<ph_bvh_node> out;
-        std::vector<int> ord;
         const int64_t count = bvh_build_device(positions, num_vertices, indices, num_tris, bins > 0 ? bins : 32, max_leaf > 0 ? max_leaf : 8,
-                                               traversal_cost > 0 ? traversal_cost : 1.0f, out, ord, *stats);
-        std::memcpy(nodes, out.data(), out.size() * sizeof(ph_bvh_node));
-        if (num_tris > 0) std::memcpy(order, ord.data(), ord.size() * sizeof(int));
+                                               traversal_cost > 0 ? traversal_cost : 1.0f, nodes, order, *stats);
         return int(count);
     });
 }

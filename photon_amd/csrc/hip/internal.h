@@ -90,9 +90,10 @@ void launch_render(hipStream_t st, const phd::DScene& s, const ph_camera_view& c
                    const phd::PathOpts& opt, float4* accum, float4* albedo, float4* nd);
 void launch_sample(hipStream_t st, const phd::DScene& s, const ph_camera_view& cam, int w, int h, uint32_t frame, uint32_t seed, const phd::PathOpts& opt,
                    float4* color, float4* albedo, float4* nd, float4* position, float4* emission);
-// The device BVH build behind ph_bvh_build; returns the node count.
+// The device BVH build behind ph_bvh_build: writes the flat nodes (room for max(1, 2n - 1)) and the
+// triangle order (n entries) to host memory and returns the node count.
 int64_t bvh_build_device(const float* positions, int num_vertices, const int* indices, int n, int bins, int max_leaf, float traversal_cost,
-                         std::vector<ph_bvh_node>& nodes_out, std::vector<int>& order_out, ph_bvh_build_stats& stats);
+                         ph_bvh_node* nodes_out, int* order_out, ph_bvh_build_stats& stats);
 
 // Guide tables of `rows` CDFs of n + 1 entries each (row r at cdf + r * stride): guide[r * (m + 1) + k]
 // is the index the CDF search returns for u = k / m (m a power of two).
