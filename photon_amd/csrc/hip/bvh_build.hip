@@ -13,8 +13,8 @@
 //     (cost, candidate) pairs to the same first minimum.
 //   * the partition is std::partition's: the k-th element on the wrong side counted from the front
 //     swaps with the k-th counted from the back; ranks come from scans, so it runs in parallel.
-// Large nodes (more than kChunk triangles) are split level by level by multi-block passes with the
-// per-node decisions on the host; every subtree of at most kChunk triangles is then built level by
+// Large nodes (more than kSubtree triangles) are split level by level by multi-block passes with the
+// per-node decisions on the host; every subtree of at most kSubtree triangles is then built level by
 // level by one block, a wave per node. The node indices of the build tree differ from the host's, but
 // the flattening, also on the device, numbers nodes by tree structure only.
 #include <algorithm>
@@ -33,7 +33,9 @@ namespace {
 
 constexpr float kBoxInf = 1e30f;  // the host builder's empty box bound
 constexpr int kMaxBins = 64;
-constexpr int kChunk = 2048;  // triangles per block: large-node passes, and the small-subtree bound
+constexpr int kChunk = 2048;    // triangles per block of the large-node passes
+constexpr int kSubtree = 2048;  // subtrees of at most this many triangles are built by one block each
+static_assert(kSubtree <= kChunk, "the subtree kernel's level lists hold kChunk entries");
 constexpr int kThreads = 256;
 
 // The host's std::fmin / std::fmax for finite values: the second operand unless the first is strictly
@@ -280,7 +282,7 @@ __global__ void swap_kernel(int* order, const int* left_pos, const int* right_po
 
 // ---- small subtrees: one block each, one wave per node, level by level ----------------------------------
 //
-// A block builds the subtree of one root (at most kChunk triangles) breadth first: the nodes of a level are
+// A block builds the subtree of one root (at most kSubtree triangles) breadth first: the nodes of a level are
 // shared out to the block's waves, each wave builds its node alone (boxes, bins, split, partition) and
 // appends the children to the next level's list; a block barrier separates levels. The nodes of a level
 // cover disjoint ranges of `order`, so the waves never touch the same elements. Node indices come from a
@@ -502,8 +504,8 @@ PH_DEV void build_node(const Pending& nd, int nb, int max_leaf, float traversal_
     }
 }
 
-// lists: per block two level lists of kChunk entries. A level of a subtree of at most kChunk triangles
-// has at most kChunk nodes (their ranges are disjoint and non-empty).
+// lists: per block two level lists of kChunk entries. A level of a subtree of at most kSubtree <= kChunk
+// triangles has at most that many nodes (their ranges are disjoint and non-empty).
 __global__ __launch_bounds__(kSubWaves * 64) void subtree_kernel(const Pending* roots, int bins, int max_leaf, float traversal_cost, int* order,
                                                                   const BBox* boxes, const float4* centres, TNode* tree, int* node_counter,
                                                                   Pending* lists, int* tmp) {
@@ -733,7 +735,7 @@ int64_t bvh_build_device(const float* positions, int num_vertices, const int* in
     std::vector<Big> level{{0, 0, n, 0}};
     std::vector<Pending> small;
     int next_node = 1;
-    if (n <= kChunk) {
+    if (n <= kSubtree) {
         small.push_back(Pending{0, 0, n, 0});
         level.clear();
     }
@@ -866,7 +868,7 @@ int64_t bvh_build_device(const float* positions, int num_vertices, const int* in
                 for (int side = 0; side < 2; side++) {
                     const int cf = side == 0 ? first : m, cc = side == 0 ? m - first : first + count - m;
                     const int idx = side == 0 ? rec.left : rec.right;
-                    if (cc > kChunk)
+                    if (cc > kSubtree)
                         next.push_back(Big{idx, cf, cc, nd.depth + 1});
                     else
                         small.push_back(Pending{idx, cf, cc, nd.depth + 1});
