@@ -73,6 +73,9 @@ int or_svgf_run(or_svgf* f, const float* color4, const float* albedo4, const flo
 /* Internal planes for tests: 0 prev_illum, 1 prev_moments, 2 prev_nd, 3 prev_pos, 4 illum, 5 moments,
  * 6 filter0, 7 filter1. */
 const float* or_svgf_plane(const or_svgf* f, int which);
+/* One a-trous pass alone (step 1, 2, 4, ...) on w*h float4 planes, for tests. */
+int or_svgf_atrous(or_svgf* f, const float* normal_depth4, const float* position4, int step, float phi_color, float phi_normal, float phi_depth,
+                   const float* src4, float* dst4);
 
 /* The compact G-buffer of the real-time workload: fp16 conversion, octahedral normals, conversion from
  * the float4 planes, and the denoiser on a compact frame traced from `cam`. */

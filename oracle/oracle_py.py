@@ -63,6 +63,7 @@ def oracle():
         L.or_svgf_reset.restype = None
         L.or_svgf_run.argtypes = [P, FP, FP, FP, FP, FP, CV, C.POINTER(SvgfSettings), FP]
         L.or_svgf_plane.argtypes = [P, C.c_int]
+        L.or_svgf_atrous.argtypes = [P, FP, FP, C.c_int, C.c_float, C.c_float, C.c_float, FP, FP]
         L.or_svgf_plane.restype = FP
         L.or_f32_to_f16.argtypes = [C.c_float]
         L.or_f32_to_f16.restype = C.c_uint32
@@ -223,6 +224,13 @@ class SvgfOracle:
                                           c["normal"].ctypes.data, c["material"].ctypes.data, C.byref(cam), C.byref(prev_cam),
                                           C.byref(settings), fptr(out))
         assert rc == 0
+        return out.reshape(self.h, self.w, 4)
+
+    def atrous(self, normal_depth, position, src, step, phi_color, phi_normal, phi_depth):
+        """One a-trous pass alone over (h, w, 4) planes."""
+        nd, pos, sp = [np.ascontiguousarray(a, np.float32).reshape(-1) for a in (normal_depth, position, src)]
+        out = np.zeros(self.w * self.h * 4, np.float32)
+        assert oracle().or_svgf_atrous(self._h, fptr(nd), fptr(pos), step, phi_color, phi_normal, phi_depth, fptr(sp), fptr(out)) == 0
         return out.reshape(self.h, self.w, 4)
 
     def plane(self, which):

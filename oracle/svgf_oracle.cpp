@@ -411,6 +411,16 @@ int or_svgf_run_compact(or_svgf* f, const float* color3, const float* albedo3, c
     return or_svgf_run(f, &c[0].x, &a[0].x, &nd[0].x, &p[0].x, &e[0].x, prev_cam, st, out4);
 }
 
+// One a-trous pass (svgf.cu:134-186) on caller planes, for tests of the pass alone: normal+depth and
+// position (w = material id) are the G-buffer it reads; src and dst hold w*h float4.
+int or_svgf_atrous(or_svgf* f, const float* normal_depth4, const float* position4, int step, float phi_color, float phi_normal, float phi_depth,
+                   const float* src4, float* dst4) {
+    if (!f || !normal_depth4 || !position4 || !src4 || !dst4 || step <= 0) return -1;
+    Inputs in{nullptr, nullptr, reinterpret_cast<const V4*>(normal_depth4), reinterpret_cast<const V4*>(position4), nullptr};
+    atrous(*f, in, step, phi_color, phi_normal, phi_depth, reinterpret_cast<const V4*>(src4), reinterpret_cast<V4*>(dst4));
+    return 0;
+}
+
 const float* or_svgf_plane(const or_svgf* f, int which) {
     if (!f || which < 0 || which >= 8) return nullptr;
     return &f->plane[which][0].x;

@@ -1,6 +1,7 @@
 """Pins of the SVGF restatement (the reference has no CPU version, so these are the checks that fix it):
 special cases with closed answers (a constant image on a constant G-buffer is a fixed point, misses pass
-through, zero a-trous passes leave the temporal result), the reference's own behavioural tests
+through, zero a-trous passes leave the temporal result, an a-trous pass with unit edge weights is the
+dilated B3-spline convolution, computed independently with scipy), the reference's own behavioural tests
 (test_gpu.cpp:105-164: frame-1 error below half the 1-spp error, later frames better, finite,
 non-negative, mean within 5%, stable under camera motion) and the temporal bookkeeping."""
 import numpy as np
@@ -248,3 +249,74 @@ def test_compact_gbuffer_of_float4_planes():
     hit = c["material"] >= 0
     dec = np.array([O.oct_decode(v) for v in c["normal"][hit]])
     assert np.max(np.abs(dec - nd[..., :3][hit])) < 2e-3
+
+
+def _b3_taps(step):
+    """The a-trous footprint as a dense kernel: the B3 spline (1/16, 1/4, 3/8, 1/4, 1/16) in x and y, taps
+    `step` pixels apart."""
+    k1 = np.array([1 / 16, 1 / 4, 3 / 8, 1 / 4, 1 / 16])
+    k = np.zeros((4 * step + 1, 4 * step + 1))
+    k[::step, ::step] = np.outer(k1, k1)
+    return k
+
+
+def _unit_weight_gbuffer(w, h, material):
+    """A G-buffer whose edge-stopping weights are all 1: one normal (n.n = 1, so the normal weight is 1 for
+    any exponent) and one depth (zero gradient, zero depth difference)."""
+    nd = np.zeros((h, w, 4), np.float32)
+    nd[..., 2] = 1
+    nd[..., 3] = 5.0
+    pos = np.zeros((h, w, 4), np.float32)
+    pos[..., 3] = material
+    return nd, pos
+
+
+def test_atrous_with_unit_edge_weights_is_the_dilated_b3_convolution():
+    """With every edge-stopping weight 1 (flat G-buffer, colour sigma ~1e30 so the luminance weight is
+    exp(-1e-30) = 1), an a-trous pass is the normalised B3-spline convolution at its step: colour =
+    sum k x / sum k, variance = sum k^2 v / (sum k)^2, over the taps inside the image. Computed here with
+    scipy's correlate (zero outside the image, so the same correlate of ones gives the in-image weight)."""
+    from scipy.ndimage import correlate
+    w, h = 37, 29
+    rng = np.random.default_rng(7)
+    src = rng.random((h, w, 4)).astype(np.float32)
+    src[..., 3] = 0.1 + 0.9 * src[..., 3]  # variances away from 0, so sigma_l is huge
+    nd, pos = _unit_weight_gbuffer(w, h, 0.0)
+    f = O.SvgfOracle(w, h)
+    for step in (1, 2, 4, 8):
+        out = f.atrous(nd, pos, src, step, 1e30, 128.0, 1.0)
+        k = _b3_taps(step)
+        den = correlate(np.ones((h, w)), k, mode="constant", cval=0.0)
+        for c in range(3):
+            want = correlate(src[..., c].astype(np.float64), k, mode="constant", cval=0.0) / den
+            np.testing.assert_allclose(out[..., c], want, rtol=2e-6, atol=0)
+        want_v = correlate(src[..., 3].astype(np.float64), k * k, mode="constant", cval=0.0) / (den * den)
+        np.testing.assert_allclose(out[..., 3], want_v, rtol=2e-6, atol=0)
+
+
+def test_atrous_taps_only_the_same_material_and_passes_misses_through():
+    """Material ids partition the taps: each pixel's result is the normalised B3 convolution restricted to
+    the pixels of its own material (a masked correlate per id), and misses (id < 0) keep their value."""
+    from scipy.ndimage import correlate
+    w, h = 33, 27
+    rng = np.random.default_rng(8)
+    src = rng.random((h, w, 4)).astype(np.float32)
+    src[..., 3] = 0.1 + 0.9 * src[..., 3]
+    mat = rng.integers(-1, 3, size=(h, w)).astype(np.float32)  # ids -1 (miss), 0, 1, 2 scattered
+    mat[10:20, 5:25] = 1  # and one large block
+    nd, pos = _unit_weight_gbuffer(w, h, mat)
+    f = O.SvgfOracle(w, h)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        for step in (1, 2, 4):
+            out = f.atrous(nd, pos, src, step, 1e30, 128.0, 1.0)
+            k = _b3_taps(step)
+            for m in (0.0, 1.0, 2.0):  # den > 0 wherever mat == m (the centre tap); elsewhere 0/0 is not compared
+                mask = (mat == m).astype(np.float64)
+                den = correlate(mask, k, mode="constant", cval=0.0)
+                sel = mat == m
+                for c in range(3):
+                    num = correlate(src[..., c] * mask, k, mode="constant", cval=0.0)
+                    np.testing.assert_allclose(out[..., c][sel], (num / den)[sel], rtol=2e-6, atol=0)
+                num_v = correlate(src[..., 3] * mask, k * k, mode="constant", cval=0.0)
+                np.testing.assert_allclose(out[..., 3][sel], (num_v / (den * den))[sel], rtol=2e-6, atol=0)
+            assert np.array_equal(out[mat < 0], src[mat < 0])
