@@ -1,7 +1,8 @@
 """Pins of the SVGF restatement (the reference has no CPU version, so these are the checks that fix it):
 special cases with closed answers (a constant image on a constant G-buffer is a fixed point, misses pass
 through, zero a-trous passes leave the temporal result, an a-trous pass with unit edge weights is the
-dilated B3-spline convolution, computed independently with scipy), the reference's own behavioural tests
+dilated B3-spline convolution and the short-history variance estimate a 7x7 box-moment variance, both
+computed independently with scipy), the reference's own behavioural tests
 (test_gpu.cpp:105-164: frame-1 error below half the 1-spp error, later frames better, finite,
 non-negative, mean within 5%, stable under camera motion) and the temporal bookkeeping."""
 import numpy as np
@@ -320,3 +321,47 @@ def test_atrous_taps_only_the_same_material_and_passes_misses_through():
                 num_v = correlate(src[..., 3] * mask, k * k, mode="constant", cval=0.0)
                 np.testing.assert_allclose(out[..., 3][sel], (num_v / (den * den))[sel], rtol=2e-6, atol=0)
             assert np.array_equal(out[mat < 0], src[mat < 0])
+
+
+def test_variance_estimate_with_unit_weights_is_the_7x7_box_moment_variance():
+    """While the history is shorter than 4 frames, the variance pass replaces each pixel by a 7x7 bilateral
+    mean and estimates variance spatially. With unit edge weights (flat G-buffer, one material) that is a
+    box filter over the in-image pixels: colour = box(illum) / box(1), variance = max(0, box(l^2)/n -
+    (box(l)/n)^2) * 4 / history length, from the frame's own illumination and moments planes. Pixels
+    whose history has reached 4 keep their temporal colour and variance."""
+    from scipy.ndimage import correlate
+    w, h = 31, 23
+    rng = np.random.default_rng(9)
+    box = np.ones((7, 7))
+    f = O.SvgfOracle(w, h)
+    n = correlate(np.ones((h, w)), box, mode="constant", cval=0.0)
+    # A still camera and points 10 units from it inside its view: every pixel reprojects onto itself, so
+    # the history grows by one a frame, while depth (10) and normal stay the same everywhere.
+    cam = CameraView()
+    cam.position[:] = (0, 0, -10)
+    cam.forward[:] = (0, 0, 1)
+    cam.right[:] = (1, 0, 0)
+    cam.up[:] = (0, 1, 0)
+    cam.tan_half_fov, cam.aspect = 0.5, w / h
+    ys, xs = np.meshgrid(np.arange(h), np.arange(w), indexing="ij")
+    d = np.stack([(2 * (xs + 0.5) / w - 1) * 0.4, (1 - 2 * (ys + 0.5) / h) * 0.4, np.ones((h, w))], -1)
+    d /= np.linalg.norm(d, axis=-1, keepdims=True)
+    for frame in range(1, 6):
+        color, albedo, nd, pos, emission = flat_gbuffer(w, h, (0, 0, 0), depth=10.0)
+        pos[..., :3] = np.array(cam.position[:]) + 10.0 * d
+        color[..., :3] = rng.random((h, w, 3))
+        f.run(color, albedo, nd, pos, emission, cam, SvgfSettings.make(atrous_iterations=0))
+        illum, moments, est = f.plane(4), f.plane(5), f.plane(6)
+        length = illum[..., 3]
+        np.testing.assert_allclose(length, frame, rtol=1e-6)  # bilinear history weights may round a hair below
+        spatial = length < 4.0
+        assert spatial.all() if frame < 4 else (~spatial).any()
+        for c in range(3):
+            want = correlate(illum[..., c].astype(np.float64), box, mode="constant", cval=0.0) / n
+            np.testing.assert_allclose(est[..., c][spatial], want[spatial], rtol=2e-6, atol=0)
+        m1 = correlate(moments[..., 0].astype(np.float64), box, mode="constant", cval=0.0) / n
+        m2 = correlate(moments[..., 1].astype(np.float64), box, mode="constant", cval=0.0) / n
+        want_v = np.maximum(0.0, m2 - m1 * m1) * 4.0 / length
+        np.testing.assert_allclose(est[..., 3][spatial], want_v[spatial], rtol=1e-4, atol=1e-7)
+        assert np.array_equal(est[..., :3][~spatial], illum[..., :3][~spatial])
+        assert np.array_equal(est[..., 3][~spatial], moments[..., 2][~spatial])
