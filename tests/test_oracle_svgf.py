@@ -1,8 +1,9 @@
 """Pins of the SVGF restatement (the reference has no CPU version, so these are the checks that fix it):
 special cases with closed answers (a constant image on a constant G-buffer is a fixed point, misses pass
 through, zero a-trous passes leave the temporal result, an a-trous pass with unit edge weights is the
-dilated B3-spline convolution and the short-history variance estimate a 7x7 box-moment variance, both
-computed independently with scipy), the reference's own behavioural tests
+dilated B3-spline convolution, the short-history variance estimate a 7x7 box-moment variance, and the
+temporal pass over the first 1/alpha frames a running mean, all computed independently with scipy and
+numpy), the reference's own behavioural tests
 (test_gpu.cpp:105-164: frame-1 error below half the 1-spp error, later frames better, finite,
 non-negative, mean within 5%, stable under camera motion) and the temporal bookkeeping."""
 import numpy as np
@@ -365,3 +366,47 @@ def test_variance_estimate_with_unit_weights_is_the_7x7_box_moment_variance():
         np.testing.assert_allclose(est[..., 3][spatial], want_v[spatial], rtol=1e-4, atol=1e-7)
         assert np.array_equal(est[..., :3][~spatial], illum[..., :3][~spatial])
         assert np.array_equal(est[..., 3][~spatial], moments[..., 2][~spatial])
+
+
+def test_temporal_accumulation_is_the_running_mean_while_the_history_is_short():
+    """With a still camera whose pixels reproject onto their own centres, the temporal pass blends history
+    and the new frame with weight max(alpha, 1/length). For the first 1/alpha frames that is the running
+    mean of the demodulated inputs, the moments are the running means of l and l^2, and the temporal
+    variance is the population variance of the luminances so far (numpy's mean and var here). The history
+    keeps the spatially filtered illumination; a material id of its own per pixel leaves the 7x7 filter
+    only the centre tap, so the filter is the identity and the running mean is exact."""
+    w, h = 20, 14
+    rng = np.random.default_rng(10)
+    cam = CameraView()
+    cam.position[:] = (1, 2, -10)
+    cam.forward[:] = (0, 0, 1)
+    cam.right[:] = (1, 0, 0)
+    cam.up[:] = (0, 1, 0)
+    cam.tan_half_fov, cam.aspect = 0.5, w / h
+    ys, xs = np.meshgrid(np.arange(h), np.arange(w), indexing="ij")
+    sx = 2 * (xs + 0.5) / w - 1
+    sy = 1 - 2 * (ys + 0.5) / h
+    d = np.stack([sx * 0.5 * (w / h), sy * 0.5, np.ones((h, w))], -1)  # through each pixel centre
+    t = 5 + 3 * rng.random((h, w, 1))  # points at varied depths along those rays
+    p = np.array(cam.position[:]) + t * d
+    alpha = 0.2
+    settings = SvgfSettings.make(atrous_iterations=0, alpha_color=alpha, alpha_moments=alpha)
+    f = O.SvgfOracle(w, h)
+    seen = []
+    for frame in range(1, int(round(1 / alpha)) + 1):
+        color, albedo, nd, pos, emission = flat_gbuffer(w, h, (0, 0, 0))
+        pos[..., :3] = p
+        pos[..., 3] = np.arange(h * w).reshape(h, w)
+        nd[..., 3] = np.linalg.norm(p - np.array(cam.position[:]), axis=-1)
+        color[..., :3] = rng.random((h, w, 3))
+        albedo[..., :3] = 0.25 + 0.75 * rng.random((h, w, 3))
+        f.run(color, albedo, nd, pos, emission, cam, settings)
+        seen.append(color[..., :3].astype(np.float64) / albedo[..., :3])
+        x = np.stack(seen)
+        lum_x = 0.2126 * x[..., 0] + 0.7152 * x[..., 1] + 0.0722 * x[..., 2]
+        illum, moments = f.plane(4), f.plane(5)
+        np.testing.assert_allclose(illum[..., 3], frame, rtol=1e-6)
+        np.testing.assert_allclose(illum[..., :3], x.mean(0), rtol=1e-5)
+        np.testing.assert_allclose(moments[..., 0], lum_x.mean(0), rtol=1e-5)
+        np.testing.assert_allclose(moments[..., 1], (lum_x ** 2).mean(0), rtol=1e-5)
+        np.testing.assert_allclose(moments[..., 2], lum_x.var(0), rtol=2e-3, atol=1e-6)
