@@ -153,6 +153,12 @@ int ph_device_memset(void* dev, int value, int64_t bytes);
  * registers it with GL interop: gpu_renderer.cu:129-175). No counterpart in the reference. */
 int ph_host_alloc(int64_t bytes, void** out_host);
 int ph_host_free(void* host);
+/* What kind of memory a pointer is, as kernels see it: PH_POINTER_DEVICE, PH_POINTER_PINNED_HOST,
+ * PH_POINTER_MANAGED, or PH_POINTER_UNUSABLE (NULL, pageable host memory, or unknown). Entry points
+ * that take caller targets refuse unusable pointers with this test before any launch. No counterpart in
+ * the reference (its display target is documented as device memory only, gpu_renderer.h:56). */
+enum { PH_POINTER_UNUSABLE = 0, PH_POINTER_DEVICE = 1, PH_POINTER_PINNED_HOST = 2, PH_POINTER_MANAGED = 3 };
+int ph_pointer_kind(const void* p, int* kind);
 
 typedef struct ph_renderer ph_renderer;
 int ph_renderer_create(ph_renderer** out);
@@ -164,7 +170,9 @@ int ph_renderer_reset(ph_renderer* r);
 int ph_renderer_render(ph_renderer* r, const ph_camera_view* camera, const ph_render_settings* settings, int spp);
 int ph_renderer_render_denoised(ph_renderer* r, const ph_camera_view* camera, const ph_camera_view* previous, const ph_render_settings* settings,
                                 const ph_svgf_settings* svgf);
-/* target: NULL (the renderer's display buffer), caller device memory, or pinned host memory from ph_host_alloc. */
+/* target: NULL (the renderer's display buffer), caller device memory, or pinned host memory from ph_host_alloc.
+ * Any other target (pageable host memory) fails with a message before a launch, and so does a device
+ * allocation smaller than width*height*4 bytes. */
 int ph_renderer_display(ph_renderer* r, const ph_render_settings* settings, int mode, uint32_t* target_dev);
 int ph_renderer_download_image(ph_renderer* r, float* rgb);
 int ph_renderer_download_display(ph_renderer* r, uint32_t* rgba);

@@ -166,6 +166,42 @@ int ph_host_alloc(int64_t bytes, void** out) {
     });
 }
 
+int ph_pointer_kind(const void* p, int* kind) {
+    if (!kind) return fail("null argument");
+    *kind = PH_POINTER_UNUSABLE;
+    if (!p) return 0;
+    hipPointerAttribute_t a{};
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();  // an unknown (pageable) pointer: clear the sticky error it leaves
+        return 0;
+    }
+    if (a.type == hipMemoryTypeDevice) *kind = PH_POINTER_DEVICE;
+    else if (a.type == hipMemoryTypeHost && a.devicePointer) *kind = PH_POINTER_PINNED_HOST;
+    else if (a.type == hipMemoryTypeManaged) *kind = PH_POINTER_MANAGED;
+    return 0;
+}
+
+namespace {
+// A caller's target of `bytes` bytes that kernels may write: device, pinned host or managed memory,
+// and for device memory, an allocation that holds all of it. Anything else (a pageable host pointer)
+// is refused before a launch could fault on it.
+void check_target(const void* p, size_t bytes, const char* what) {
+    int kind = PH_POINTER_UNUSABLE;
+    if (ph_pointer_kind(p, &kind) != 0 || kind == PH_POINTER_UNUSABLE)
+        throw std::runtime_error(std::string(what) + " must be device memory or pinned host memory (ph_host_alloc)");
+    if (kind == PH_POINTER_DEVICE) {
+        hipDeviceptr_t base = nullptr;
+        size_t size = 0;
+        if (hipMemGetAddressRange(&base, &size, const_cast<void*>(p)) == hipSuccess) {
+            const size_t off = size_t(static_cast<const char*>(p) - static_cast<const char*>(base));
+            if (off > size || size - off < bytes) throw std::runtime_error(std::string(what) + " is smaller than the frame");
+        } else {
+            (void)hipGetLastError();
+        }
+    }
+}
+}  // namespace
+
 int ph_host_free(void* p) {
     return guarded([&] {
         if (p) PH_CHECK(hipHostFree(p));
@@ -375,6 +411,7 @@ int ph_renderer_display(ph_renderer* r, const ph_render_settings* rs, int mode, 
     return guarded([&] {
         ph_renderer& m = *r;
         if (m.width == 0 || m.height == 0) return 0;
+        if (target) check_target(target, size_t(m.width) * size_t(m.height) * sizeof(uint32_t), "display target");
         if (mode == PH_DISPLAY_HEATMAP && !m.heat_valid && m.uploaded) {
             launch_heatmap(m.stream, m.view, m.last_camera, m.width, m.height, m.heat.ptr);
             m.heat_valid = true;

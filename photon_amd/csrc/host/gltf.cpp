@@ -206,6 +206,25 @@ size_t byte_size(const json::Value& v, double fallback, const char* what) {
     return size_t(d);
 }
 
+// The stride between elements of an accessor: the view's byteStride, which glTF 2.0 bounds to [4, 252]
+// and which cannot be shorter than one element, or the element size when the view has none.
+size_t element_stride(const json::Value& view, size_t elem) {
+    if (!view.has("byteStride")) return elem;
+    const size_t stride = byte_size(view["byteStride"], 0, "byteStride");
+    if (stride < 4 || stride > 252 || stride < elem) fail("invalid byteStride");
+    return stride;
+}
+
+// True when `count` elements of `elem` bytes, `stride` apart from `base`, all lie inside `size` bytes.
+// The arithmetic is checked, so no product or sum can wrap.
+bool span_fits(size_t base, size_t stride, int count, size_t elem, size_t size) {
+    if (count <= 0) return true;
+    size_t last = 0, end = 0;
+    if (__builtin_mul_overflow(stride, size_t(count - 1), &last)) return false;
+    if (__builtin_add_overflow(base, last, &end) || __builtin_add_overflow(end, elem, &end)) return false;
+    return end <= size;
+}
+
 class GltfImport {
 public:
     GltfImport(Scene& s, const std::string& path) : scene_(s), path_(path) {}
@@ -333,8 +352,8 @@ private:
         const Value& view = doc_["bufferViews"][size_t(a["bufferView"].integer())];
         const std::vector<uint8_t>& buf = buffer_of(view);
         const size_t base = byte_size(view["byteOffset"], 0, "byteOffset") + byte_size(a["byteOffset"], 0, "byteOffset");
-        const size_t stride = view.has("byteStride") ? byte_size(view["byteStride"], 0, "byteStride") : size_t(nc * cs);
-        if (count > 0 && base + stride * size_t(count - 1) + size_t(nc * cs) > buf.size()) fail("accessor out of the buffer's range");
+        const size_t stride = element_stride(view, size_t(nc * cs));
+        if (!span_fits(base, stride, count, size_t(nc * cs), buf.size())) fail("accessor out of the buffer's range");
         const int take = std::min(nc, want);
         for (int i = 0; i < count; i++) {
             const uint8_t* p = buf.data() + base + stride * size_t(i);
@@ -379,8 +398,8 @@ private:
         const Value& view = doc_["bufferViews"][size_t(a["bufferView"].integer())];
         const std::vector<uint8_t>& buf = buffer_of(view);
         const size_t base = byte_size(view["byteOffset"], 0, "byteOffset") + byte_size(a["byteOffset"], 0, "byteOffset");
-        const size_t stride = view.has("byteStride") ? byte_size(view["byteStride"], 0, "byteStride") : size_t(cs);
-        if (count > 0 && base + stride * size_t(count - 1) + size_t(cs) > buf.size()) fail("index accessor out of range");
+        const size_t stride = element_stride(view, size_t(cs));
+        if (!span_fits(base, stride, count, size_t(cs), buf.size())) fail("index accessor out of range");
         std::vector<int> out(size_t(std::max(count, 0)));
         for (int i = 0; i < count; i++) {
             const uint8_t* p = buf.data() + base + stride * size_t(i);

@@ -226,7 +226,8 @@ void decode(const uint8_t* d, size_t n, Texture& out) {
                     comps[size_t(k)].id = s[6 + 3 * k];
                     comps[size_t(k)].h = s[7 + 3 * k] >> 4;
                     comps[size_t(k)].v = s[7 + 3 * k] & 15;
-                    comps[size_t(k)].tq = s[8 + 3 * k] & 3;
+                    comps[size_t(k)].tq = s[8 + 3 * k];
+                    if (comps[size_t(k)].tq > 3) fail("bad quantisation table selector");
                     if (comps[size_t(k)].h < 1 || comps[size_t(k)].h > 4 || comps[size_t(k)].v < 1 || comps[size_t(k)].v > 4) fail("bad sampling factors");
                     hmax = std::max(hmax, comps[size_t(k)].h);
                     vmax = std::max(vmax, comps[size_t(k)].v);
@@ -254,7 +255,9 @@ void decode(const uint8_t* d, size_t n, Texture& out) {
             case 0xDB: {
                 size_t p = 0;
                 while (p < sl) {
-                    const int pq = s[p] >> 4, tq = s[p] & 3;
+                    const int pq = s[p] >> 4, tq = s[p] & 15;
+                    if (pq > 1) fail("bad quantisation table precision");
+                    if (tq > 3) fail("bad quantisation table selector");
                     p++;
                     if (p + size_t(pq ? 128 : 64) > sl) fail("truncated quantisation table");
                     for (int k = 0; k < 64; k++) qt[tq][kZigzag[k]] = pq ? (s[p + 2 * size_t(k)] << 8 | s[p + 2 * size_t(k) + 1]) : s[p + size_t(k)];
@@ -266,7 +269,8 @@ void decode(const uint8_t* d, size_t n, Texture& out) {
             case 0xC4: {
                 size_t p = 0;
                 while (p + 17 <= sl) {
-                    const int tc = s[p] >> 4, th = s[p] & 3;
+                    const int tc = s[p] >> 4, th = s[p] & 15;
+                    if (tc > 1 || th > 3) fail("bad Huffman table selector");
                     int total = 0;
                     for (int k = 0; k < 16; k++) total += s[p + 1 + size_t(k)];
                     if (p + 17 + size_t(total) > sl || total > 256) fail("truncated Huffman table");
@@ -293,7 +297,8 @@ void decode(const uint8_t* d, size_t n, Texture& out) {
                         if (cc.id == s[1 + 2 * k]) c = &cc;
                     if (!c) fail("scan names an unknown component");
                     c->td = s[2 + 2 * k] >> 4;
-                    c->ta = s[2 + 2 * k] & 3;
+                    c->ta = s[2 + 2 * k] & 15;
+                    if (c->td > 3 || c->ta > 3) fail("bad Huffman table selector");
                     if (!dc[c->td].present || !ac[c->ta].present || !qt_set[c->tq]) fail("missing table");
                     sc.push_back(c);
                 }

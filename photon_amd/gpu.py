@@ -137,6 +137,7 @@ def lib():
         L.ph_device_free.argtypes = [P]
         L.ph_host_alloc.argtypes = [C.c_int64, C.POINTER(P)]
         L.ph_host_free.argtypes = [P]
+        L.ph_pointer_kind.argtypes = [P, C.POINTER(C.c_int)]
         L.ph_copy_to_device.argtypes = [P, P, C.c_int64]
         L.ph_copy_to_host.argtypes = [P, P, C.c_int64]
         L.ph_device_memset.argtypes = [P, C.c_int, C.c_int64]
@@ -156,6 +157,16 @@ def lib():
         L.ph_server_now_ms.restype = C.c_double
         _sig_done = True
     return L
+
+
+POINTER_UNUSABLE, POINTER_DEVICE, POINTER_PINNED_HOST, POINTER_MANAGED = 0, 1, 2, 3
+
+
+def pointer_kind(ptr):
+    """What memory `ptr` is as kernels see it (POINTER_*); pageable host memory is POINTER_UNUSABLE."""
+    k = C.c_int(0)
+    _check(lib().ph_pointer_kind(ptr, C.byref(k)))
+    return k.value
 
 
 class PinnedArray:

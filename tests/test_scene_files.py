@@ -546,6 +546,58 @@ def test_image_decode_rejects_other_formats():
         host.decode_image(png[:60])
 
 
+
+def _jpeg_segments(data):
+    """(marker, offset of the segment's first payload byte, payload length) of each header segment before
+    the first scan's entropy-coded data."""
+    out, p = [], 2
+    while p + 4 <= len(data):
+        marker, length = data[p + 1], (data[p + 2] << 8) | data[p + 3]
+        out.append((marker, p + 4, length - 2))
+        if marker == 0xDA:
+            break
+        p += 2 + length
+    return out
+
+
+@pytest.mark.parametrize("field", ["scan_dc", "scan_ac", "dht", "dqt", "frame_tq"])
+@pytest.mark.parametrize("selector", [4, 9, 15])
+def test_jpeg_rejects_table_selectors_beyond_three(field, selector):
+    """JPEG names at most four Huffman tables per class and four quantisation tables (T.81 B.2.2-B.2.4):
+    a selector of 4 or more is refused with a message instead of indexing past the decoder's tables."""
+    b = io.BytesIO()
+    Image.fromarray(_jpeg_test_image(16, 16, 5)).save(b, "JPEG", quality=80)
+    data = bytearray(b.getvalue())
+    seg = {m: (o, n) for m, o, n in _jpeg_segments(data)}
+    if field in ("scan_dc", "scan_ac"):
+        o = seg[0xDA][0] + 2  # Cs, then Td/Ta of the scan's first component
+        data[o] = (data[o] & 0x0F) | (selector << 4) if field == "scan_dc" else (data[o] & 0xF0) | selector
+    elif field == "dht":
+        o = seg[0xC4][0]
+        data[o] = (data[o] & 0xF0) | selector
+    elif field == "dqt":
+        o = seg[0xDB][0]
+        data[o] = (data[o] & 0xF0) | selector
+    else:
+        o = seg[0xC0][0] + 6 + 2  # P, Y, X, Nf, then C/HV/Tq of the first component
+        data[o] = selector
+    with pytest.raises(RuntimeError, match="selector"):
+        host.decode_image(bytes(data))
+
+
+def test_gltf_rejects_strides_whose_span_wraps(tmp_path):
+    """An accessor's span stride*(count-1) + element must not wrap around 2^64: a stride beyond glTF's
+    limit of 252 bytes is refused, and so is a span past the buffer's end."""
+    pos = np.zeros(9, np.float32).tobytes()
+    base = {"meshes": [{"primitives": [{"attributes": {"POSITION": 0}}]}], "nodes": [{"mesh": 0}], "scenes": [{"nodes": [0]}]}
+    for i, (stride, count, msg) in enumerate([(2**39, 2**25 + 1, "byteStride"), (253, 3, "byteStride"),
+                                               (8, 3, "byteStride"), (252, 3, "out of the buffer")]):
+        body = dict(base, bufferViews=[{"buffer": 0, "byteLength": 36, "byteStride": stride}],
+                    accessors=[{"bufferView": 0, "componentType": 5126, "count": count, "type": "VEC3"}])
+        with pytest.raises(RuntimeError, match=msg):
+            host.Scene.from_gltf(_write(tmp_path / f"s{i}.gltf", _doc_with_buffer(pos, body)))
+
+
 # ---- PFM and HDR ----------------------------------------------------------------------------------------
 
 def test_pfm_layout_and_round_trip(tmp_path):
