@@ -197,7 +197,8 @@ int ph_renderer_set_counting(ph_renderer* r, int enable);
 int ph_renderer_stats(const ph_renderer* r, ph_render_stats* out);
 /* Device pointers of the frame buffers (4 floats per pixel): 0 accum (rgb sum, w sum of luminance^2),
  * 1 albedo, 2 normal+depth, 3 colour, 4 position+material, 5 emission, 6 denoised; 7 is the heat map
- * (1 float per pixel: BVH nodes visited by the pixel-centre primary ray). */
+ * (1 float per pixel: BVH nodes visited by the pixel-centre primary ray); 8 is the denoiser's variance
+ * plane (.w) that the Variance view of a denoised frame shows (svgf.h:55 variance_buffer). */
 const float* ph_renderer_buffer(const ph_renderer* r, int which);
 
 /* Traversal of a ray buffer against the uploaded scene: closest hit (t, tri, u, v) or any hit

@@ -182,6 +182,63 @@ int or_display_color(const float* sum, int n, int samples, float exposure, int t
     return 0;
 }
 
+// heat_color of gpu_renderer.cu:108-116: a blue -> green -> yellow -> red ramp over [0, 1].
+static V3 heat_ramp(float t) {
+    t = sat(t);
+    const V3 c0 = v3(0.05f, 0.05f, 0.35f), c1 = v3(0.0f, 0.6f, 0.4f), c2 = v3(0.95f, 0.85f, 0.1f), c3 = v3(0.85f, 0.1f, 0.05f);
+    auto mix = [](V3 a, V3 b, float u) { return a + (b - a) * u; };
+    if (t < 1.0f / 3) return mix(c0, c1, t * 3);
+    if (t < 2.0f / 3) return mix(c1, c2, t * 3 - 1);
+    return mix(c2, c3, t * 3 - 2);
+}
+
+// display_kernel of gpu_renderer.cu:129-175, every mode, over the renderer's float4 planes.
+int or_display(int n, int mode, float exposure, int tonemap, int samples, float max_depth, float heat_scale, int denoised,
+               const float* accum4, const float* denoised4, const float* albedo4, const float* nd4, const float* variance4,
+               const float* heat, uint32_t* out) {
+    if (!out || mode < 0 || mode > 5) return -1;
+    const float fs = float(samples);
+    const float inv = samples > 0 ? 1.0f / fs : 0.0f;
+    for (int i = 0; i < n; i++) {
+        V3 c;
+        switch (mode) {
+            case 1: c = v3p(albedo4 + 4 * i); break;
+            case 2: {
+                const float* v = nd4 + 4 * i;
+                c = v[3] > 0 ? v3p(v) * 0.5f + v3s(0.5f) : v3s(0);
+                break;
+            }
+            case 3: {
+                const float d = nd4[4 * i + 3];
+                c = d > 0 ? v3s(1.0f - sat(d / max_depth)) : v3s(0);
+                break;
+            }
+            case 4: {
+                float var;
+                if (denoised) {
+                    var = variance4[4 * i + 3];
+                } else {
+                    const float* a = accum4 + 4 * i;
+                    const float mean = lum(v3p(a)) * inv;
+                    var = fs > 1 ? fmaxf(0.0f, a[3] * inv - mean * mean) / (fs - 1) : 0.0f;
+                }
+                const float t = sqrtf(var);
+                c = v3s(t / (1.0f + t));
+                break;
+            }
+            case 5: c = heat_ramp(heat[i] * heat_scale); break;
+            default:
+                c = denoised ? v3p(denoised4 + 4 * i) : v3p(accum4 + 4 * i) * inv;
+                c = orc::tonemap(c, exposure, tonemap);
+                break;
+        }
+        // The debug views are display values already; packing sRGB-encodes, so they are decoded first.
+        if (mode != 0 && mode != 1) c = v3(srgb_decode(c.x), srgb_decode(c.y), srgb_decode(c.z));
+        out[i] = pack8(c);
+    }
+    return 0;
+}
+
 int or_rng_u32(uint64_t seed, uint64_t stream, int n, uint32_t* out) {
     Pcg r(seed, stream);
     for (int i = 0; i < n; i++) out[i] = r.u32();

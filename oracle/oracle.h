@@ -39,6 +39,13 @@ int or_heatmap(const ph_scene_desc* s, const ph_camera_view* cam, int width, int
 /* display_kernel for the Color mode of a progressive image (gpu_renderer.cu:129-175). */
 int or_display_color(const float* sum, int n, int samples, float exposure, int tonemap, uint32_t* out);
 
+/* display_kernel, every mode (0 colour, 1 albedo, 2 normal, 3 depth, 4 variance, 5 heat map), over
+ * the renderer's planes: accum = (sum of radiance, sum of luminance^2), denoised colour, albedo,
+ * normal+depth, the denoiser's variance plane (.w) and the node-visit counts (gpu_renderer.cu:108-175). */
+int or_display(int n, int mode, float exposure, int tonemap, int samples, float max_depth, float heat_scale, int denoised,
+               const float* accum4, const float* denoised4, const float* albedo4, const float* nd4, const float* variance4,
+               const float* heat, uint32_t* out);
+
 /* Function-level probes (batch form) for the statistical tests. */
 int or_rng_u32(uint64_t seed, uint64_t stream, int n, uint32_t* out);
 int or_rng_float(uint64_t seed, uint64_t stream, int n, float* out);

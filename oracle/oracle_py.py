@@ -40,6 +40,7 @@ def oracle():
         L.or_trace.argtypes = [D, P, C.c_int, C.c_int, P, P, C.c_int]
         L.or_heatmap.argtypes = [D, CV, C.c_int, C.c_int, FP]
         L.or_display_color.argtypes = [FP, C.c_int, C.c_int, C.c_float, C.c_int, P]
+        L.or_display.argtypes = [C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, C.c_float, C.c_float, C.c_int, FP, FP, FP, FP, FP, FP, P]
         L.or_rng_u32.argtypes = [C.c_uint64, C.c_uint64, C.c_int, P]
         L.or_rng_float.argtypes = [C.c_uint64, C.c_uint64, C.c_int, FP]
         L.or_hash_u32.argtypes = [P, C.c_int, P]
@@ -144,6 +145,23 @@ def render_sum(desc, cam, rs, spp, frame0=0, threads=0):
     s = np.zeros(rs.width * rs.height * 3, np.float32)
     assert oracle().or_render(C.byref(desc), C.byref(cam), C.byref(rs), spp, frame0, threads, fptr(s), None, None) == 0
     return s.reshape(rs.height, rs.width, 3)
+
+
+def display(mode, n, exposure=1.0, tonemap=0, samples=1, max_depth=1.0, heat_scale=1.0 / 160.0, denoised=False,
+            accum=None, denoised_rgb=None, albedo=None, nd=None, variance=None, heat=None):
+    """display_kernel's packed RGBA8 pixels for `n` pixels of float4 planes (absent planes read as zero)."""
+    def plane(a, width):
+        if a is None:
+            return np.zeros(n * width, np.float32)
+        a = np.ascontiguousarray(a, np.float32).reshape(-1)
+        assert a.size == n * width
+        return a
+    planes = [plane(accum, 4), plane(denoised_rgb, 4), plane(albedo, 4), plane(nd, 4), plane(variance, 4), plane(heat, 1)]
+    out = np.zeros(n, np.uint32)
+    rc = oracle().or_display(n, mode, exposure, tonemap, samples, max_depth, heat_scale, int(denoised), *[fptr(p) for p in planes],
+                             vptr(out))
+    assert rc == 0
+    return out
 
 
 def render_gbuffer(desc, cam, rs, frame, threads=0):
@@ -326,4 +344,10 @@ def desc_arrays(desc):
     }
     if d.tangents:
         out["tangents"] = arr(d.tangents, d.num_vertices * 4, np.float32)
+    if d.normals:
+        out["normals"] = arr(d.normals, d.num_vertices * 3, np.float32)
+    if d.uvs:
+        out["uvs"] = arr(d.uvs, d.num_vertices * 2, np.float32)
+    if d.punctual and d.num_punctual:
+        out["punctual"] = arr(d.punctual, d.num_punctual, np.uint8, 48).reshape(-1, 48)
     return out

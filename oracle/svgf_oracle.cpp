@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "oracle.h"
@@ -50,11 +51,29 @@ bool reproject(const ph_camera_view& c, V3 p, int w, int h, float& px, float& py
 
 float mixf(float a, float b, float t) { return a + (b - a) * t; }
 
+// Runs body(y) for every row; rows are independent in every pass, so large frames are split over the
+// host's threads in contiguous bands (each pixel's arithmetic is unchanged).
+template <class Body>
+void rows(int h, int w, const Body& body) {
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int nt = int64_t(w) * h < 65536 ? 1 : int(std::min<unsigned>(hw, unsigned(h)));
+    if (nt <= 1) {
+        for (int y = 0; y < h; y++) body(y);
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; t++)
+        pool.emplace_back([&, t] {
+            for (int y = h * t / nt; y < h * (t + 1) / nt; y++) body(y);
+        });
+    for (std::thread& th : pool) th.join();
+}
+
 void temporal(or_svgf& f, const Inputs& in, const ph_camera_view& prev_cam, float alpha_c, float alpha_m) {
     const int w = f.w, h = f.h;
     const V4 *prev_illum = f.plane[0].data(), *prev_moments = f.plane[1].data(), *prev_nd = f.plane[2].data(), *prev_pos = f.plane[3].data();
     V4 *out_illum = f.plane[4].data(), *out_moments = f.plane[5].data();
-    for (int y = 0; y < h; y++)
+    rows(h, w, [&](int y) {
         for (int x = 0; x < w; x++) {
             int i = y * w + x;
             V4 pos = in.pos[i];
@@ -111,13 +130,14 @@ void temporal(or_svgf& f, const Inputs& in, const ph_camera_view& prev_cam, floa
             out_illum[i] = v4(c, len);
             out_moments[i] = v4(m0, m1, variance, 0);
         }
+    });
 }
 
 void variance(or_svgf& f, const Inputs& in, float phi_normal) {
     const int w = f.w, h = f.h;
     const V4 *illum = f.plane[4].data(), *moments = f.plane[5].data();
     V4* out = f.plane[6].data();
-    for (int y = 0; y < h; y++)
+    rows(h, w, [&](int y) {
         for (int x = 0; x < w; x++) {
             int i = y * w + x;
             V4 c = illum[i];
@@ -153,12 +173,13 @@ void variance(or_svgf& f, const Inputs& in, float phi_normal) {
             }
             out[i] = v4(c.xyz(), var);
         }
+    });
 }
 
 void atrous(or_svgf& f, const Inputs& in, int step, float phi_color, float phi_normal, float phi_depth, const V4* src, V4* dst) {
     const int w = f.w, h = f.h;
     const float kernel[3] = {3.0f / 8.0f, 1.0f / 4.0f, 1.0f / 16.0f};
-    for (int y = 0; y < h; y++)
+    rows(h, w, [&](int y) {
         for (int x = 0; x < w; x++) {
             int i = y * w + x;
             V4 center = src[i];
@@ -208,6 +229,7 @@ void atrous(or_svgf& f, const Inputs& in, int step, float phi_color, float phi_n
             else
                 dst[i] = center;
         }
+    });
 }
 
 void history(or_svgf& f, const Inputs& in, const V4* filtered) {

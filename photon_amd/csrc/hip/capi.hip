@@ -533,6 +533,7 @@ int ph_renderer_stats(const ph_renderer* r, ph_render_stats* out) {
 const float* ph_renderer_buffer(const ph_renderer* r, int which) {
     if (!r) return nullptr;
     if (which == 7) return r->heat.ptr;
+    if (which == 8) return r->svgf.plane[6].ptr ? &r->svgf.plane[6].ptr->x : nullptr;
     const DevBuf<float4>* b[] = {&r->accum, &r->albedo, &r->nd, &r->color, &r->position, &r->emission, &r->denoised_buf};
     if (which < 0 || which > 6) return nullptr;
     return b[which]->ptr ? &b[which]->ptr->x : nullptr;

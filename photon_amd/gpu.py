@@ -13,7 +13,7 @@ from . import _lib
 from .types import (CameraView, PathOptions, RenderSettings, SceneDesc, SvgfSettings, DISPLAY_COLOR)
 
 ENGINE_MEGAKERNEL, ENGINE_WAVEFRONT = 0, 1
-BUF_ACCUM, BUF_ALBEDO, BUF_NORMAL_DEPTH, BUF_COLOR, BUF_POSITION, BUF_EMISSION, BUF_DENOISED, BUF_HEAT = range(8)
+BUF_ACCUM, BUF_ALBEDO, BUF_NORMAL_DEPTH, BUF_COLOR, BUF_POSITION, BUF_EMISSION, BUF_DENOISED, BUF_HEAT, BUF_VARIANCE = range(9)
 
 
 class GpuInfo(C.Structure):

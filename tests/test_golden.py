@@ -13,7 +13,7 @@ from photon_amd.types import CameraView, RenderSettings
 
 from golden_scenes import GOLDEN, RENDERS, make_scene, soup_scene
 
-BVH_SCENES = ["cornell-c1", "cornell", "furnace", "soup", "field-small"]
+BVH_SCENES = ["cornell-c1", "cornell", "furnace", "soup", "field-small", "showcase"]
 
 
 def sha(a):

@@ -134,7 +134,7 @@ def test_svgf_workload_gbuffers(field):
     sv = SvgfSettings.make()
     cam0 = field.camera()
     views = []
-    for f in range(2):
+    for f in range(8):  # the history reaches 4 frames (the end of the spatial variance estimate) and beyond
         a = np.deg2rad(0.5 * f)
         p = np.array(cam0["position"]) - np.array(cam0["target"])
         p = np.array([np.cos(a) * p[0] + np.sin(a) * p[2], p[1], -np.sin(a) * p[0] + np.cos(a) * p[2]]) + np.array(cam0["target"])
@@ -158,11 +158,15 @@ def test_svgf_workload_gbuffers(field):
         print(f"svgf 1600x900 frame {k}: {100 * ok.mean():.4f}% within 1e-3")
         assert ok.mean() >= 0.999
         prev = cam
+    long_history = np.mean(ora.plane(4)[..., 3] >= 4)
+    print(f"history of 4 frames or more at the last frame: {100 * long_history:.1f}% of pixels")
+    assert long_history > 0.5
 
 
 def test_svgf_workload_compact_gbuffers(field):
     """The real-time workload in its own G-buffer format: the compact sample traced on the device at
-    1600x900, denoised on the device and by the oracle from the same compact planes, two frames."""
+    1600x900, denoised on the device and by the oracle from the same compact planes, ten frames of the
+    orbit (0.5 degrees a frame), every frame compared."""
     W, H = 1600, 900
     rs = RenderSettings.make(W, H, max_depth=16, rr_depth=3, seed=1)
     sv = SvgfSettings.make()
@@ -174,7 +178,7 @@ def test_svgf_workload_compact_gbuffers(field):
     gb = gpu.CompactGBuffer(W, H)
     ora = O.SvgfOracle(W, H)
     prev = None
-    for k in range(2):
+    for k in range(10):
         a = np.deg2rad(0.5 * k)
         p = np.array(cam0["position"]) - np.array(cam0["target"])
         p = np.array([np.cos(a) * p[0] + np.sin(a) * p[2], p[1], -np.sin(a) * p[0] + np.cos(a) * p[2]]) + np.array(cam0["target"])
@@ -190,5 +194,6 @@ def test_svgf_workload_compact_gbuffers(field):
         print(f"compact svgf 1600x900 frame {k}: {100 * ok.mean():.4f}% within 1e-3")
         assert ok.mean() >= 0.999
         prev = cam
+    assert np.mean(ora.plane(4)[..., 3] >= 4) > 0.5
     field.set_camera(cam0["position"], cam0["target"], cam0["up"], cam0["vfov"])
     r.close()
