@@ -8,7 +8,9 @@ complete 1080p/64-spp render with the scene resident in HBM. Alongside it:
   * roofline of the dominant kernel (closest-hit traversal): algorithmic bytes per launch from the
     node/triangle/alpha visit counts of an instrumented pass over the same deterministic work, divided
     by the launch duration measured with HIP events on the library's stream;
-  * cpu_baseline: the reference CPU renderer (or the oracle, bit-identical to it) on a bounded sample.
+  * cpu_baseline: the oracle (the CPU restatement, bit-identical to the reference CpuRenderer on the
+    committed fixtures) on a bounded sample, with the host's lscpu core and thread counts; CPU baselines
+    of the Cornell, traversal and SVGF legs sit in those legs' "cpu" entries.
 """
 import argparse
 import json
@@ -41,8 +43,50 @@ SVGF_BYTES_PER_PIXEL = 11 * 16 + 5 * 16 + 5 * 4 * 16 + 9 * 16 + 6 * 16
 SVGF_COMPACT_BYTES_PER_PIXEL = 124 + 60 + 5 * 44 + 104 + 72
 
 
+# Algorithmic bytes of a path vertex beyond its rays (per sample: the scope table's B_sample): a shading
+# vertex reads its triangle (48), indices (12), normals (36), uvs (24) and material (80); a light sample
+# reads an alias entry (16), the emissive record (8), the emitter's triangle (48) and tri_emissive (4); the
+# accumulation reads and writes 16 B per pixel per render call.
+B_SHADE, B_NEE, B_ACC = 48 + 12 + 36 + 24 + 80, 16 + 8 + 48 + 4, 32
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_threads():
+    """Host threads for the CPU baselines: the CPUs this process may run on, capped by OMP_NUM_THREADS
+    (the host-core share a job on a shared box is given)."""
+    n = len(os.sched_getaffinity(0))
+    omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return min(n, omp) if omp > 0 else n
+
+
+def host_cpu_info():
+    """The host's cores and threads as lscpu reports them, and the CPUs this process may use."""
+    import subprocess
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)"):
+                info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    info["usable_cpus"] = len(os.sched_getaffinity(0))
+    info["threads_used"] = cpu_threads()
+    return info
+
+
+def sample_bytes(cs, pixels):
+    """Algorithmic bytes of one render call from its instrumented counts: every traced ray (extension and
+    shadow, with their node, triangle and alpha tests), a shading vertex per extension ray (an upper
+    bound: a miss reads one environment texel instead) and a light sample per shadow ray, plus the
+    accumulation."""
+    rays = traversal_bytes(cs.rays_extend, cs.node_visits, cs.tri_tests, cs.alpha_tests)
+    rays += traversal_bytes(cs.rays_shadow, cs.shadow_node_visits, cs.shadow_tri_tests, cs.shadow_alpha_tests)
+    return rays + cs.rays_extend * B_SHADE + cs.rays_shadow * B_NEE + pixels * B_ACC
 
 
 def traversal_bytes(rays, nodes, tris, alpha):
@@ -110,7 +154,9 @@ def bench_render(args, gpu, host, RenderSettings):
     avg_launch_ms = ext_ms / max(1, ext_launches)
     bytes_per_launch = ext_bytes / max(1, cs.extend_launches)
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9 if avg_launch_ms > 0 else 0.0
+    b_sample = sample_bytes(cs, W * H) / samples
     out = dict(value=value, ms_per_step=total / args.steps * 1e3, samples=samples, config=cfg.as_dict() if cfg else r.wavefront_config().as_dict(),
+               bytes_per_sample=b_sample, sample_achieved_gbs=b_sample * value * 1e6 / 1e9,
                rays_extend=rays_ext, rays_shadow=rays_sh, rays_per_sample=(rays_ext + rays_sh) / samples,
                mrays_s=(rays_ext + rays_sh) * args.steps / total / 1e6,
                extend_ms_total=ext_ms, shadow_ms_total=shadow_ms,
@@ -160,7 +206,7 @@ def bench_display(gpu, r, rs, W, H, reps=20):
     return res
 
 
-def bench_traversal(gpu, host):
+def bench_traversal(gpu, host, cpu=False):
     import numpy as np
     from photon_amd.types import HIT_DTYPE
     t = time.time()
@@ -185,12 +231,84 @@ def bench_traversal(gpu, host):
                 if best is None or ms < best[0]:
                     best = (ms, block, depth, top)
         res[name] = dict(mrays_s=len(rays) / (best[0] * 1e-3) / 1e6, ms=best[0], block=best[1], stack_depth=best[2], lds_nodes=best[3])
+        # Per-ray roofline: algorithmic bytes from the instrumented traversal of the same rays.
+        cd = gpu.DeviceArray(len(rays) * 12)
+        r.trace(rd.ptr, len(rays), hd.ptr, anyhit=anyhit, counts_dev=cd.ptr)
+        c = cd.numpy(np.uint32, (len(rays), 3)).astype(np.int64).sum(0)
+        nbytes = traversal_bytes(len(rays), int(c[0]), int(c[1]), int(c[2]))
+        res[name].update(bytes_per_ray=round(nbytes / len(rays), 1), achieved_gbs=round(nbytes / (best[0] * 1e-3) / 1e9, 1),
+                         roofline_frac=round(nbytes / (best[0] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                         node_visits_per_ray=round(c[0] / len(rays), 2), tri_tests_per_ray=round(c[1] / len(rays), 2))
+        cd.free()
         log(f"traversal {name}: {res[name]}")
     r.close()
+    if cpu:
+        # The oracle's multithreaded traversal of the same rays (CPU baseline of the traversal workload).
+        from oracle import oracle_py as O
+        threads = cpu_threads()
+        for name, rays, anyhit in (("closest", cam_rays, False), ("any", shadow, True)):
+            O.trace(scene.desc(), rays[:65536], anyhit=anyhit, threads=threads)
+            t0 = time.perf_counter()
+            O.trace(scene.desc(), rays, anyhit=anyhit, threads=threads)
+            dt = time.perf_counter() - t0
+            res[name]["cpu_mrays_s"] = round(len(rays) / dt / 1e6, 2)
+        res["cpu"] = dict(kind="port", threads=threads, sample="the same 4,194,304 + 4,194,304 rays, oracle traversal on all threads")
+        log(f"traversal CPU: closest {res['closest']['cpu_mrays_s']} Mrays/s, any {res['any']['cpu_mrays_s']} Mrays/s on {threads} threads")
     return res
 
 
-def bench_svgf(args, gpu, host, field):
+def bench_cornell(args, gpu, host, cpu):
+    """The plain-method workload: the 36-triangle Cornell box at 512x512, 16 spp, depth 8, seed 1, on the
+    wavefront engine and the per-pixel kernel, with its per-sample roofline; and the oracle on the same
+    render on the host's threads."""
+    from photon_amd.types import RenderSettings
+    W = H = 512
+    SPP = 16
+    scene = host.Scene.builtin("cornell-c1").build()
+    rs = RenderSettings.make(W, H, max_depth=8, rr_depth=3, seed=1)
+    cam = scene.camera_view(1.0)
+    r = gpu.GpuRenderer()
+    ok, err = r.upload(scene)
+    assert ok, err
+    r.resize(W, H)
+    res = dict(workload="Cornell box, 36 triangles, 512x512, 16 spp, max depth 8, seed 1")
+    for engine, key in ((gpu.ENGINE_WAVEFRONT, "wavefront"), (gpu.ENGINE_MEGAKERNEL, "per_pixel")):
+        r.set_engine(engine)
+        if engine == gpu.ENGINE_WAVEFRONT and not args.no_autotune:
+            r.autotune(cam, rs, SPP)
+        best = None
+        for _ in range(5):
+            r.reset()
+            t0 = time.perf_counter()
+            r.render(cam, rs, SPP)
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        res[key + "_msamples_s"] = round(W * H * SPP / best / 1e6, 1)
+    r.set_engine(gpu.ENGINE_WAVEFRONT)
+    r.reset()
+    r.set_counting(True)
+    r.render(cam, rs, SPP)
+    cs = r.stats()
+    r.set_counting(False)
+    r.close()
+    b = sample_bytes(cs, W * H) / (W * H * SPP)
+    res.update(bytes_per_sample=round(b, 1), rays_per_sample=round((cs.rays_extend + cs.rays_shadow) / (W * H * SPP), 3),
+               achieved_gbs=round(b * res["wavefront_msamples_s"] * 1e6 / 1e9, 1))
+    res["roofline_frac"] = round(res["achieved_gbs"] / HBM_PEAK_GBS, 4)
+    if cpu:
+        from oracle import oracle_py as O
+        threads = cpu_threads()
+        O.render(scene.desc(), cam, rs, 1, threads=threads)  # 1-spp warm-up, as the reference's bench does
+        t0 = time.perf_counter()
+        O.render(scene.desc(), cam, rs, SPP, threads=threads)
+        dt = time.perf_counter() - t0
+        res["cpu"] = dict(value=round(W * H * SPP / dt / 1e6, 2), unit="Msamples/s", kind="port", threads=threads,
+                          sample=f"the full render, {dt:.2f} s wall")
+    log(f"cornell: {res}")
+    return res
+
+
+def bench_svgf(args, gpu, host, field, cpu=False):
     """Real-time sequence at 1600x900, camera orbiting 0.5 deg/frame: a 1-spp sample into the workload's
     compact G-buffer, then SVGF on it, per frame; then the reference's float4 G-buffer path
     (render_denoised) on the same sequence for comparison."""
@@ -229,6 +347,24 @@ def bench_svgf(args, gpu, host, field):
         svgf_ms.append(f.last_ms())
         wall.append((time.perf_counter() - t0) * 1e3)
         prev = cur
+    cpu_ms = None
+    if cpu:
+        # The oracle's denoiser (row-parallel passes) on the last frames' compact G-buffers.
+        from oracle import oracle_py as O
+        ora = O.SvgfOracle(W, H)
+        frames, prev_c = [], view(0)
+        for k in range(4):
+            cur = view(k)
+            r.sample_compact(cur, rs, gb)
+            frames.append((cur, prev_c, gb.numpy()))
+            prev_c = cur
+        times = []
+        for cur, prev_c, g in frames:
+            t0 = time.perf_counter()
+            ora.run_compact(g, cur, prev_c, sv)
+            times.append((time.perf_counter() - t0) * 1e3)
+        cpu_ms = dict(value=round(sum(times[1:]) / 3, 2), unit="ms/frame", kind="port", threads=cpu_threads(),
+                      sample="4 frames of the orbit at 1600x900 on the device-traced compact G-buffers, the last 3 timed")
     # The reference's format: render_denoised (float4 G-buffer) over the same orbit.
     ref_ms = []
     prev = view(0)
@@ -246,6 +382,9 @@ def bench_svgf(args, gpu, host, field):
                svgf_bytes_per_pixel=SVGF_COMPACT_BYTES_PER_PIXEL, float4_svgf_ms=round(mean(ref_ms), 4), float4_bytes_per_pixel=SVGF_BYTES_PER_PIXEL)
     res["svgf_achieved_gbs"] = round(W * H * SVGF_COMPACT_BYTES_PER_PIXEL / (res["svgf_ms"] * 1e-3) / 1e9, 1)
     res["float4_achieved_gbs"] = round(W * H * SVGF_BYTES_PER_PIXEL / (res["float4_svgf_ms"] * 1e-3) / 1e9, 1)
+    res["roofline_frac"] = round(res["svgf_achieved_gbs"] / HBM_PEAK_GBS, 4)
+    if cpu_ms:
+        res["cpu"] = cpu_ms
     log(f"svgf: {res}")
     return res
 
@@ -437,25 +576,19 @@ def measured_traffic(spp, launch):
 
 
 def cpu_baseline(scene, cam, rs, spp):
-    """The reference CPU renderer (or the oracle) on the full 1080p frame at `spp` samples with all host threads:
-    a bounded sample of the workload (about 20 s of CPU time at 4 spp on 16 threads)."""
+    """The oracle (the repository's CPU restatement, bit-identical to the reference CpuRenderer on the
+    committed fixtures) on the full 1080p frame at `spp` samples with the host's threads: a bounded sample
+    of the workload (about 20 s of CPU time at 4 spp on 16 threads), after a 1-spp warm-up as the
+    reference's bench does (render_cli.cpp:319-327)."""
     from oracle import oracle_py as O
-    threads = os.cpu_count() or 1
-    kind = "reference" if O.ref_available() else "port"
-    if kind == "reference":
-        ref = O.RefScene(raw=scene.raw())
-        ref.build()
-        ref.render(cam, rs, 1, threads)  # warm-up as the reference bench does (render_cli.cpp:319-327)
-        t0 = time.perf_counter()
-        ref.render(cam, rs, spp, threads)
-        dt = time.perf_counter() - t0
-    else:
-        O.render(scene.desc(), cam, rs, 1, threads=threads)
-        t0 = time.perf_counter()
-        O.render(scene.desc(), cam, rs, spp, threads=threads)
-        dt = time.perf_counter() - t0
-    return dict(value=rs.width * rs.height * spp / dt / 1e6, unit="Msamples/s", cores=threads, kind=kind,
-                sample=f"{rs.width}x{rs.height} at {spp} spp, max depth {rs.path.max_depth}, same scene and sequences; {dt:.2f} s wall")
+    threads = cpu_threads()
+    O.render(scene.desc(), cam, rs, 1, threads=threads)
+    t0 = time.perf_counter()
+    O.render(scene.desc(), cam, rs, spp, threads=threads)
+    dt = time.perf_counter() - t0
+    return dict(value=rs.width * rs.height * spp / dt / 1e6, unit="Msamples/s", cores=threads, kind="port",
+                sample=f"{rs.width}x{rs.height} at {spp} spp, max depth {rs.path.max_depth}, same scene and sequences; {dt:.2f} s wall",
+                host=host_cpu_info())
 
 
 def main():
@@ -468,6 +601,7 @@ def main():
     ap.add_argument("--no-megakernel", action="store_true")
     ap.add_argument("--megakernel-divisor", type=int, default=1, help="time the plain kernel on spp/divisor samples")
     ap.add_argument("--no-traversal", action="store_true")
+    ap.add_argument("--no-cornell", action="store_true", help="skip the 512x512 Cornell box leg")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-spp", type=int, default=4, help="samples per pixel of the CPU baseline's bounded sample")
     ap.add_argument("--no-serve", action="store_true")
@@ -510,8 +644,9 @@ def main():
         src.free()
         dst.free()
     scene, cam, rs, rnd = bench_render(args, gpu, host, RenderSettings)
-    trav = None if args.no_traversal else bench_traversal(gpu, host)
-    svgf = None if args.no_svgf else bench_svgf(args, gpu, host, scene)
+    cornell = None if args.no_cornell else bench_cornell(args, gpu, host, not args.no_cpu)
+    trav = None if args.no_traversal else bench_traversal(gpu, host, not args.no_cpu)
+    svgf = None if args.no_svgf else bench_svgf(args, gpu, host, scene, not args.no_cpu)
     serve = None if args.no_serve else bench_serving(args, gpu, host, scene, rnd["config"])
     build = None if args.no_scene_build else bench_scene_build(gpu, host)
     cpu = None if args.no_cpu else cpu_baseline(scene, cam, rs, args.cpu_spp)
@@ -534,6 +669,9 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(rnd["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.spp, rnd["config"]),
                      "bytes_per_launch": int(rnd["extend_bytes_per_launch"]), "avg_launch_ms": round(rnd["extend_avg_launch_ms"], 4)},
         "cpu_baseline": cpu,
+        "sample_roofline": {"bytes_per_sample": round(rnd["bytes_per_sample"], 1), "achieved": round(rnd["sample_achieved_gbs"], 1),
+                            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(rnd["sample_achieved_gbs"] / HBM_PEAK_GBS, 4),
+                            "note": "rays, a shading vertex per extension ray (upper bound) and a light sample per shadow ray, plus accumulation"},
         "mrays_s": round(rnd["mrays_s"], 1),
         "rays_per_sample": round(rnd["rays_per_sample"], 3),
         "node_visits_per_ray": round(rnd["node_visits_per_ray"], 2),
@@ -541,6 +679,7 @@ def main():
         "display": rnd.get("display"),
         "megakernel_msamples_s": round(rnd.get("megakernel_msamples_s", 0.0), 3) or None,
         "speedup_vs_megakernel": round(rnd.get("speedup_vs_megakernel", 0.0), 3) or None,
+        "cornell": cornell,
         "traversal": trav,
         "svgf": svgf,
         "serving": serve,

@@ -27,7 +27,7 @@ PathOpts opts_of(const ph_render_settings* rs) {
 
 template <class F>
 void parallel_rows(int rows, int threads, F&& body) {
-    int n = threads > 0 ? threads : int(std::max(1u, std::thread::hardware_concurrency()));
+    int n = threads > 0 ? threads : default_threads();
     n = std::max(1, std::min(n, rows));
     std::atomic<int> next{0};
     auto work = [&] {

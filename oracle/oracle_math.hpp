@@ -11,9 +11,19 @@
 
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
+#include <thread>
 
 namespace orc {
+
+// Threads of the oracle's parallel loops when the caller asks for the default: OMP_NUM_THREADS when set
+// (the share of host cores a job was given), else every hardware thread.
+inline int default_threads() {
+    const char* e = std::getenv("OMP_NUM_THREADS");
+    const int n = e ? std::atoi(e) : 0;
+    return n > 0 ? n : int(std::thread::hardware_concurrency() ? std::thread::hardware_concurrency() : 1);
+}
 
 constexpr float kPi = 3.14159265358979323846f;
 constexpr float kTwoPi = 6.28318530717958647692f;

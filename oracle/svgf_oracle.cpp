@@ -55,8 +55,7 @@ float mixf(float a, float b, float t) { return a + (b - a) * t; }
 // host's threads in contiguous bands (each pixel's arithmetic is unchanged).
 template <class Body>
 void rows(int h, int w, const Body& body) {
-    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-    const int nt = int64_t(w) * h < 65536 ? 1 : int(std::min<unsigned>(hw, unsigned(h)));
+    const int nt = int64_t(w) * h < 65536 ? 1 : std::min(default_threads(), h);
     if (nt <= 1) {
         for (int y = 0; y < h; y++) body(y);
         return;
