@@ -96,7 +96,7 @@ int phs_scene_add_mesh(phs_scene* s, const float* positions, int num_vertices, c
             for (int i = 0; i < num_vertices; i++) uv[size_t(i)] = vec2(uvs[2 * i], uvs[2 * i + 1]);
         }
         std::vector<int> idx(indices, indices + num_indices);
-        s->scene.add_mesh(pos, idx, material, nrm, uv);
+        append_mesh(s->scene.mesh, pos, idx, material, nrm, uv);
         return 0;
     });
 }
@@ -121,43 +121,42 @@ int phs_scene_get_material(const phs_scene* s, int index, ph_material* out) {
 int phs_scene_add_texture(phs_scene* s, int width, int height, const uint32_t* texels, int repeat, int srgb) {
     if (!s || width <= 0 || height <= 0 || !texels) return fail("invalid texture");
     return guarded([&] {
-        Texture t;
+        Image8 t;
         t.width = width;
         t.height = height;
-        t.texels.assign(texels, texels + size_t(width) * size_t(height));
-        t.repeat = repeat != 0;
+        t.px.assign(texels, texels + size_t(width) * size_t(height));
+        t.tile = repeat != 0;
         t.srgb = srgb != 0;
-        s->scene.textures.push_back(std::move(t));
-        return int(s->scene.textures.size()) - 1;
+        return s->scene.add_image(std::move(t));
     });
 }
 
 int phs_scene_add_punctual(phs_scene* s, const ph_punctual_light* l) {
     if (!s || !l) return fail("null argument");
     return guarded([&] {
-        s->scene.punctual.push_back(*l);
-        return int(s->scene.punctual.size()) - 1;
+        s->scene.lamps.push_back(*l);
+        return int(s->scene.lamps.size()) - 1;
     });
 }
 
 int phs_scene_set_env_color(phs_scene* s, const float rgb[3], float intensity) {
     if (!s || !rgb) return fail("null argument");
-    s->scene.env = Environment();
-    s->scene.env.color = v3(rgb);
-    s->scene.env.intensity = intensity;
+    s->scene.backdrop = Backdrop();
+    s->scene.backdrop.flat = v3(rgb);
+    s->scene.backdrop.gain = intensity;
     return 0;
 }
 
 int phs_scene_set_env_image(phs_scene* s, int width, int height, const float* rgb, float intensity, float rotation) {
     if (!s || width <= 0 || height <= 0 || !rgb) return fail("invalid environment image");
     return guarded([&] {
-        Environment& e = s->scene.env;
+        Backdrop& e = s->scene.backdrop;
         e.width = width;
         e.height = height;
-        e.texels.resize(size_t(width) * size_t(height));
-        for (size_t i = 0; i < e.texels.size(); i++) e.texels[i] = v3(rgb + 3 * i);
-        e.intensity = intensity;
-        e.rotation = rotation;
+        e.rgb.resize(size_t(width) * size_t(height));
+        for (size_t i = 0; i < e.rgb.size(); i++) e.rgb[i] = v3(rgb + 3 * i);
+        e.gain = intensity;
+        e.yaw = rotation;
         return 0;
     });
 }
@@ -165,43 +164,43 @@ int phs_scene_set_env_image(phs_scene* s, int width, int height, const float* rg
 int phs_scene_set_sky(phs_scene* s, const float sun[3], float sun_intensity, int width) {
     if (!s || !sun || width < 2) return fail("invalid argument");
     return guarded([&] {
-        make_sky(s->scene.env, v3(sun), sun_intensity, width);
+        make_sky(s->scene.backdrop, v3(sun), sun_intensity, width);
         return 0;
     });
 }
 
 int phs_scene_clear_normals(phs_scene* s) {
     if (!s) return fail("null argument");
-    s->scene.normals.clear();
+    s->scene.mesh.nrm.clear();
     return 0;
 }
 
 int phs_scene_set_camera(phs_scene* s, const float position[3], const float target[3], const float up[3], float vfov, float aperture,
                          float focus_distance) {
     if (!s || !position || !target || !up) return fail("null argument");
-    Camera& c = s->scene.camera;
-    c.position = v3(position);
-    c.target = v3(target);
-    c.up = v3(up);
-    c.vfov = vfov;
-    c.aperture = aperture;
-    c.focus_distance = focus_distance;
+    Viewpoint& c = s->scene.view;
+    c.eye = v3(position);
+    c.aim = v3(target);
+    c.sky = v3(up);
+    c.fov_deg = vfov;
+    c.lens = aperture;
+    c.focus = focus_distance;
     return 0;
 }
 
 int phs_scene_get_camera(const phs_scene* s, float position[3], float target[3], float up[3], float* vfov) {
     if (!s || !position || !target || !up || !vfov) return fail("null argument");
-    const Camera& c = s->scene.camera;
-    std::memcpy(position, &c.position, 12);
-    std::memcpy(target, &c.target, 12);
-    std::memcpy(up, &c.up, 12);
-    *vfov = c.vfov;
+    const Viewpoint& c = s->scene.view;
+    std::memcpy(position, &c.eye, 12);
+    std::memcpy(target, &c.aim, 12);
+    std::memcpy(up, &c.sky, 12);
+    *vfov = c.fov_deg;
     return 0;
 }
 
 int phs_scene_camera_view(const phs_scene* s, float aspect, ph_camera_view* out) {
     if (!s || !out) return fail("null argument");
-    *out = s->scene.camera.view(aspect);
+    *out = camera_frame(s->scene.view, aspect);
     return 0;
 }
 
@@ -209,101 +208,103 @@ int phs_scene_build(phs_scene* s, int bins, int max_leaf, float traversal_cost, 
     if (!s) return fail("null argument");
     if (bins > 64) return fail("at most 64 bins");
     return guarded([&] {
-        BvhOptions o;
-        if (bins > 0) o.bins = bins;
-        if (max_leaf > 0) o.max_leaf_size = max_leaf;
-        if (traversal_cost > 0) o.traversal_cost = traversal_cost;
-        o.threads = threads > 0 ? threads : 0;
-        s->scene.build(o);
+        BvhSettings o;
+        if (bins > 0) o.bin_count = bins;
+        if (max_leaf > 0) o.leaf_limit = max_leaf;
+        if (traversal_cost > 0) o.visit_cost = traversal_cost;
+        o.workers = threads > 0 ? threads : 0;
+        s->scene.compile(o);
         return 0;
     });
 }
 
 int phs_scene_rebuild_lights(phs_scene* s) {
-    if (!s || !s->scene.built) return fail("the scene is not built");
+    if (!s || !s->scene.out.done) return fail("the scene is not built");
     return guarded([&] {
-        s->scene.rebuild_lights();
+        s->scene.relight();
         return 0;
     });
 }
 
 int phs_scene_desc(const phs_scene* s, ph_scene_desc* out) {
     if (!s || !out) return fail("null argument");
-    if (!s->scene.built) return fail("the scene is not built");
-    *out = s->scene.desc();
+    if (!s->scene.out.done) return fail("the scene is not built");
+    *out = s->scene.describe();
     return 0;
 }
 
-int phs_scene_num_tris(const phs_scene* s) { return s ? s->scene.num_tris() : fail("null argument"); }
+int phs_scene_num_tris(const phs_scene* s) { return s ? s->scene.mesh.triangles() : fail("null argument"); }
 int phs_scene_num_materials(const phs_scene* s) { return s ? int(s->scene.materials.size()) : fail("null argument"); }
-const int32_t* phs_scene_order(const phs_scene* s) { return s && s->scene.built ? s->scene.bvh.order.data() : nullptr; }
+const int32_t* phs_scene_order(const phs_scene* s) { return s && s->scene.out.done ? s->scene.out.tree.order.data() : nullptr; }
 
 int phs_scene_stats(const phs_scene* s, phs_bvh_stats* out) {
     if (!s || !out) return fail("null argument");
-    const BvhStats& st = s->scene.bvh.stats;
+    const BvhReport& st = s->scene.out.tree.report;
     out->nodes = st.nodes;
     out->leaves = st.leaves;
-    out->max_depth = st.max_depth;
-    out->sah_cost = st.sah_cost;
-    out->build_ms = st.build_ms;
+    out->max_depth = st.depth;
+    out->sah_cost = st.sah;
+    out->build_ms = st.ms;
     return 0;
 }
 
 int64_t phs_scene_device_bytes(const phs_scene* s) {
-    if (!s || !s->scene.built) return fail("the scene is not built");
-    const Scene& c = s->scene;
-    int64_t b = 0;
-    b += int64_t(c.bvh.nodes.size()) * 64 + int64_t(c.tris.size()) * 48 + int64_t(c.bvh_indices.size()) * 4;
-    b += int64_t(c.normals.size()) * 12 + int64_t(c.uvs.size()) * 8 + int64_t(c.tangents.size()) * 16;
-    b += int64_t(c.materials.size()) * 80 + int64_t(c.texture_infos.size()) * 20 + int64_t(c.texel_pool.size()) * 4 + 256 * 4;
-    b += int64_t(c.emissive.size()) * 8 + int64_t(c.tri_emissive.size()) * 4 + int64_t(c.emissive_alias.size()) * 16;
-    b += int64_t(c.punctual.size()) * 48 + int64_t(c.env.texels.size()) * 12;
-    b += int64_t(c.env_marginal.size() + c.env_conditional.size()) * 4;
+    if (!s || !s->scene.out.done) return fail("the scene is not built");
+    const ph_scene_desc d = s->scene.describe();
+    const int64_t nv = d.num_vertices, tris = d.num_tris;
+    int64_t b = int64_t(d.num_nodes) * 64 + tris * (48 + 12);
+    b += (d.normals ? nv * 12 : 0) + (d.uvs ? nv * 8 : 0) + (d.tangents ? nv * 16 : 0);
+    b += int64_t(d.num_materials) * 80 + int64_t(d.num_textures) * 20 + d.num_texels * 4 + 256 * 4;
+    b += int64_t(d.num_emissive) * (8 + 16) + (d.tri_emissive ? tris * 4 : 0) + int64_t(d.num_punctual) * 48;
+    if (d.env_texels) b += int64_t(d.env_width) * d.env_height * 12 + (int64_t(d.env_height) + 1 + int64_t(d.env_height) * (d.env_width + 1)) * 4;
     return b;
 }
 
 int phs_scene_raw(phs_scene* s, phs_raw_scene* out) {
     if (!s || !out) return fail("null argument");
     const Scene& c = s->scene;
+    const Mesh& m = c.mesh;
     *out = phs_raw_scene{};
-    out->positions = c.positions.empty() ? nullptr : &c.positions[0].x;
-    out->num_vertices = int(c.positions.size());
-    out->normals = c.normals.empty() ? nullptr : &c.normals[0].x;
-    out->uvs = c.uvs.empty() ? nullptr : &c.uvs[0].x;
-    out->indices = c.indices.data();
-    out->num_tris = c.num_tris();
-    out->tri_material = c.tri_material.data();
+    out->positions = m.pos.empty() ? nullptr : &m.pos[0].x;
+    out->num_vertices = int(m.pos.size());
+    out->normals = m.nrm.empty() ? nullptr : &m.nrm[0].x;
+    out->uvs = m.uv.empty() ? nullptr : &m.uv[0].x;
+    out->tangents = m.tan.empty() ? nullptr : &m.tan[0].x;
+    out->indices = m.corner.data();
+    out->num_tris = m.triangles();
+    out->tri_material = m.mat.data();
     out->materials = c.materials.data();
     out->num_materials = int(c.materials.size());
     s->tex_w.clear(), s->tex_h.clear(), s->tex_repeat.clear(), s->tex_srgb.clear(), s->tex_ptr.clear();
-    for (const Texture& t : c.textures) {
+    for (const Image8& t : c.images) {
         s->tex_w.push_back(t.width);
         s->tex_h.push_back(t.height);
-        s->tex_repeat.push_back(t.repeat ? 1 : 0);
+        s->tex_repeat.push_back(t.tile ? 1 : 0);
         s->tex_srgb.push_back(t.srgb ? 1 : 0);
-        s->tex_ptr.push_back(t.texels.data());
+        s->tex_ptr.push_back(t.px.data());
     }
-    out->num_textures = int(c.textures.size());
+    out->num_textures = int(c.images.size());
     out->tex_width = s->tex_w.data();
     out->tex_height = s->tex_h.data();
     out->tex_texels = s->tex_ptr.data();
     out->tex_repeat = s->tex_repeat.data();
     out->tex_srgb = s->tex_srgb.data();
-    out->punctual = c.punctual.empty() ? nullptr : c.punctual.data();
-    out->num_punctual = int(c.punctual.size());
-    out->env_texels = c.env.texels.empty() ? nullptr : &c.env.texels[0].x;
-    out->env_width = c.env.width;
-    out->env_height = c.env.height;
-    std::memcpy(out->env_color, &c.env.color, 12);
-    out->env_intensity = c.env.intensity;
-    out->env_rotation = c.env.rotation;
-    std::memcpy(out->cam_position, &c.camera.position, 12);
-    std::memcpy(out->cam_target, &c.camera.target, 12);
-    std::memcpy(out->cam_up, &c.camera.up, 12);
-    out->cam_vfov = c.camera.vfov;
-    out->cam_aperture = c.camera.aperture;
-    out->cam_focus = c.camera.focus_distance;
-    out->tangents = c.tangents.empty() ? nullptr : &c.tangents[0].x;
+    out->punctual = c.lamps.empty() ? nullptr : c.lamps.data();
+    out->num_punctual = int(c.lamps.size());
+    const Backdrop& b = c.backdrop;
+    out->env_texels = b.rgb.empty() ? nullptr : &b.rgb[0].x;
+    out->env_width = b.width;
+    out->env_height = b.height;
+    std::memcpy(out->env_color, &b.flat, 12);
+    out->env_intensity = b.gain;
+    out->env_rotation = b.yaw;
+    const Viewpoint& v = c.view;
+    std::memcpy(out->cam_position, &v.eye, 12);
+    std::memcpy(out->cam_target, &v.aim, 12);
+    std::memcpy(out->cam_up, &v.sky, 12);
+    out->cam_vfov = v.fov_deg;
+    out->cam_aperture = v.lens;
+    out->cam_focus = v.focus;
     return 0;
 }
 
@@ -323,28 +324,28 @@ int phs_scene_save_gltf(const phs_scene* s, const char* path) {
     });
 }
 
-int phs_scene_num_cameras(const phs_scene* s) { return s ? int(s->scene.cameras.size()) : fail("null argument"); }
+int phs_scene_num_cameras(const phs_scene* s) { return s ? int(s->scene.file_views.size()) : fail("null argument"); }
 
 int phs_scene_camera_at(const phs_scene* s, int index, float position[3], float target[3], float up[3], float* vfov) {
     if (!s || !position || !target || !up || !vfov) return fail("null argument");
-    if (index < 0 || size_t(index) >= s->scene.cameras.size()) return fail("camera index out of range");
-    const Camera& c = s->scene.cameras[size_t(index)];
-    std::memcpy(position, &c.position, 12);
-    std::memcpy(target, &c.target, 12);
-    std::memcpy(up, &c.up, 12);
-    *vfov = c.vfov;
+    if (index < 0 || size_t(index) >= s->scene.file_views.size()) return fail("camera index out of range");
+    const Viewpoint& c = s->scene.file_views[size_t(index)];
+    std::memcpy(position, &c.eye, 12);
+    std::memcpy(target, &c.aim, 12);
+    std::memcpy(up, &c.sky, 12);
+    *vfov = c.fov_deg;
     return 0;
 }
 
 int phs_scene_load_environment(phs_scene* s, const char* path, float intensity, float rotation) {
     if (!s || !path) return fail("null argument");
     return guarded([&] {
-        Environment env;
+        Backdrop env;
         std::string error;
         if (!load_environment(path, env, error)) return fail(error);
-        env.intensity = intensity;
-        env.rotation = rotation;
-        s->scene.env = std::move(env);
+        env.gain = intensity;
+        env.yaw = rotation;
+        s->scene.backdrop = std::move(env);
         return 0;
     });
 }
@@ -352,14 +353,14 @@ int phs_scene_load_environment(phs_scene* s, const char* path, float intensity, 
 int phs_image_decode(const uint8_t* data, size_t size, int* width, int* height, uint32_t* rgba, size_t capacity) {
     if (!data || !width || !height) return fail("null argument");
     return guarded([&] {
-        Texture t;
+        Image8 t;
         std::string error;
         if (!decode_image(data, size, t, error)) return fail(error);
         *width = t.width;
         *height = t.height;
         if (rgba) {
-            if (capacity < t.texels.size()) return fail("output buffer too small");
-            std::memcpy(rgba, t.texels.data(), t.texels.size() * 4);
+            if (capacity < t.px.size()) return fail("output buffer too small");
+            std::memcpy(rgba, t.px.data(), t.px.size() * 4);
         }
         return 0;
     });
@@ -391,10 +392,11 @@ int phs_image_load_pfm(const char* path, int* width, int* height, float* rgb, si
 int64_t phs_json_canonical(const char* text, char* out, size_t capacity) {
     if (!text) return fail("null argument");
     try {
-        const std::string s = json::write(json::parse(text));
+        const json::Document doc = json::Document::read(text);
+        const std::string s = json::canonical(doc.root());
         if (out && capacity > s.size()) std::memcpy(out, s.c_str(), s.size() + 1);
         return int64_t(s.size());
-    } catch (const json::ParseError& e) {
+    } catch (const json::SyntaxError& e) {
         return fail(e.what());
     } catch (const std::exception& e) {
         return fail(e.what());
@@ -420,7 +422,7 @@ int phs_camera_rays(const ph_camera_view* cam, int width, int height, ph_ray* ou
 
 int phs_sphere_shadow_rays(const phs_scene* s, int count, uint64_t seed, ph_ray* out) {
     if (!s || !out || count < 0) return fail("invalid argument");
-    const std::vector<vec4>& sp = s->scene.spheres;
+    const std::vector<vec4>& sp = s->scene.sphere_meta;
     if (sp.size() < 2) return fail("the scene has fewer than two procedural spheres");
     Pcg32 rng(seed, 0x5AD0u);
     const int n = int(sp.size());

@@ -40,7 +40,7 @@ void add_sphere(Scene& scene, vec3 centre, float radius, int material, int segme
             if (r > 0) idx.insert(idx.end(), {a, a + 1, b});
             if (r < rings - 1) idx.insert(idx.end(), {a + 1, b + 1, b});
         }
-    scene.add_mesh(pos, idx, material, nrm, uv);
+    append_mesh(scene.mesh, pos, idx, material, nrm, uv);
 }
 
 void add_quad(Scene& scene, vec3 corner, vec3 eu, vec3 ev, int material) {
@@ -48,7 +48,7 @@ void add_quad(Scene& scene, vec3 corner, vec3 eu, vec3 ev, int material) {
     std::vector<vec3> pos = {corner, corner + eu, corner + eu + ev, corner + ev};
     std::vector<vec3> nrm(4, n);
     std::vector<vec2> uv = {vec2(0, 0), vec2(1, 0), vec2(1, 1), vec2(0, 1)};
-    scene.add_mesh(pos, {0, 1, 2, 0, 2, 3}, material, nrm, uv);
+    append_mesh(scene.mesh, pos, {0, 1, 2, 0, 2, 3}, material, nrm, uv);
 }
 
 void add_box(Scene& scene, vec3 centre, vec3 h, float rot, int material) {
@@ -67,11 +67,11 @@ void add_box(Scene& scene, vec3 centre, vec3 h, float rot, int material) {
     for (const auto& f : faces) add_quad(scene, place(f[0]), turn(f[1]), turn(f[2]), material);
 }
 
-void make_sky(Environment& env, vec3 sun, float sun_intensity, int width) {
+void make_sky(Backdrop& env, vec3 sun, float sun_intensity, int width) {
     const int height = width / 2;
     env.width = width;
     env.height = height;
-    env.texels.assign(size_t(width) * size_t(height), vec3(0));
+    env.rgb.assign(size_t(width) * size_t(height), vec3(0));
     sun = normalize(sun);
     const float sun_cos = std::cos(0.6f * kPi / 180.0f);
     for (int y = 0; y < height; y++)
@@ -87,7 +87,7 @@ void make_sky(Environment& env, vec3 sun, float sun_intensity, int width) {
             } else {
                 col = vec3(0.25f, 0.23f, 0.21f) * (0.5f + 0.5f * std::exp(8.0f * d.y));
             }
-            env.texels[size_t(y) * size_t(width) + size_t(x)] = col;
+            env.rgb[size_t(y) * size_t(width) + size_t(x)] = col;
         }
 }
 
@@ -109,13 +109,13 @@ static void cornell_geometry(Scene& s, int white, int red, int green, int lamp, 
     add_quad(s, vec3(2.13f, S - 0.01f, 2.27f), vec3(1.3f, 0, 0), vec3(0, 0, 1.05f), lamp);
     add_box(s, vec3(1.85f, 0.825f, 1.69f), vec3(0.825f), -0.29f, short_box);
     add_box(s, vec3(3.68f, 1.65f, 3.51f), vec3(0.825f, 1.65f, 0.825f), 0.29f, tall_box);
-    s.camera.position = vec3(2.78f, 2.73f, -8.0f);
-    s.camera.target = vec3(2.78f, 2.73f, 0.0f);
+    s.view.eye = vec3(2.78f, 2.73f, -8.0f);
+    s.view.aim = vec3(2.78f, 2.73f, 0.0f);
 }
 
 bool make_builtin_scene(const std::string& name, Scene& s) {
     s = Scene();
-    s.name = name;
+    s.label = name;
     if (name == "cornell") {
         int white = s.add_material(matte(vec3(0.725f, 0.71f, 0.68f)));
         int red = s.add_material(matte(vec3(0.63f, 0.065f, 0.05f)));
@@ -124,7 +124,7 @@ bool make_builtin_scene(const std::string& name, Scene& s) {
         light.emission[0] = 17, light.emission[1] = 12, light.emission[2] = 4;
         int lamp = s.add_material(light);
         cornell_geometry(s, white, red, green, lamp, white, white);
-        s.camera.vfov = 39.3f;
+        s.view.fov_deg = 39.3f;
         return true;
     }
     if (name == "cornell-c1") {
@@ -146,7 +146,7 @@ bool make_builtin_scene(const std::string& name, Scene& s) {
         glass.roughness = 0.1f;
         int tall_box = s.add_material(glass);
         cornell_geometry(s, white, red, green, lamp, short_box, tall_box);
-        s.camera.vfov = 60.0f;
+        s.view.fov_deg = 60.0f;
         return true;
     }
     if (name == "materials" || name == "glass") {
@@ -172,9 +172,9 @@ bool make_builtin_scene(const std::string& name, Scene& s) {
                     }
                     add_sphere(s, vec3(-5.0f + 2.0f * float(col), 0.8f, 2.4f * float(row) - 2.4f), 0.8f, s.add_material(m));
                 }
-            s.camera.position = vec3(0, 6.5f, 11.0f);
-            s.camera.target = vec3(0, 0.3f, 0);
-            s.camera.vfov = 40;
+            s.view.eye = vec3(0, 6.5f, 11.0f);
+            s.view.aim = vec3(0, 0.3f, 0);
+            s.view.fov_deg = 40;
         } else {
             ph_material glass = default_material();
             glass.metallic = 0;
@@ -186,21 +186,21 @@ bool make_builtin_scene(const std::string& name, Scene& s) {
             frosted.roughness = 0.35f;
             add_sphere(s, vec3(1.2f, 1.0f, 0), 1.0f, s.add_material(frosted), 96);
             add_box(s, vec3(0, 0.5f, -2.5f), vec3(2.5f, 0.5f, 0.3f), 0, s.add_material(matte(vec3(0.9f, 0.3f, 0.2f))));
-            s.camera.position = vec3(0, 2.2f, 6.0f);
-            s.camera.target = vec3(0, 0.9f, 0);
-            s.camera.vfov = 40;
+            s.view.eye = vec3(0, 2.2f, 6.0f);
+            s.view.aim = vec3(0, 0.9f, 0);
+            s.view.fov_deg = 40;
         }
-        make_sky(s.env, vec3(0.5f, 0.7f, 0.4f), 30000.0f, 1024);
+        make_sky(s.backdrop, vec3(0.5f, 0.7f, 0.4f), 30000.0f, 1024);
         return true;
     }
     if (name == "furnace") {
         ph_material m = matte(vec3(1.0f));
         m.ior = 1.0f;  // index matched: a pure Lambertian
         add_sphere(s, vec3(0), 1.0f, s.add_material(m), 64);
-        s.env.color = vec3(1.0f);
-        s.camera.position = vec3(0, 0, 4);
-        s.camera.target = vec3(0);
-        s.camera.vfov = 40;
+        s.backdrop.flat = vec3(1.0f);
+        s.view.eye = vec3(0, 0, 4);
+        s.view.aim = vec3(0);
+        s.view.fov_deg = 40;
         return true;
     }
     return false;
@@ -275,9 +275,9 @@ int add_noisy_sphere(Scene& s, vec3 centre, float radius, int material, uint64_t
         double v = std::acos(std::max(-1.0, std::min(1.0, double(d.y)))) / 3.14159265358979323846;
         uv.push_back(vec2(float(u), float(v)));
     }
-    int first = s.num_tris();
-    s.add_mesh(pos, m.idx, material, nrm, uv);
-    s.spheres.push_back(vec4(centre, radius));
+    int first = s.mesh.triangles();
+    append_mesh(s.mesh, pos, m.idx, material, nrm, uv);
+    s.sphere_meta.push_back(vec4(centre, radius));
     return first;
 }
 
@@ -285,20 +285,20 @@ int add_noisy_sphere(Scene& s, vec3 centre, float radius, int material, uint64_t
 
 // Alpha texture for masked triangles: 64x64, each texel opaque or clear with probability 1/2.
 static int add_alpha_texture(Scene& s, uint64_t seed) {
-    Texture t;
+    Image8 t;
     t.width = t.height = 64;
-    t.repeat = true;
+    t.tile = true;
     t.srgb = true;
     Pcg32 rng(seed, 0xA1FAu);
-    t.texels.resize(64 * 64);
-    for (auto& px : t.texels) px = 0x00FFFFFFu | (rng.next() < 0.5f ? 0xFF000000u : 0u);
-    s.textures.push_back(t);
-    return int(s.textures.size()) - 1;
+    t.px.resize(64 * 64);
+    for (auto& px : t.px) px = 0x00FFFFFFu | (rng.next() < 0.5f ? 0xFF000000u : 0u);
+    s.images.push_back(t);
+    return int(s.images.size()) - 1;
 }
 
 void make_sphere_soup(Scene& s, int count, float box, float alpha_fraction, uint64_t seed) {
     s = Scene();
-    s.name = "spheres";
+    s.label = "spheres";
     ph_material opaque = matte(vec3(0.7f));
     int m_opaque = s.add_material(opaque);
     int m_masked = m_opaque;
@@ -317,20 +317,20 @@ void make_sphere_soup(Scene& s, int count, float box, float alpha_fraction, uint
     }
     if (alpha_fraction > 0) {
         Pcg32 pick(seed, 0xA1F4u);
-        for (int& m : s.tri_material) m = pick.next() < alpha_fraction ? m_masked : m_opaque;
+        for (int& m : s.mesh.mat) m = pick.next() < alpha_fraction ? m_masked : m_opaque;
     }
-    s.camera.position = vec3(0.5f * box, 0.5f * box, -0.8f * box);
-    s.camera.target = vec3(0.5f * box, 0.5f * box, 0.5f * box);
-    s.camera.vfov = 60.0f;
+    s.view.eye = vec3(0.5f * box, 0.5f * box, -0.8f * box);
+    s.view.aim = vec3(0.5f * box, 0.5f * box, 0.5f * box);
+    s.view.fov_deg = 60.0f;
 }
 
 // The sky of the path tracing workload: luminance 0.2 at the horizon rising linearly in d.y to 1.0 at
 // the zenith, 0.2 below the horizon, and a sun disk of 0.5 degrees diameter with luminance 1e4.
-void make_gradient_sky(Environment& env, vec3 sun, int width) {
+void make_gradient_sky(Backdrop& env, vec3 sun, int width) {
     const int height = width / 2;
     env.width = width;
     env.height = height;
-    env.texels.assign(size_t(width) * size_t(height), vec3(0));
+    env.rgb.assign(size_t(width) * size_t(height), vec3(0));
     sun = normalize(sun);
     const double sun_cos = std::cos(0.25 * 3.14159265358979323846 / 180.0);
     for (int y = 0; y < height; y++)
@@ -338,13 +338,13 @@ void make_gradient_sky(Environment& env, vec3 sun, int width) {
             vec3 d = equirect_to_direction(vec2((float(x) + 0.5f) / float(width), (float(y) + 0.5f) / float(height)));
             float lum = d.y > 0 ? 0.2f + 0.8f * d.y : 0.2f;
             if (double(dot(d, sun)) > sun_cos) lum = 1e4f;
-            env.texels[size_t(y) * size_t(width) + size_t(x)] = vec3(lum);
+            env.rgb[size_t(y) * size_t(width) + size_t(x)] = vec3(lum);
         }
 }
 
 void make_sphere_field(Scene& s, int count, uint64_t seed, int env_width) {
     s = Scene();
-    s.name = "field";
+    s.label = "field";
     Pcg32 rng(seed, 0xF1E1Du);
     ph_material ground = matte(vec3(0.5f));
     int g = s.add_material(ground);
@@ -365,10 +365,10 @@ void make_sphere_field(Scene& s, int count, uint64_t seed, int env_width) {
         }
         add_noisy_sphere(s, c, r, s.add_material(m), seed, uint64_t(i) + 1);
     }
-    make_gradient_sky(s.env, vec3(0.4f, 0.7f, 0.3f), env_width);
-    s.camera.position = vec3(0, 45, -170);
-    s.camera.target = vec3(0, 5, 0);
-    s.camera.vfov = 60.0f;
+    make_gradient_sky(s.backdrop, vec3(0.4f, 0.7f, 0.3f), env_width);
+    s.view.eye = vec3(0, 45, -170);
+    s.view.aim = vec3(0, 5, 0);
+    s.view.fov_deg = 60.0f;
 }
 
 }  // namespace phh

@@ -26,6 +26,8 @@
 #include <iterator>
 #include <map>
 #include <sstream>
+#include <string_view>
+#include <unordered_map>
 
 #include "image_io.hpp"
 #include "json.hpp"
@@ -35,7 +37,7 @@ namespace phh {
 namespace {
 
 namespace fs = std::filesystem;
-using json::Value;
+using json::Node;
 
 // Row-major 4x4 transform.
 struct Xform {
@@ -200,17 +202,17 @@ struct Failure {
 
 // A byte offset, length or stride of the file: a whole number in [0, 2^40), so that sums and products
 // of them cannot wrap.
-size_t byte_size(const json::Value& v, double fallback, const char* what) {
-    const double d = v.number(fallback);
+size_t byte_size(Node v, double fallback, const char* what) {
+    const double d = v.real(fallback);
     if (!(d >= 0.0) || d >= 1099511627776.0 || d != std::floor(d)) fail(std::string("invalid ") + what);
     return size_t(d);
 }
 
 // The stride between elements of an accessor: the view's byteStride, which glTF 2.0 bounds to [4, 252]
 // and which cannot be shorter than one element, or the element size when the view has none.
-size_t element_stride(const json::Value& view, size_t elem) {
-    if (!view.has("byteStride")) return elem;
-    const size_t stride = byte_size(view["byteStride"], 0, "byteStride");
+size_t element_stride(Node view, size_t elem) {
+    if (!view.holds("byteStride")) return elem;
+    const size_t stride = byte_size(view.member("byteStride"), 0, "byteStride");
     if (stride < 4 || stride > 252 || stride < elem) fail("invalid byteStride");
     return stride;
 }
@@ -225,6 +227,89 @@ bool span_fits(size_t base, size_t stride, int count, size_t elem, size_t size) 
     return end <= size;
 }
 
+// Component types of accessors (glTF 2.0, 3.6.2.2).
+enum Component : int { kI8 = 5120, kU8 = 5121, kI16 = 5122, kU16 = 5123, kU32 = 5125, kF32 = 5126 };
+
+int component_bytes(int ct) { return (ct == kF32 || ct == kU32) ? 4 : (ct == kI16 || ct == kU16) ? 2 : 1; }
+
+// One component as float: integers normalised as the specification says when `norm` is set
+// (unsigned x / (2^n - 1), signed max(x / (2^(n-1) - 1), -1)), converted as they are otherwise.
+float component_value(const uint8_t* q, int ct, bool norm) {
+    switch (ct) {
+        case kF32: {
+            float v;
+            std::memcpy(&v, q, 4);
+            return v;
+        }
+        case kU8: return norm ? float(q[0]) / 255.0f : float(q[0]);
+        case kI8: return norm ? std::max(float(int8_t(q[0])) / 127.0f, -1.0f) : float(int8_t(q[0]));
+        case kU16: {
+            uint16_t u;
+            std::memcpy(&u, q, 2);
+            return norm ? float(u) / 65535.0f : float(u);
+        }
+        case kI16: {
+            int16_t v;
+            std::memcpy(&v, q, 2);
+            return norm ? std::max(float(v) / 32767.0f, -1.0f) : float(v);
+        }
+        case kU32: {
+            uint32_t u;
+            std::memcpy(&u, q, 4);
+            return float(u);
+        }
+        default: fail("unsupported component type");
+    }
+}
+
+int components_of(std::string_view type) {
+    static const std::pair<std::string_view, int> known[] = {{"SCALAR", 1}, {"VEC2", 2}, {"VEC3", 3}, {"VEC4", 4}, {"MAT4", 16}};
+    for (const auto& [name, n] : known)
+        if (name == type) return n;
+    fail("unsupported accessor type");
+}
+
+vec3 vec3_at(Node a, float otherwise) {
+    return vec3(float(a.element(0).real(otherwise)), float(a.element(1).real(otherwise)), float(a.element(2).real(otherwise)));
+}
+
+// A node's local transform: its matrix (column major in the file) or T * R * S.
+Xform node_local(Node nd) {
+    if (nd.holds("matrix")) {
+        const Node a = nd.member("matrix");
+        Xform x;
+        for (int col = 0; col < 4; col++)
+            for (int row = 0; row < 4; row++) x.m[row][col] = float(a.element(size_t(col * 4 + row)).real(row == col ? 1.0 : 0.0));
+        return x;
+    }
+    const vec3 t = nd.holds("translation") ? vec3_at(nd.member("translation"), 0.0f) : vec3(0.0f);
+    const vec3 sc = nd.holds("scale") ? vec3_at(nd.member("scale"), 1.0f) : vec3(1.0f);
+    vec4 q(0, 0, 0, 1);
+    if (nd.holds("rotation")) {
+        const Node r = nd.member("rotation");
+        q = vec4(float(r.element(0).real(0)), float(r.element(1).real(0)), float(r.element(2).real(0)), float(r.element(3).real(1)));
+    }
+    return trs(t, q, sc);
+}
+
+// Material extensions that carry one scalar each, and the field it sets.
+struct ScalarExtension {
+    const char* extension;
+    const char* key;
+    float ph_material::*field;
+    double otherwise;
+};
+constexpr ScalarExtension kScalarExtensions[] = {
+    {"KHR_materials_transmission", "transmissionFactor", &ph_material::transmission, 0.0},
+    {"KHR_materials_ior", "ior", &ph_material::ior, 1.5},
+};
+
+int alpha_mode_of(std::string_view mode) {
+    if (mode == "MASK") return PH_ALPHA_MASK;
+    if (mode == "BLEND") return PH_ALPHA_BLEND;
+    return PH_ALPHA_OPAQUE;
+}
+
 class GltfImport {
 public:
     GltfImport(Scene& s, const std::string& path) : scene_(s), path_(path) {}
@@ -234,59 +319,69 @@ public:
         if (!read_file(utf8_path(path_), file)) fail("cannot open " + path_);
         dir_ = utf8_path(path_).parent_path();
         std::string text;
-        bool glb = false;
-        if (file.size() >= 12 && std::memcmp(file.data(), "glTF", 4) == 0) {
-            glb = true;
+        const bool glb = file.size() >= 12 && std::memcmp(file.data(), "glTF", 4) == 0;
+        if (glb)
             split_glb(file, text);
-        } else {
+        else
             text.assign(file.begin(), file.end());
-        }
         try {
-            doc_ = json::parse(text);
-        } catch (const json::ParseError& e) {
+            doc_ = json::Document::read(text);
+        } catch (const json::SyntaxError& e) {
             fail(e.what());
         }
-        if (doc_["asset"]["version"].string().rfind('2', 0) != 0) fail("only glTF 2.0 is supported");
+        root_ = doc_.root();
+        if (root_.member("asset").member("version").text().substr(0, 1) != "2") fail("only glTF 2.0 is supported");
         load_buffers(glb);
-        images_.resize(doc_["images"].size());
-        image_state_.assign(images_.size(), 0);
-        load_materials();
-        const Value& sc = doc_["scenes"][size_t(doc_["scene"].integer(0))];
-        const Xform root;
-        if (sc.is_null()) {
-            std::vector<bool> is_child(doc_["nodes"].size(), false);
-            for (size_t i = 0; i < doc_["nodes"].size(); i++) {
-                const Value& kids = doc_["nodes"][i]["children"];
-                for (size_t k = 0; k < kids.size(); k++) {
-                    const int c = kids[k].integer();
-                    if (c >= 0 && size_t(c) < is_child.size()) is_child[size_t(c)] = true;
-                }
-            }
-            for (size_t i = 0; i < is_child.size(); i++)
-                if (!is_child[i]) node(int(i), root, 0);
-        } else {
-            const Value& roots = sc["nodes"];
-            for (size_t i = 0; i < roots.size(); i++) node(roots[i].integer(), root, 0);
-        }
+        decoded_.resize(root_.member("images").length());
+        decode_state_.assign(decoded_.size(), 0);
+        const Node mats = root_.member("materials");
+        material_base_ = int(scene_.materials.size());
+        for (size_t i = 0; i < mats.length(); i++) scene_.materials.push_back(read_material(mats.element(i)));
+        for (const int n : root_nodes()) visit(n, Xform(), 0);
     }
 
 private:
     Scene& scene_;
     std::string path_;
     fs::path dir_;
-    Value doc_;
+    json::Document doc_;
+    Node root_;
     std::vector<uint8_t> glb_bin_;
     std::vector<std::vector<uint8_t>> buffers_;
-    std::vector<Texture> images_;
-    std::vector<int> image_state_;  // 0 not tried, 1 decoded, -1 failed
-    std::map<std::pair<int, bool>, int> textures_;
+    std::vector<Image8> decoded_;
+    std::vector<int> decode_state_;  // 0 not tried, 1 decoded, -1 failed
+    std::map<std::pair<int, bool>, int> texture_ids_;
     int material_base_ = 0;
-    int default_material_ = -1;
+    int fallback_ = -1;
+
+    Node at(const char* array, int i) const { return i < 0 ? Node() : root_.member(array).element(size_t(i)); }
+
+    // The default scene's nodes; with no scenes list, every node that is no other node's child.
+    std::vector<int> root_nodes() const {
+        const Node sc = at("scenes", root_.member("scene").whole(0));
+        std::vector<int> out;
+        if (!sc.null()) {
+            const Node list = sc.member("nodes");
+            for (size_t i = 0; i < list.length(); i++) out.push_back(list.element(i).whole());
+            return out;
+        }
+        const Node nodes = root_.member("nodes");
+        std::vector<char> child(nodes.length(), 0);
+        for (size_t i = 0; i < nodes.length(); i++) {
+            const Node kids = nodes.element(i).member("children");
+            for (size_t k = 0; k < kids.length(); k++) {
+                const int c = kids.element(k).whole();
+                if (c >= 0 && size_t(c) < child.size()) child[size_t(c)] = 1;
+            }
+        }
+        for (size_t i = 0; i < child.size(); i++)
+            if (!child[i]) out.push_back(int(i));
+        return out;
+    }
 
     void split_glb(const std::vector<uint8_t>& file, std::string& text) {
         // Header (magic, version, length), then chunks of (length, type, data padded to 4 bytes).
-        size_t off = 12;
-        while (off + 8 <= file.size()) {
+        for (size_t off = 12; off + 8 <= file.size();) {
             uint32_t len, type;
             std::memcpy(&len, &file[off], 4);
             std::memcpy(&type, &file[off + 4], 4);
@@ -301,15 +396,15 @@ private:
     }
 
     void load_buffers(bool glb) {
-        const Value& bufs = doc_["buffers"];
-        for (size_t i = 0; i < bufs.size(); i++) {
-            const Value& b = bufs[i];
+        const Node bufs = root_.member("buffers");
+        for (size_t i = 0; i < bufs.length(); i++) {
+            const Node b = bufs.element(i);
             std::vector<uint8_t> data;
-            if (!b.has("uri")) {
+            if (!b.holds("uri")) {
                 if (!glb) fail("buffer " + std::to_string(i) + " has no data");
                 data = glb_bin_;
             } else {
-                const std::string& uri = b["uri"].string();
+                const std::string uri(b.member("uri").text());
                 if (uri.rfind("data:", 0) == 0) {
                     const size_t comma = uri.find(',');
                     if (comma == std::string::npos || !base64_decode(uri.substr(comma + 1), data)) fail("invalid data URI");
@@ -321,94 +416,70 @@ private:
         }
     }
 
-    const std::vector<uint8_t>& buffer_of(const Value& view) {
-        const int b = view["buffer"].integer(0);
+    const std::vector<uint8_t>& buffer_of(Node view) const {
+        const int b = view.member("buffer").whole(0);
         if (b < 0 || size_t(b) >= buffers_.size()) fail("buffer view refers to a missing buffer");
         return buffers_[size_t(b)];
     }
 
-    static int accessor_count(const Value& a) {
-        const size_t n = byte_size(a["count"], 0, "accessor count");
+    static int count_of(Node a) {
+        const size_t n = byte_size(a.member("count"), 0, "accessor count");
         if (n > size_t(1) << 30) fail("accessor count out of range");
         return int(n);
     }
 
-    static int component_size(int ct) { return (ct == 5126 || ct == 5125) ? 4 : (ct == 5122 || ct == 5123) ? 2 : 1; }
+    // Where an accessor's elements lie: the buffer, the first byte and the stride, range-checked.
+    struct Span {
+        const std::vector<uint8_t>* buf = nullptr;
+        size_t base = 0, stride = 0;
+    };
+    Span span_of(Node a, int count, size_t element, const char* what) const {
+        const Node view = at("bufferViews", a.member("bufferView").whole());
+        Span s;
+        s.buf = &buffer_of(view);
+        s.base = byte_size(view.member("byteOffset"), 0, "byteOffset") + byte_size(a.member("byteOffset"), 0, "byteOffset");
+        s.stride = element_stride(view, element);
+        if (!span_fits(s.base, s.stride, count, element, s.buf->size())) fail(what);
+        return s;
+    }
 
-    // Accessor -> count x `want` floats (missing components stay 0).
-    std::vector<float> floats(int accessor, int want) {
-        const Value& a = doc_["accessors"][size_t(accessor)];
-        if (a.is_null()) fail("missing accessor");
-        const int count = accessor_count(a);
-        static const std::map<std::string, int> widths = {{"SCALAR", 1}, {"VEC2", 2}, {"VEC3", 3}, {"VEC4", 4}, {"MAT4", 16}};
-        const auto wit = widths.find(a["type"].string());
-        if (wit == widths.end()) fail("unsupported accessor type");
-        const int nc = wit->second;
-        const int ct = a["componentType"].integer();
-        const bool norm = a["normalized"].boolean();
-        const int cs = component_size(ct);
-        std::vector<float> out(size_t(std::max(count, 0)) * size_t(want), 0.0f);
-        if (!a.has("bufferView")) return out;
-        const Value& view = doc_["bufferViews"][size_t(a["bufferView"].integer())];
-        const std::vector<uint8_t>& buf = buffer_of(view);
-        const size_t base = byte_size(view["byteOffset"], 0, "byteOffset") + byte_size(a["byteOffset"], 0, "byteOffset");
-        const size_t stride = element_stride(view, size_t(nc * cs));
-        if (!span_fits(base, stride, count, size_t(nc * cs), buf.size())) fail("accessor out of the buffer's range");
+    // Accessor -> count x `want` floats (components the accessor lacks stay 0; an accessor without a
+    // buffer view reads as zeros).
+    std::vector<float> floats(int accessor, int want) const {
+        const Node a = at("accessors", accessor);
+        if (a.null()) fail("missing accessor");
+        const int count = count_of(a);
+        const int nc = components_of(a.member("type").text());
+        const int ct = a.member("componentType").whole();
+        const bool norm = a.member("normalized").truth();
+        const int cb = component_bytes(ct);
+        std::vector<float> out(size_t(count) * size_t(want), 0.0f);
+        if (!a.holds("bufferView")) return out;
+        const Span s = span_of(a, count, size_t(nc * cb), "accessor out of the buffer's range");
         const int take = std::min(nc, want);
         for (int i = 0; i < count; i++) {
-            const uint8_t* p = buf.data() + base + stride * size_t(i);
-            for (int c = 0; c < take; c++) {
-                const uint8_t* q = p + size_t(c * cs);
-                float v;
-                switch (ct) {
-                    case 5126: std::memcpy(&v, q, 4); break;
-                    case 5121: v = norm ? float(q[0]) / 255.0f : float(q[0]); break;
-                    case 5120: v = norm ? std::max(float(int8_t(q[0])) / 127.0f, -1.0f) : float(int8_t(q[0])); break;
-                    case 5123: {
-                        uint16_t u;
-                        std::memcpy(&u, q, 2);
-                        v = norm ? float(u) / 65535.0f : float(u);
-                        break;
-                    }
-                    case 5122: {
-                        int16_t s;
-                        std::memcpy(&s, q, 2);
-                        v = norm ? std::max(float(s) / 32767.0f, -1.0f) : float(s);
-                        break;
-                    }
-                    case 5125: {
-                        uint32_t u;
-                        std::memcpy(&u, q, 4);
-                        v = float(u);
-                        break;
-                    }
-                    default: fail("unsupported component type");
-                }
-                out[size_t(i) * size_t(want) + size_t(c)] = v;
-            }
+            const uint8_t* elem = s.buf->data() + s.base + s.stride * size_t(i);
+            for (int c = 0; c < take; c++) out[size_t(i) * size_t(want) + size_t(c)] = component_value(elem + size_t(c * cb), ct, norm);
         }
         return out;
     }
 
-    std::vector<int> indices(int accessor) {
-        const Value& a = doc_["accessors"][size_t(accessor)];
-        const int count = accessor_count(a);
-        const int cs = a["componentType"].integer() == 5125 ? 4 : a["componentType"].integer() == 5123 ? 2 : 1;
-        if (!a.has("bufferView")) fail("index accessor without data");
-        const Value& view = doc_["bufferViews"][size_t(a["bufferView"].integer())];
-        const std::vector<uint8_t>& buf = buffer_of(view);
-        const size_t base = byte_size(view["byteOffset"], 0, "byteOffset") + byte_size(a["byteOffset"], 0, "byteOffset");
-        const size_t stride = element_stride(view, size_t(cs));
-        if (!span_fits(base, stride, count, size_t(cs), buf.size())) fail("index accessor out of range");
-        std::vector<int> out(size_t(std::max(count, 0)));
+    std::vector<int> indices(int accessor) const {
+        const Node a = at("accessors", accessor);
+        const int count = count_of(a);
+        const int ct = a.member("componentType").whole();
+        const int cb = ct == kU32 ? 4 : ct == kU16 ? 2 : 1;
+        if (!a.holds("bufferView")) fail("index accessor without data");
+        const Span s = span_of(a, count, size_t(cb), "index accessor out of range");
+        std::vector<int> out(static_cast<size_t>(count));
         for (int i = 0; i < count; i++) {
-            const uint8_t* p = buf.data() + base + stride * size_t(i);
+            const uint8_t* p = s.buf->data() + s.base + s.stride * size_t(i);
             uint32_t v = p[0];
-            if (cs == 2) {
+            if (cb == 2) {
                 uint16_t u;
                 std::memcpy(&u, p, 2);
                 v = u;
-            } else if (cs == 4) {
+            } else if (cb == 4) {
                 std::memcpy(&v, p, 4);
             }
             out[size_t(i)] = int(v);
@@ -416,165 +487,149 @@ private:
         return out;
     }
 
-    // Decodes glTF image `i` into images_[i]; false (with the reason) when it cannot.
+    // Decodes glTF image `i` into decoded_[i]; false (with the reason) when it cannot.
     bool decode(int i, std::string& why) {
-        const Value& img = doc_["images"][size_t(i)];
-        Texture& tex = images_[size_t(i)];
-        if (img.has("uri")) {
-            const std::string& uri = img["uri"].string();
-            if (uri.rfind("data:", 0) == 0) {
-                std::vector<uint8_t> data;
-                const size_t comma = uri.find(',');
-                if (comma == std::string::npos || !base64_decode(uri.substr(comma + 1), data)) {
-                    why = "invalid image data URI";
-                    return false;
-                }
-                return decode_image(data.data(), data.size(), tex, why);
-            }
-            return load_image((dir_ / utf8_path(percent_decode(uri))).string(), tex, why);
-        }
-        if (img.has("bufferView")) {
-            const Value& view = doc_["bufferViews"][size_t(img["bufferView"].integer())];
-            const int b = view["buffer"].integer(0);
-            if (b < 0 || size_t(b) >= buffers_.size()) {
-                why = "image buffer view refers to a missing buffer";
+        const Node img = at("images", i);
+        Image8& out = decoded_[size_t(i)];
+        if (img.holds("uri")) {
+            const std::string uri(img.member("uri").text());
+            if (uri.rfind("data:", 0) != 0) return load_image((dir_ / utf8_path(percent_decode(uri))).string(), out, why);
+            std::vector<uint8_t> data;
+            const size_t comma = uri.find(',');
+            if (comma == std::string::npos || !base64_decode(uri.substr(comma + 1), data)) {
+                why = "invalid image data URI";
                 return false;
             }
-            const std::vector<uint8_t>& buf = buffers_[size_t(b)];
-            const double od = view["byteOffset"].number(0), ld = view["byteLength"].number(0);
-            if (!(od >= 0 && ld >= 0 && od + ld <= double(buf.size()))) {
-                why = "image buffer view out of range";
-                return false;
-            }
-            return decode_image(buf.data() + size_t(od), size_t(ld), tex, why);
+            return decode_image(data.data(), data.size(), out, why);
         }
-        why = "image without data";
-        return false;
+        if (!img.holds("bufferView")) {
+            why = "image without data";
+            return false;
+        }
+        const Node view = at("bufferViews", img.member("bufferView").whole());
+        const int b = view.member("buffer").whole(0);
+        if (b < 0 || size_t(b) >= buffers_.size()) {
+            why = "image buffer view refers to a missing buffer";
+            return false;
+        }
+        const std::vector<uint8_t>& buf = buffers_[size_t(b)];
+        const double off = view.member("byteOffset").real(0), len = view.member("byteLength").real(0);
+        if (!(off >= 0 && len >= 0 && off + len <= double(buf.size()))) {
+            why = "image buffer view out of range";
+            return false;
+        }
+        return decode_image(buf.data() + size_t(off), size_t(len), out, why);
     }
 
-    // Scene texture of a material's texture reference in a colour space; -1 when absent or unusable.
-    int texture(const Value& ref, bool srgb) {
-        if (!ref.is_object() || !ref.has("index")) return -1;
-        const int ti = ref["index"].integer();
+    // The scene image for a material's texture reference in a colour space (each glTF texture is added
+    // once per colour space it is used in); -1 when absent or undecodable.
+    int texture(Node ref, bool srgb) {
+        if (!ref.is_object() || !ref.holds("index")) return -1;
+        const int ti = ref.member("index").whole();
         const auto key = std::make_pair(ti, srgb);
-        if (const auto it = textures_.find(key); it != textures_.end()) return it->second;
-        const Value& t = doc_["textures"][size_t(ti)];
-        int source = t["source"].integer(-1);
-        if (source < 0 && t["extensions"].has("EXT_texture_webp")) source = t["extensions"]["EXT_texture_webp"]["source"].integer(-1);
-        if (source < 0 || size_t(source) >= images_.size()) return -1;
-        if (image_state_[size_t(source)] == 0) {
-            std::string why;
-            image_state_[size_t(source)] = decode(source, why) ? 1 : -1;
-            if (image_state_[size_t(source)] < 0) std::fprintf(stderr, "warning: texture skipped: %s\n", why.c_str());
+        if (const auto hit = texture_ids_.find(key); hit != texture_ids_.end()) return hit->second;
+        const Node t = at("textures", ti);
+        int source = t.member("source").whole(-1);
+        if (source < 0) source = t.member("extensions").member("EXT_texture_webp").member("source").whole(-1);
+        int id = -1;
+        if (source >= 0 && size_t(source) < decoded_.size()) {
+            int& state = decode_state_[size_t(source)];
+            if (state == 0) {
+                std::string why;
+                state = decode(source, why) ? 1 : -1;
+                if (state < 0) std::fprintf(stderr, "warning: texture skipped: %s\n", why.c_str());
+            }
+            if (state > 0) {
+                Image8 img = decoded_[size_t(source)];
+                img.srgb = srgb;
+                // CLAMP_TO_EDGE (33071) clamps; the repeating wraps (and no sampler) repeat.
+                img.tile = at("samplers", t.member("sampler").whole(-1)).member("wrapS").whole(10497) != 33071;
+                id = scene_.add_image(std::move(img));
+            }
         }
-        if (image_state_[size_t(source)] < 0) {
-            textures_[key] = -1;
-            return -1;
-        }
-        Texture tex = images_[size_t(source)];
-        tex.srgb = srgb;
-        const Value& sampler = doc_["samplers"][size_t(t["sampler"].integer(-1))];
-        tex.repeat = sampler["wrapS"].integer(10497) != 33071;  // CLAMP_TO_EDGE clamps, the others repeat
-        const int id = int(scene_.textures.size());
-        scene_.textures.push_back(std::move(tex));
-        textures_[key] = id;
+        texture_ids_[key] = id;
         return id;
     }
 
-    void load_materials() {
-        const Value& mats = doc_["materials"];
-        material_base_ = int(scene_.materials.size());
-        for (size_t i = 0; i < mats.size(); i++) {
-            const Value& m = mats[i];
-            const Value& pbr = m["pbrMetallicRoughness"];
-            const Value& ext = m["extensions"];
-            ph_material mat = default_material();
-            if (pbr.has("baseColorFactor"))
-                for (int k = 0; k < 4; k++) mat.base_color[k] = float(pbr["baseColorFactor"][size_t(k)].number(1.0));
-            mat.metallic = float(pbr["metallicFactor"].number(1.0));
-            mat.roughness = float(pbr["roughnessFactor"].number(1.0));
-            mat.base_color_tex = texture(pbr["baseColorTexture"], true);
-            mat.metallic_roughness_tex = texture(pbr["metallicRoughnessTexture"], false);
-            mat.normal_tex = texture(m["normalTexture"], false);
-            mat.normal_scale = float(m["normalTexture"]["scale"].number(1.0));
-            mat.emissive_tex = texture(m["emissiveTexture"], true);
-            const float strength = float(ext["KHR_materials_emissive_strength"]["emissiveStrength"].number(1.0));
-            for (int k = 0; k < 3; k++) mat.emission[k] = (m.has("emissiveFactor") ? float(m["emissiveFactor"][size_t(k)].number(0.0)) : 0.0f) * strength;
-            mat.transmission = float(ext["KHR_materials_transmission"]["transmissionFactor"].number(0.0));
-            mat.ior = float(ext["KHR_materials_ior"]["ior"].number(1.5));
-            mat.double_sided = m["doubleSided"].boolean() ? 1 : 0;
-            const std::string& mode = m["alphaMode"].string();
-            mat.alpha_mode = mode == "MASK" ? PH_ALPHA_MASK : mode == "BLEND" ? PH_ALPHA_BLEND : PH_ALPHA_OPAQUE;
-            mat.alpha_cutoff = mode == "BLEND" ? 0.5f : float(m["alphaCutoff"].number(0.5));
-            scene_.materials.push_back(mat);
+    ph_material read_material(Node m) {
+        ph_material out = default_material();
+        const Node pbr = m.member("pbrMetallicRoughness");
+        if (pbr.holds("baseColorFactor")) {
+            const Node f = pbr.member("baseColorFactor");
+            for (int k = 0; k < 4; k++) out.base_color[k] = float(f.element(size_t(k)).real(1.0));
         }
+        out.metallic = float(pbr.member("metallicFactor").real(1.0));
+        out.roughness = float(pbr.member("roughnessFactor").real(1.0));
+        // Colour textures are sRGB-encoded, data textures linear.
+        out.base_color_tex = texture(pbr.member("baseColorTexture"), true);
+        out.metallic_roughness_tex = texture(pbr.member("metallicRoughnessTexture"), false);
+        out.normal_tex = texture(m.member("normalTexture"), false);
+        out.normal_scale = float(m.member("normalTexture").member("scale").real(1.0));
+        out.emissive_tex = texture(m.member("emissiveTexture"), true);
+        const Node ext = m.member("extensions");
+        const float strength = float(ext.member("KHR_materials_emissive_strength").member("emissiveStrength").real(1.0));
+        const Node emissive = m.member("emissiveFactor");
+        for (int k = 0; k < 3; k++) out.emission[k] = (emissive.null() ? 0.0f : float(emissive.element(size_t(k)).real(0.0))) * strength;
+        for (const ScalarExtension& e : kScalarExtensions) out.*e.field = float(ext.member(e.extension).member(e.key).real(e.otherwise));
+        out.double_sided = m.member("doubleSided").truth() ? 1 : 0;
+        out.alpha_mode = alpha_mode_of(m.member("alphaMode").text());
+        out.alpha_cutoff = out.alpha_mode == PH_ALPHA_BLEND ? 0.5f : float(m.member("alphaCutoff").real(0.5));
+        return out;
     }
 
+    // The material of primitives that name none: shared, grey 0.8, dielectric, roughness 0.6.
     int fallback_material() {
-        if (default_material_ < 0) {
+        if (fallback_ < 0) {
             ph_material m = default_material();
-            m.base_color[0] = m.base_color[1] = m.base_color[2] = 0.8f;
+            std::fill(m.base_color, m.base_color + 3, 0.8f);
             m.metallic = 0.0f;
             m.roughness = 0.6f;
-            default_material_ = scene_.add_material(m);
+            fallback_ = scene_.add_material(m);
         }
-        return default_material_;
+        return fallback_;
     }
 
-    static vec3 vec3_of(const Value& a, float fallback) {
-        return vec3(float(a[0].number(fallback)), float(a[1].number(fallback)), float(a[2].number(fallback)));
-    }
-
-    static Xform local_xform(const Value& nd) {
-        if (nd.has("matrix")) {
-            Xform x;
-            const Value& a = nd["matrix"];
-            for (int c = 0; c < 4; c++)
-                for (int r = 0; r < 4; r++) x.m[r][c] = float(a[size_t(c * 4 + r)].number(r == c ? 1.0 : 0.0));
-            return x;
-        }
-        const vec3 t = nd.has("translation") ? vec3_of(nd["translation"], 0.0f) : vec3(0.0f);
-        const vec3 s = nd.has("scale") ? vec3_of(nd["scale"], 1.0f) : vec3(1.0f);
-        vec4 q(0, 0, 0, 1);
-        if (nd.has("rotation")) {
-            const Value& r = nd["rotation"];
-            q = vec4(float(r[0].number(0)), float(r[1].number(0)), float(r[2].number(0)), float(r[3].number(1)));
-        }
-        return trs(t, q, s);
-    }
-
-    void primitive(const Value& prim, const Xform& world) {
-        const int mode = prim["mode"].integer(4);
-        if (mode != 4 && mode != 5 && mode != 6) return;
-        const Value& attrs = prim["attributes"];
-        if (!attrs.has("POSITION")) return;
-        const std::vector<float> pos = floats(attrs["POSITION"].integer(), 3);
-        const std::vector<float> nrm = attrs.has("NORMAL") ? floats(attrs["NORMAL"].integer(), 3) : std::vector<float>{};
-        const std::vector<float> uv = attrs.has("TEXCOORD_0") ? floats(attrs["TEXCOORD_0"].integer(), 2) : std::vector<float>{};
-        const std::vector<float> tan = attrs.has("TANGENT") ? floats(attrs["TANGENT"].integer(), 4) : std::vector<float>{};
-        const size_t nv = pos.size() / 3;
+    // Triangle list of a primitive: its indices (or 0..n-1), strips and fans unrolled with the strip's
+    // alternating winding.
+    std::vector<int> triangle_list(Node prim, int mode, size_t nv) const {
         std::vector<int> idx;
-        if (prim.has("indices")) {
-            idx = indices(prim["indices"].integer());
+        if (prim.holds("indices")) {
+            idx = indices(prim.member("indices").whole());
         } else {
             idx.resize(nv);
             for (size_t i = 0; i < nv; i++) idx[i] = int(i);
         }
-        if (mode != 4) {
-            std::vector<int> list;
-            for (size_t i = 2; i < idx.size(); i++) {
-                if (mode == 6)
-                    list.insert(list.end(), {idx[0], idx[i - 1], idx[i]});
-                else if (i % 2 == 0)
-                    list.insert(list.end(), {idx[i - 2], idx[i - 1], idx[i]});
-                else
-                    list.insert(list.end(), {idx[i - 1], idx[i - 2], idx[i]});
-            }
-            idx.swap(list);
+        if (mode == 4) return idx;
+        std::vector<int> list;
+        for (size_t i = 2; i < idx.size(); i++) {
+            if (mode == 6)
+                list.insert(list.end(), {idx[0], idx[i - 1], idx[i]});
+            else if (i % 2 == 0)
+                list.insert(list.end(), {idx[i - 2], idx[i - 1], idx[i]});
+            else
+                list.insert(list.end(), {idx[i - 1], idx[i - 2], idx[i]});
         }
+        return list;
+    }
+
+    void primitive(Node prim, const Xform& world) {
+        const int mode = prim.member("mode").whole(4);
+        if (mode != 4 && mode != 5 && mode != 6) return;  // points and lines carry no surface
+        const Node attrs = prim.member("attributes");
+        if (!attrs.holds("POSITION")) return;
+        auto stream = [&](const char* name, int width) {
+            return attrs.holds(name) ? floats(attrs.member(name).whole(), width) : std::vector<float>{};
+        };
+        const std::vector<float> pos = floats(attrs.member("POSITION").whole(), 3);
+        const std::vector<float> nrm = stream("NORMAL", 3), uv = stream("TEXCOORD_0", 2), tan = stream("TANGENT", 4);
+        const size_t nv = pos.size() / 3;
+        std::vector<int> idx = triangle_list(prim, mode, nv);
         for (const int i : idx)
             if (i < 0 || size_t(i) >= nv) fail("vertex index out of range");
+        // A mirroring transform turns the winding: swap each triangle's last two corners.
         const bool mirrored = det3(world) < 0.0f;
+        if (mirrored)
+            for (size_t i = 0; i + 2 < idx.size(); i += 3) std::swap(idx[i + 1], idx[i + 2]);
         std::vector<vec3> p(nv), n;
         std::vector<vec2> t;
         for (size_t i = 0; i < nv; i++) p[i] = apply_point(world, vec3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]));
@@ -587,73 +642,275 @@ private:
             t.resize(nv);
             for (size_t i = 0; i < nv; i++) t[i] = vec2(uv[2 * i], uv[2 * i + 1]);
         }
-        if (mirrored)
-            for (size_t i = 0; i + 2 < idx.size(); i += 3) std::swap(idx[i + 1], idx[i + 2]);
-        const int material = prim.has("material") ? material_base_ + prim["material"].integer() : fallback_material();
-        const size_t vbase = scene_.positions.size();
-        scene_.add_mesh(p, idx, material, n, t);
+        const int material = prim.holds("material") ? material_base_ + prim.member("material").whole() : fallback_material();
+        Mesh& mesh = scene_.mesh;
+        const size_t first = mesh.pos.size();
+        append_mesh(mesh, p, idx, material, n, t);
+        // Tangents: the file's, transformed (handedness flipped under mirroring), once any primitive has
+        // them; other primitives' vertices get +x.
         if (!tan.empty()) {
-            scene_.tangents.resize(vbase, vec4(1, 0, 0, 1));
+            mesh.tan.resize(first, vec4(1, 0, 0, 1));
             for (size_t i = 0; i < nv; i++) {
-                const vec3 tv = normalize(apply_dir(world, vec3(tan[4 * i], tan[4 * i + 1], tan[4 * i + 2])));
-                scene_.tangents.push_back(vec4(tv, tan[4 * i + 3] * (mirrored ? -1.0f : 1.0f)));
+                const vec3 dir = normalize(apply_dir(world, vec3(tan[4 * i], tan[4 * i + 1], tan[4 * i + 2])));
+                mesh.tan.push_back(vec4(dir, tan[4 * i + 3] * (mirrored ? -1.0f : 1.0f)));
             }
-        } else if (!scene_.tangents.empty()) {
-            scene_.tangents.resize(scene_.positions.size(), vec4(1, 0, 0, 1));
+        } else if (!mesh.tan.empty()) {
+            mesh.tan.resize(mesh.pos.size(), vec4(1, 0, 0, 1));
         }
     }
 
-    void node(int index, const Xform& parent, int depth) {
+    void camera(Node cam, const Xform& world) {
+        if (cam.member("type").text() != "perspective") return;
+        Viewpoint v;
+        v.eye = apply_point(world, vec3(0.0f));
+        v.aim = v.eye + normalize(apply_dir(world, vec3(0, 0, -1)));
+        v.sky = normalize(apply_dir(world, vec3(0, 1, 0)));
+        v.fov_deg = float(cam.member("perspective").member("yfov").real(0.8)) * 180.0f / kPi;
+        scene_.file_views.push_back(v);
+    }
+
+    // A KHR_lights_punctual light at the node's origin, shining down its -z axis.
+    void lamp(Node l, const Xform& world) {
+        const vec3 colour = l.holds("color") ? vec3_at(l.member("color"), 1.0f) : vec3(1.0f);
+        const vec3 power = colour * float(l.member("intensity").real(1.0));
+        const vec3 at_pos = apply_point(world, vec3(0.0f));
+        const vec3 dir = normalize(apply_dir(world, vec3(0, 0, -1)));
+        const std::string_view type = l.member("type").text();
+        ph_punctual_light pl{};
+        pl.type = type == "directional" ? PH_LIGHT_DIRECTIONAL : type == "spot" ? PH_LIGHT_SPOT : PH_LIGHT_POINT;
+        for (int k = 0; k < 3; k++) {
+            pl.position[k] = at_pos[k];
+            pl.direction[k] = dir[k];
+            pl.intensity[k] = power[k];
+        }
+        const Node spot = l.member("spot");
+        pl.cos_inner = std::cos(float(spot.member("innerConeAngle").real(0.0)));
+        pl.cos_outer = std::cos(float(spot.member("outerConeAngle").real(double(kPi / 4))));
+        scene_.lamps.push_back(pl);
+    }
+
+    void visit(int index, const Xform& parent, int depth) {
         if (depth > 64) fail("node hierarchy too deep (or cyclic)");
-        const Value& nd = doc_["nodes"][size_t(index)];
-        const Xform world = parent * local_xform(nd);
-        if (nd.has("mesh")) {
-            const Value& prims = doc_["meshes"][size_t(nd["mesh"].integer())]["primitives"];
-            for (size_t i = 0; i < prims.size(); i++) primitive(prims[i], world);
+        const Node nd = at("nodes", index);
+        const Xform world = parent * node_local(nd);
+        if (nd.holds("mesh")) {
+            const Node prims = at("meshes", nd.member("mesh").whole()).member("primitives");
+            for (size_t i = 0; i < prims.length(); i++) primitive(prims.element(i), world);
         }
-        if (nd.has("camera")) {
-            const Value& cam = doc_["cameras"][size_t(nd["camera"].integer())];
-            if (cam["type"].string() == "perspective") {
-                Camera c;
-                c.position = apply_point(world, vec3(0.0f));
-                c.target = c.position + normalize(apply_dir(world, vec3(0, 0, -1)));
-                c.up = normalize(apply_dir(world, vec3(0, 1, 0)));
-                c.vfov = float(cam["perspective"]["yfov"].number(0.8)) * 180.0f / kPi;
-                scene_.cameras.push_back(c);
-            }
-        }
-        const Value& lref = nd["extensions"]["KHR_lights_punctual"];
-        if (lref.has("light")) {
-            const Value& l = doc_["extensions"]["KHR_lights_punctual"]["lights"][size_t(lref["light"].integer())];
-            const vec3 colour = l.has("color") ? vec3_of(l["color"], 1.0f) : vec3(1.0f);
-            const vec3 power = colour * float(l["intensity"].number(1.0));
-            const vec3 at = apply_point(world, vec3(0.0f));
-            const vec3 dir = normalize(apply_dir(world, vec3(0, 0, -1)));
-            const std::string& type = l["type"].string();
-            ph_punctual_light pl{};
-            pl.type = type == "directional" ? PH_LIGHT_DIRECTIONAL : type == "spot" ? PH_LIGHT_SPOT : PH_LIGHT_POINT;
-            const float p3[3] = {at.x, at.y, at.z}, d3[3] = {dir.x, dir.y, dir.z}, i3[3] = {power.x, power.y, power.z};
-            for (int k = 0; k < 3; k++) {
-                pl.position[k] = p3[k];
-                pl.direction[k] = d3[k];
-                pl.intensity[k] = i3[k];
-            }
-            pl.cos_inner = std::cos(float(l["spot"]["innerConeAngle"].number(0.0)));
-            pl.cos_outer = std::cos(float(l["spot"]["outerConeAngle"].number(double(kPi / 4))));
-            scene_.punctual.push_back(pl);
-        }
-        const Value& kids = nd["children"];
-        for (size_t i = 0; i < kids.size(); i++) node(kids[i].integer(), world, depth + 1);
+        if (nd.holds("camera")) camera(at("cameras", nd.member("camera").whole()), world);
+        const Node light_ref = nd.member("extensions").member("KHR_lights_punctual");
+        if (light_ref.holds("light"))
+            lamp(root_.member("extensions").member("KHR_lights_punctual").member("lights").element(size_t(light_ref.member("light").whole())), world);
+        const Node kids = nd.member("children");
+        for (size_t i = 0; i < kids.length(); i++) visit(kids.element(i).whole(), world, depth + 1);
     }
 };
 
-// Appends raw bytes to the export buffer, padded to 4 bytes; returns their offset.
-size_t put(std::vector<uint8_t>& bin, const void* p, size_t n) {
-    const size_t off = bin.size();
-    const uint8_t* b = static_cast<const uint8_t*>(p);
-    bin.insert(bin.end(), b, b + n);
-    while (bin.size() % 4) bin.push_back(0);
-    return off;
+// ---- export ----------------------------------------------------------------------------------------
+
+// The binary payload of an export: arrays appended back to back, each padded to 4 bytes.
+struct Payload {
+    std::vector<uint8_t> bytes;
+    size_t add(const void* p, size_t n) {
+        const size_t off = bytes.size();
+        const uint8_t* b = static_cast<const uint8_t*>(p);
+        bytes.insert(bytes.end(), b, b + n);
+        bytes.resize((bytes.size() + 3) & ~size_t(3), 0);
+        return off;
+    }
+};
+
+// One primitive per material: the vertices its triangles use (scene ids, in order of first use within
+// the primitive) and its triangles' corners as indices into that list.
+struct Part {
+    std::vector<uint32_t> verts;
+    std::vector<uint32_t> corners;
+    size_t first = 0;  // its first vertex in the written streams
+};
+
+std::map<int, Part> parts_by_material(const Mesh& mesh) {
+    std::map<int, Part> parts;
+    std::map<int, std::unordered_map<uint32_t, uint32_t>> local;  // per material: scene vertex -> index in the part
+    for (int t = 0; t < mesh.triangles(); t++) {
+        const int m = mesh.mat[size_t(t)];
+        Part& p = parts[m];
+        auto& ids = local[m];
+        for (int k = 0; k < 3; k++) {
+            const uint32_t v = uint32_t(mesh.corner[size_t(3 * t + k)]);
+            const auto [it, added] = ids.emplace(v, uint32_t(p.verts.size()));
+            if (added) p.verts.push_back(v);
+            p.corners.push_back(it->second);
+        }
+    }
+    return parts;
+}
+
+// A column-major node matrix with axes x, y, z and origin t.
+void node_matrix(json::Writer& w, vec3 x, vec3 y, vec3 z, vec3 t) {
+    const float m[16] = {x.x, x.y, x.z, 0, y.x, y.y, y.z, 0, z.x, z.y, z.z, 0, t.x, t.y, t.z, 1};
+    w.name("matrix").numbers(m, 16);
+}
+
+std::string gltf_text(const Scene& scene) {
+    const Mesh& mesh = scene.mesh;
+    const size_t nv = mesh.pos.size();
+    const bool with_normals = nv > 0 && mesh.nrm.size() == nv;
+    const bool with_uvs = nv > 0 && mesh.uv.size() == nv;
+    std::map<int, Part> parts = parts_by_material(mesh);
+
+    // Vertex streams, part after part, then each part's index list.
+    std::vector<vec3> pos, nrm;
+    std::vector<vec2> uv;
+    for (auto& [m, p] : parts) {
+        p.first = pos.size();
+        for (const uint32_t v : p.verts) {
+            pos.push_back(mesh.pos[v]);
+            if (with_normals) nrm.push_back(mesh.nrm[v]);
+            if (with_uvs) uv.push_back(mesh.uv[v]);
+        }
+    }
+    const size_t n = pos.size();
+    Payload bin;
+    struct View {
+        size_t offset, length;
+    };
+    std::vector<View> views;
+    views.push_back({bin.add(pos.data(), n * 12), n * 12});
+    const int normal_view = with_normals ? int(views.size()) : -1;
+    if (with_normals) views.push_back({bin.add(nrm.data(), n * 12), n * 12});
+    const int uv_view = with_uvs ? int(views.size()) : -1;
+    if (with_uvs) views.push_back({bin.add(uv.data(), n * 8), n * 8});
+
+    struct Accessor {
+        int view;
+        size_t offset, count;
+        int component;
+        const char* type;
+        bool bounds;
+        vec3 lo, hi;
+    };
+    std::vector<Accessor> accessors;
+    struct Prim {
+        int material, position, normal, uv, index;
+    };
+    std::vector<Prim> prims;
+    for (const auto& [m, p] : parts) {
+        vec3 lo(kInf), hi(-kInf);
+        for (const uint32_t v : p.verts) {
+            lo = vmin(lo, mesh.pos[v]);
+            hi = vmax(hi, mesh.pos[v]);
+        }
+        Prim pr{m, int(accessors.size()), -1, -1, -1};
+        accessors.push_back({0, p.first * 12, p.verts.size(), kF32, "VEC3", true, lo, hi});
+        if (with_normals) {
+            pr.normal = int(accessors.size());
+            accessors.push_back({normal_view, p.first * 12, p.verts.size(), kF32, "VEC3", false, {}, {}});
+        }
+        if (with_uvs) {
+            pr.uv = int(accessors.size());
+            accessors.push_back({uv_view, p.first * 8, p.verts.size(), kF32, "VEC2", false, {}, {}});
+        }
+        views.push_back({bin.add(p.corners.data(), p.corners.size() * 4), p.corners.size() * 4});
+        pr.index = int(accessors.size());
+        accessors.push_back({int(views.size()) - 1, 0, p.corners.size(), kU32, "SCALAR", false, {}, {}});
+        prims.push_back(pr);
+    }
+
+    json::Writer w;
+    w.open_object();
+    w.name("asset").open_object().field("version", "2.0").field("generator", "photon_amd").close_object();
+    w.name("extensionsUsed").open_array();
+    for (const char* e : {"KHR_materials_emissive_strength", "KHR_materials_transmission", "KHR_materials_ior", "KHR_lights_punctual"}) w.string(e);
+    w.close_array();
+    w.name("buffers").open_array().open_object().field("byteLength", bin.bytes.size());
+    w.field("uri", "data:application/octet-stream;base64," + base64_encode(bin.bytes)).close_object().close_array();
+    w.name("bufferViews").open_array();
+    for (const View& v : views) w.open_object().field("buffer", 0).field("byteOffset", v.offset).field("byteLength", v.length).close_object();
+    w.close_array();
+    w.name("accessors").open_array();
+    for (const Accessor& a : accessors) {
+        w.open_object().field("bufferView", a.view).field("byteOffset", a.offset).field("componentType", a.component);
+        w.field("count", a.count).field("type", a.type);
+        if (a.bounds) {
+            const float lo[3] = {a.lo.x, a.lo.y, a.lo.z}, hi[3] = {a.hi.x, a.hi.y, a.hi.z};
+            w.name("min").numbers(lo, 3);
+            w.name("max").numbers(hi, 3);
+        }
+        w.close_object();
+    }
+    w.close_array();
+
+    // Emission is written as a factor of at most 1 times KHR_materials_emissive_strength.
+    w.name("materials").open_array();
+    for (const ph_material& m : scene.materials) {
+        const float strength = std::max({m.emission[0], m.emission[1], m.emission[2], 1.0f});
+        const float factor[3] = {m.emission[0] / strength, m.emission[1] / strength, m.emission[2] / strength};
+        w.open_object();
+        w.name("pbrMetallicRoughness").open_object();
+        w.name("baseColorFactor").numbers(m.base_color, 4);
+        w.field("metallicFactor", m.metallic).field("roughnessFactor", m.roughness).close_object();
+        w.name("emissiveFactor").numbers(factor, 3);
+        w.field("doubleSided", m.double_sided != 0);
+        w.name("extensions").open_object();
+        w.name("KHR_materials_emissive_strength").open_object().field("emissiveStrength", strength).close_object();
+        w.name("KHR_materials_transmission").open_object().field("transmissionFactor", m.transmission).close_object();
+        w.name("KHR_materials_ior").open_object().field("ior", m.ior).close_object();
+        w.close_object().close_object();
+    }
+    w.close_array();
+
+    w.name("meshes").open_array().open_object().name("primitives").open_array();
+    for (const Prim& p : prims) {
+        w.open_object().name("attributes").open_object().field("POSITION", p.position);
+        if (p.normal >= 0) w.field("NORMAL", p.normal);
+        if (p.uv >= 0) w.field("TEXCOORD_0", p.uv);
+        w.close_object().field("indices", p.index).field("material", p.material).close_object();
+    }
+    w.close_array().close_object().close_array();
+
+    const Viewpoint& v = scene.view;
+    w.name("cameras").open_array().open_object().field("type", "perspective");
+    w.name("perspective").open_object().field("yfov", v.fov_deg * kPi / 180.0f).field("znear", 0.01).close_object();
+    w.close_object().close_array();
+
+    // Lights: colour of largest component 1, times an intensity.
+    w.name("extensions").open_object().name("KHR_lights_punctual").open_object().name("lights").open_array();
+    for (const ph_punctual_light& l : scene.lamps) {
+        const float power = std::max({l.intensity[0], l.intensity[1], l.intensity[2], 1e-6f});
+        const float colour[3] = {l.intensity[0] / power, l.intensity[1] / power, l.intensity[2] / power};
+        w.open_object().field("type", l.type == PH_LIGHT_DIRECTIONAL ? "directional" : l.type == PH_LIGHT_SPOT ? "spot" : "point");
+        w.name("color").numbers(colour, 3);
+        w.field("intensity", power);
+        if (l.type == PH_LIGHT_SPOT)
+            w.name("spot").open_object().field("innerConeAngle", std::acos(l.cos_inner)).field("outerConeAngle", std::acos(l.cos_outer)).close_object();
+        w.close_object();
+    }
+    w.close_array().close_object().close_object();
+
+    // Nodes: the mesh, the camera (looking down its -z), then one node per light (its -z the light's
+    // direction).
+    w.name("nodes").open_array();
+    w.open_object().field("mesh", 0).close_object();
+    const vec3 f = normalize(v.aim - v.eye), r = normalize(cross(f, v.sky)), u = cross(r, f);
+    w.open_object().field("camera", 0);
+    node_matrix(w, r, u, -f, v.eye);
+    w.close_object();
+    for (size_t i = 0; i < scene.lamps.size(); i++) {
+        const ph_punctual_light& l = scene.lamps[i];
+        const vec3 z = -normalize(vec3(l.direction[0], l.direction[1], l.direction[2]));
+        const vec3 x = normalize(cross(std::fabs(z.y) > 0.99f ? vec3(1, 0, 0) : vec3(0, 1, 0), z));
+        w.open_object();
+        node_matrix(w, x, cross(z, x), z, vec3(l.position[0], l.position[1], l.position[2]));
+        w.name("extensions").open_object().name("KHR_lights_punctual").open_object().field("light", i).close_object().close_object();
+        w.close_object();
+    }
+    w.close_array();
+    w.name("scenes").open_array().open_object().name("nodes").open_array();
+    for (size_t i = 0; i < 2 + scene.lamps.size(); i++) w.integer(int64_t(i));
+    w.close_array().close_object().close_array();
+    w.field("scene", 0);
+    w.close_object();
+    return w.str();
 }
 
 }  // namespace
@@ -665,178 +922,15 @@ bool load_gltf(const std::string& path, Scene& scene, std::string& error) {
         error = f.msg;
         return false;
     }
-    if (scene.name.empty()) scene.name = utf8_path(path).stem().string();
-    if (!scene.cameras.empty()) scene.camera = scene.cameras.front();
+    if (scene.label.empty()) scene.label = utf8_path(path).stem().string();
+    if (!scene.file_views.empty()) scene.view = scene.file_views.front();
     return true;
 }
 
 bool save_gltf(const std::string& path, const Scene& scene, std::string& error) {
-    const size_t nv = scene.positions.size();
-    const bool has_n = scene.normals.size() == nv && nv > 0;
-    const bool has_uv = scene.uvs.size() == nv && nv > 0;
-    // One primitive per material, in material order. Each carries the vertices it references, in order
-    // of first use, so that a file of many materials does not repeat the whole vertex array per
-    // primitive (a vertex shared by several materials is written once for each).
-    struct Group {
-        std::vector<uint32_t> vertices;  // scene vertex ids
-        std::vector<uint32_t> indices;   // into `vertices`
-        size_t first = 0;                // first of its vertices in the written arrays
-    };
-    std::map<int, Group> groups;
-    std::map<int, std::map<uint32_t, uint32_t>> shared;  // local ids of vertices used by a second group
-    std::vector<int> owner(nv, -1);
-    std::vector<uint32_t> local(nv, 0);
-    for (int t = 0; t < scene.num_tris(); t++) {
-        const int m = scene.tri_material[size_t(t)];
-        Group& g = groups[m];
-        for (int k = 0; k < 3; k++) {
-            const uint32_t v = uint32_t(scene.indices[size_t(t) * 3 + size_t(k)]);
-            if (owner[v] < 0) {
-                owner[v] = m;
-                local[v] = uint32_t(g.vertices.size());
-                g.vertices.push_back(v);
-            }
-            if (owner[v] == m) {
-                g.indices.push_back(local[v]);
-                continue;
-            }
-            auto& sh = shared[m];
-            auto it = sh.find(v);
-            if (it == sh.end()) {
-                it = sh.emplace(v, uint32_t(g.vertices.size())).first;
-                g.vertices.push_back(v);
-            }
-            g.indices.push_back(it->second);
-        }
-    }
-    std::vector<vec3> pos, nrm;
-    std::vector<vec2> uvs;
-    for (auto& [m, g] : groups) {
-        g.first = pos.size();
-        for (const uint32_t v : g.vertices) {
-            pos.push_back(scene.positions[v]);
-            if (has_n) nrm.push_back(scene.normals[v]);
-            if (has_uv) uvs.push_back(scene.uvs[v]);
-        }
-    }
-    const size_t nw = pos.size();
-    std::vector<uint8_t> bin;
-    const size_t pos_off = put(bin, pos.data(), nw * sizeof(vec3));
-    const size_t nrm_off = has_n ? put(bin, nrm.data(), nw * sizeof(vec3)) : 0;
-    const size_t uv_off = has_uv ? put(bin, uvs.data(), nw * sizeof(vec2)) : 0;
-    std::vector<std::pair<size_t, size_t>> index_views;  // (offset, count)
-    for (const auto& [m, g] : groups) index_views.push_back({put(bin, g.indices.data(), g.indices.size() * 4), g.indices.size()});
-
-    std::ostringstream o;
-    o.precision(9);
-    auto v3 = [&](vec3 v) {
-        std::ostringstream s;
-        s.precision(9);
-        s << "[" << v.x << ", " << v.y << ", " << v.z << "]";
-        return s.str();
-    };
-    o << "{\n\"asset\": {\"version\": \"2.0\", \"generator\": \"photon_amd\"},\n";
-    o << "\"extensionsUsed\": [\"KHR_materials_emissive_strength\", \"KHR_materials_transmission\", \"KHR_materials_ior\", \"KHR_lights_punctual\"],\n";
-    o << "\"buffers\": [{\"byteLength\": " << bin.size() << ", \"uri\": \"data:application/octet-stream;base64," << base64_encode(bin) << "\"}],\n";
-    // Views: positions, normals and uvs when present, then one index list per group. Accessors: per
-    // group its ranges of the vertex views (positions with their bounds), then its indices.
-    std::vector<std::string> views, accessors;
-    auto view = [&](size_t off, size_t len) {
-        views.push_back("{\"buffer\": 0, \"byteOffset\": " + std::to_string(off) + ", \"byteLength\": " + std::to_string(len) + "}");
-        return int(views.size()) - 1;
-    };
-    auto accessor = [&](int v, size_t byte_off, int ct, size_t count, const char* type, const std::string& extra) {
-        accessors.push_back("{\"bufferView\": " + std::to_string(v) + ", \"byteOffset\": " + std::to_string(byte_off) + ", \"componentType\": " +
-                            std::to_string(ct) + ", \"count\": " + std::to_string(count) + ", \"type\": \"" + type + "\"" + extra + "}");
-        return int(accessors.size()) - 1;
-    };
-    const int pos_view = view(pos_off, nw * 12);
-    const int nrm_view = has_n ? view(nrm_off, nw * 12) : -1;
-    const int uv_view = has_uv ? view(uv_off, nw * 8) : -1;
-    struct Prim {
-        int material, pos, nrm, uv, idx;
-    };
-    std::vector<Prim> prims;
-    size_t gi = 0;
-    for (const auto& [m, g] : groups) {
-        vec3 lo(kInf), hi(-kInf);
-        for (const uint32_t v : g.vertices) {
-            lo = vmin(lo, scene.positions[v]);
-            hi = vmax(hi, scene.positions[v]);
-        }
-        const size_t n = g.vertices.size();
-        Prim pr{m, accessor(pos_view, g.first * 12, 5126, n, "VEC3", ", \"min\": " + v3(lo) + ", \"max\": " + v3(hi)), -1, -1, -1};
-        if (has_n) pr.nrm = accessor(nrm_view, g.first * 12, 5126, n, "VEC3", "");
-        if (has_uv) pr.uv = accessor(uv_view, g.first * 8, 5126, n, "VEC2", "");
-        pr.idx = accessor(view(index_views[gi].first, index_views[gi].second * 4), 0, 5125, index_views[gi].second, "SCALAR", "");
-        prims.push_back(pr);
-        gi++;
-    }
-    auto list = [&](const std::vector<std::string>& items) {
-        std::string s;
-        for (size_t i = 0; i < items.size(); i++) s += (i ? ", " : "") + items[i];
-        return s;
-    };
-    o << "\"bufferViews\": [" << list(views) << "],\n\"accessors\": [" << list(accessors) << "],\n\"materials\": [";
-    for (size_t i = 0; i < scene.materials.size(); i++) {
-        const ph_material& m = scene.materials[i];
-        // Emission splits into a factor of at most 1 and a strength.
-        const float strength = std::max({m.emission[0], m.emission[1], m.emission[2], 1.0f});
-        const vec3 factor = vec3(m.emission[0], m.emission[1], m.emission[2]) / strength;
-        o << (i ? ", " : "") << "{\"pbrMetallicRoughness\": {\"baseColorFactor\": [" << m.base_color[0] << ", " << m.base_color[1] << ", " << m.base_color[2]
-          << ", " << m.base_color[3] << "], \"metallicFactor\": " << m.metallic << ", \"roughnessFactor\": " << m.roughness << "}, \"emissiveFactor\": " << v3(factor)
-          << ", \"doubleSided\": " << (m.double_sided ? "true" : "false") << ", \"extensions\": {\"KHR_materials_emissive_strength\": {\"emissiveStrength\": "
-          << strength << "}, \"KHR_materials_transmission\": {\"transmissionFactor\": " << m.transmission << "}, \"KHR_materials_ior\": {\"ior\": " << m.ior
-          << "}}}";
-    }
-    o << "],\n\"meshes\": [{\"primitives\": [";
-    for (size_t i = 0; i < prims.size(); i++) {
-        const Prim& pr = prims[i];
-        o << (i ? ", " : "") << "{\"attributes\": {\"POSITION\": " << pr.pos;
-        if (pr.nrm >= 0) o << ", \"NORMAL\": " << pr.nrm;
-        if (pr.uv >= 0) o << ", \"TEXCOORD_0\": " << pr.uv;
-        o << "}, \"indices\": " << pr.idx << ", \"material\": " << pr.material << "}";
-    }
-    o << "]}],\n";
-    const Camera& c = scene.camera;
-    const vec3 f = normalize(c.target - c.position), r = normalize(cross(f, c.up)), u = cross(r, f);
-    o << "\"cameras\": [{\"type\": \"perspective\", \"perspective\": {\"yfov\": " << c.vfov * kPi / 180.0f << ", \"znear\": 0.01}}],\n";
-    o << "\"extensions\": {\"KHR_lights_punctual\": {\"lights\": [";
-    for (size_t i = 0; i < scene.punctual.size(); i++) {
-        const ph_punctual_light& l = scene.punctual[i];
-        const float power = std::max({l.intensity[0], l.intensity[1], l.intensity[2], 1e-6f});
-        const vec3 colour = vec3(l.intensity[0], l.intensity[1], l.intensity[2]) / power;
-        const char* type = l.type == PH_LIGHT_DIRECTIONAL ? "directional" : l.type == PH_LIGHT_SPOT ? "spot" : "point";
-        o << (i ? ", " : "") << "{\"type\": \"" << type << "\", \"color\": " << v3(colour) << ", \"intensity\": " << power;
-        if (l.type == PH_LIGHT_SPOT) o << ", \"spot\": {\"innerConeAngle\": " << std::acos(l.cos_inner) << ", \"outerConeAngle\": " << std::acos(l.cos_outer) << "}";
-        o << "}";
-    }
-    o << "]}},\n";
-    // Node matrices, column major: x (right), y (up), z (backwards), translation.
-    auto matrix = [&](vec3 x, vec3 y, vec3 z, vec3 t) {
-        std::ostringstream s;
-        s.precision(9);
-        s << "[" << x.x << ", " << x.y << ", " << x.z << ", 0, " << y.x << ", " << y.y << ", " << y.z << ", 0, " << z.x << ", " << z.y << ", " << z.z << ", 0, " << t.x
-          << ", " << t.y << ", " << t.z << ", 1]";
-        return s.str();
-    };
-    o << "\"nodes\": [{\"mesh\": 0}, {\"camera\": 0, \"matrix\": " << matrix(r, u, -f, c.position) << "}";
-    for (size_t i = 0; i < scene.punctual.size(); i++) {
-        const ph_punctual_light& l = scene.punctual[i];
-        const vec3 z = -normalize(vec3(l.direction[0], l.direction[1], l.direction[2]));
-        const vec3 x = normalize(cross(std::fabs(z.y) > 0.99f ? vec3(1, 0, 0) : vec3(0, 1, 0), z));
-        o << ", {\"matrix\": " << matrix(x, cross(z, x), z, vec3(l.position[0], l.position[1], l.position[2])) << ", \"extensions\": {\"KHR_lights_punctual\": {\"light\": " << i
-          << "}}}";
-    }
-    o << "],\n\"scenes\": [{\"nodes\": [0, 1";
-    for (size_t i = 0; i < scene.punctual.size(); i++) o << ", " << i + 2;
-    o << "]}],\n\"scene\": 0\n}\n";
+    const std::string text = gltf_text(scene);
     std::ofstream out(utf8_path(path), std::ios::binary);
-    if (!out) {
-        error = "cannot write " + path;
-        return false;
-    }
-    out << o.str();
+    if (out) out.write(text.data(), std::streamsize(text.size()));
     if (!out) {
         error = "cannot write " + path;
         return false;

@@ -91,7 +91,7 @@ uint32_t sample_at(const uint8_t* row, size_t k, int depth) {
     return (row[bit / 8] >> shift) & ((1u << depth) - 1u);
 }
 
-bool decode_png(const uint8_t* data, size_t size, Texture& out, std::string& error) {
+bool decode_png(const uint8_t* data, size_t size, Image8& out, std::string& error) {
     static const uint8_t sig[8] = {0x89, 'P', 'N', 'G', '\r', '\n', 0x1A, '\n'};
     if (size < 8 || std::memcmp(data, sig, 8) != 0) {
         error = "not a PNG file";
@@ -192,7 +192,7 @@ bool decode_png(const uint8_t* data, size_t size, Texture& out, std::string& err
     };
     out.width = int(hd.width);
     out.height = int(hd.height);
-    out.texels.assign(size_t(hd.width) * hd.height, 0);
+    out.px.assign(size_t(hd.width) * hd.height, 0);
     const int bpp_bytes = (hd.bits_per_pixel() + 7) / 8;
     size_t at = 0;
     std::vector<uint8_t> rows;
@@ -229,7 +229,7 @@ bool decode_png(const uint8_t* data, size_t size, Texture& out, std::string& err
                     px = rgba(to8(s[0]), to8(s[1]), to8(s[2]), to8(s[3]));
                 }
                 const size_t X = size_t(p.x0) + size_t(x) * p.dx, Y = size_t(p.y0) + size_t(y) * p.dy;
-                out.texels[Y * hd.width + X] = px;
+                out.px[Y * hd.width + X] = px;
             }
         }
     }
@@ -238,7 +238,7 @@ bool decode_png(const uint8_t* data, size_t size, Texture& out, std::string& err
 
 // ---- Radiance RGBE ------------------------------------------------------------------------------------
 
-bool decode_hdr(const std::vector<uint8_t>& f, Environment& env, std::string& error) {
+bool decode_hdr(const std::vector<uint8_t>& f, Backdrop& env, std::string& error) {
     size_t pos = 0;
     auto line = [&](std::string& s) {
         s.clear();
@@ -323,14 +323,14 @@ bool decode_hdr(const std::vector<uint8_t>& f, Environment& env, std::string& er
     }
     env.width = w;
     env.height = h;
-    env.texels.resize(size_t(w) * size_t(h));
-    for (size_t i = 0; i < env.texels.size(); i++) {
+    env.rgb.resize(size_t(w) * size_t(h));
+    for (size_t i = 0; i < env.rgb.size(); i++) {
         const uint8_t* p = &rgbe[4 * i];
         if (p[3] == 0) {
-            env.texels[i] = vec3(0);
+            env.rgb[i] = vec3(0);
         } else {
             const float scale = std::ldexp(1.0f, int(p[3]) - 136);
-            env.texels[i] = vec3(float(p[0]) * scale, float(p[1]) * scale, float(p[2]) * scale);
+            env.rgb[i] = vec3(float(p[0]) * scale, float(p[1]) * scale, float(p[2]) * scale);
         }
     }
     return true;
@@ -352,14 +352,14 @@ void put_chunk(std::vector<uint8_t>& o, const char* type, const uint8_t* body, s
 
 }  // namespace
 
-bool decode_image(const uint8_t* data, size_t size, Texture& out, std::string& error) {
+bool decode_image(const uint8_t* data, size_t size, Image8& out, std::string& error) {
     if (size >= 8 && data[0] == 0x89 && data[1] == 'P') return decode_png(data, size, out, error);
     if (size >= 3 && data[0] == 0xFF && data[1] == 0xD8 && data[2] == 0xFF) return decode_jpeg(data, size, out, error);
     error = "unsupported image format (PNG and baseline JPEG only)";
     return false;
 }
 
-bool load_image(const std::string& path, Texture& out, std::string& error) {
+bool load_image(const std::string& path, Image8& out, std::string& error) {
     std::vector<uint8_t> data;
     if (!read_all(path, data)) {
         error = "cannot open " + path;
@@ -372,7 +372,7 @@ bool load_image(const std::string& path, Texture& out, std::string& error) {
     return true;
 }
 
-bool load_environment(const std::string& path, Environment& env, std::string& error) {
+bool load_environment(const std::string& path, Backdrop& env, std::string& error) {
     std::vector<uint8_t> data;
     if (!read_all(path, data)) {
         error = "cannot open " + path;
@@ -392,24 +392,24 @@ bool load_environment(const std::string& path, Environment& env, std::string& er
         }
         env.width = w;
         env.height = h;
-        env.texels.resize(size_t(w) * size_t(h));
-        for (size_t i = 0; i < env.texels.size(); i++) env.texels[i] = vec3(rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2]);
+        env.rgb.resize(size_t(w) * size_t(h));
+        for (size_t i = 0; i < env.rgb.size(); i++) env.rgb[i] = vec3(rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2]);
     } else {
-        Texture t;
+        Image8 t;
         if (!decode_image(data.data(), data.size(), t, error)) {
             error = path + ": " + error;
             return false;
         }
         env.width = t.width;
         env.height = t.height;
-        env.texels.resize(t.texels.size());
-        for (size_t i = 0; i < t.texels.size(); i++) {
-            const uint32_t p = t.texels[i];
-            env.texels[i] = vec3(srgb_to_linear(float(p & 0xFF) / 255.0f), srgb_to_linear(float((p >> 8) & 0xFF) / 255.0f),
+        env.rgb.resize(t.px.size());
+        for (size_t i = 0; i < t.px.size(); i++) {
+            const uint32_t p = t.px[i];
+            env.rgb[i] = vec3(srgb_to_linear(float(p & 0xFF) / 255.0f), srgb_to_linear(float((p >> 8) & 0xFF) / 255.0f),
                                  srgb_to_linear(float((p >> 16) & 0xFF) / 255.0f));
         }
     }
-    for (vec3& t : env.texels)
+    for (vec3& t : env.rgb)
         if (!std::isfinite(t.x) || !std::isfinite(t.y) || !std::isfinite(t.z) || t.x < 0 || t.y < 0 || t.z < 0) t = vec3(0);
     return true;
 }

@@ -188,7 +188,7 @@ std::vector<uint8_t> upsample(const Component& c, int hmax, int vmax, int W, int
     return out;
 }
 
-void decode(const uint8_t* d, size_t n, Texture& out) {
+void decode(const uint8_t* d, size_t n, Image8& out) {
     if (n < 4 || d[0] != 0xFF || d[1] != 0xD8) fail("missing SOI marker");
     int qt[4][64] = {};
     bool qt_set[4] = {};
@@ -368,10 +368,10 @@ void decode(const uint8_t* d, size_t n, Texture& out) {
     if (!frame || !scanned) fail("no image data");
     out.width = W;
     out.height = H;
-    out.texels.assign(size_t(W) * size_t(H), 0);
+    out.px.assign(size_t(W) * size_t(H), 0);
     if (comps.size() == 1) {
         const std::vector<uint8_t> g = upsample(comps[0], hmax, vmax, W, H);
-        for (size_t i = 0; i < g.size(); i++) out.texels[i] = uint32_t(g[i]) * 0x010101u | 0xFF000000u;
+        for (size_t i = 0; i < g.size(); i++) out.px[i] = uint32_t(g[i]) * 0x010101u | 0xFF000000u;
         return;
     }
     const std::vector<uint8_t> a = upsample(comps[0], hmax, vmax, W, H), b = upsample(comps[1], hmax, vmax, W, H), c = upsample(comps[2], hmax, vmax, W, H);
@@ -389,13 +389,13 @@ void decode(const uint8_t* d, size_t n, Texture& out) {
             G = int(std::lround(y - 0.344136f * cb - 0.714136f * cr));
             B = int(std::lround(y + 1.772f * cb));
         }
-        out.texels[i] = uint32_t(clamp8(R)) | uint32_t(clamp8(G)) << 8 | uint32_t(clamp8(B)) << 16 | 0xFF000000u;
+        out.px[i] = uint32_t(clamp8(R)) | uint32_t(clamp8(G)) << 8 | uint32_t(clamp8(B)) << 16 | 0xFF000000u;
     }
 }
 
 }  // namespace
 
-bool decode_jpeg(const uint8_t* data, size_t size, Texture& out, std::string& error) {
+bool decode_jpeg(const uint8_t* data, size_t size, Image8& out, std::string& error) {
     try {
         decode(data, size, out);
         return true;

@@ -28,7 +28,7 @@ def test_json_parser_reference_case():
     """test_gltf.cpp:47-65: values, escapes, UTF-8, and an error on a malformed array."""
     text = '{"a": [1, 2.5e1, -3], "b": {"c": "xé\\n"}, "t": true, "n": null}'
     assert json.loads(host.json_canonical(text)) == json.loads(text)
-    with pytest.raises(RuntimeError, match="JSON error at line 1"):
+    with pytest.raises(RuntimeError, match=r"json: .*\(line 1, column 12\)"):
         host.json_canonical('{"a": [1, 2}')
 
 
@@ -62,7 +62,7 @@ def test_json_matches_python_json_on_random_documents():
 def test_json_rejects_what_python_rejects(bad):
     with pytest.raises(ValueError):
         json.loads(bad)
-    with pytest.raises(RuntimeError, match="JSON error"):
+    with pytest.raises(RuntimeError, match="json: "):
         host.json_canonical(bad)
 
 
@@ -191,7 +191,7 @@ def test_gltf_errors_are_reported(tmp_path):
     with pytest.raises(RuntimeError, match="cannot open"):
         host.Scene.from_gltf(tmp_path / "does_not_exist.gltf")
     (tmp_path / "broken.gltf").write_text('{"asset": {"version": "2.0"}, "meshes": [')
-    with pytest.raises(RuntimeError, match="JSON error"):
+    with pytest.raises(RuntimeError, match="json: .*line 1"):
         host.Scene.from_gltf(tmp_path / "broken.gltf")
     cases = {
         "only glTF 2.0": {"asset": {"version": "1.0"}},
