@@ -449,7 +449,7 @@ def bench_serving(args, gpu, host, field, launch):
               host.Scene.spheres(256, box=100.0, alpha_fraction=0.1, seed=1).build(), field]
     log(f"serving scenes: {[s.num_tris for s in scenes]} triangles, setup {time.time() - t:.1f} s")
     srv = gpu.Server(path_slots=args.serve_slots, iterations_per_step=args.serve_iterations,
-                     schedule={"fifo": gpu.SCHEDULE_FIFO, "srpt": gpu.SCHEDULE_SRPT, "hrrn": gpu.SCHEDULE_HRRN}[args.serve_schedule],
+                     schedule={"fifo": gpu.SCHEDULE_FIFO, "srpt": gpu.SCHEDULE_SRPT, "hrrn": gpu.SCHEDULE_HRRN, "share": gpu.SCHEDULE_SHARE}[args.serve_schedule],
                      pass_samples=args.serve_pass_samples, min_items=args.serve_min_items,
                      express_slots=args.serve_express_slots, express_samples=args.serve_express_samples,
                      express_iterations=args.serve_express_iterations)
@@ -512,17 +512,32 @@ def bench_serving(args, gpu, host, field, launch):
             reported = len(lat) // 500 * 500
             log(f"serving: {len(lat)} of {len(jobs)} jobs done")
     end = srv.now_ms()
-    # Isolated service time: each job again, alone on the idle server (submission to completion).
-    iso = np.zeros(len(jobs))
-    for k, job in enumerate(jobs):
+    # Isolated service time: each job again, alone on the idle server (submission to completion). With
+    # --serve-isolated classes, each distinct (scene, resolution, spp, depth) is timed once alone, on the
+    # first job of the stream in that class, and stands for every job of the class.
+    def alone_ms(job):
         j = submit(job)
         while srv.status(j).state != gpu.JOB_DONE:
             srv.step()
         st = srv.status(j)
-        iso[k] = st.done_ms - st.submit_ms
         srv.release(j)
-        if (k + 1) % 500 == 0:
-            log(f"serving: {k + 1} of {len(jobs)} jobs timed alone")
+        return st.done_ms - st.submit_ms
+
+    iso = np.zeros(len(jobs))
+    if args.serve_isolated == "classes":
+        key = lambda job: (job["scene"], job["w"], job["h"], job["spp"], job["depth"])  # noqa: E731
+        timed = {}
+        for k, job in enumerate(jobs):
+            if key(job) not in timed:
+                timed[key(job)] = alone_ms(job)
+                if len(timed) % 24 == 0:
+                    log(f"serving: {len(timed)} job classes timed alone")
+            iso[k] = timed[key(job)]
+    else:
+        for k, job in enumerate(jobs):
+            iso[k] = alone_ms(job)
+            if (k + 1) % 500 == 0:
+                log(f"serving: {k + 1} of {len(jobs)} jobs timed alone")
     srv.close()
     L = np.array([lat[k] for k in range(len(jobs))])
     samples = np.array([j["w"] * j["h"] * j["spp"] for j in jobs], np.float64)
@@ -542,7 +557,8 @@ def bench_serving(args, gpu, host, field, launch):
                isolated_ms={"p50": round(float(np.percentile(iso, 50)), 2), "p99": round(float(np.percentile(iso, 99)), 2)},
                by_job_size=by_size, path_slots=args.serve_slots, iterations_per_step=args.serve_iterations, schedule=args.serve_schedule,
                pass_samples=args.serve_pass_samples, min_items=args.serve_min_items, express_slots=args.serve_express_slots,
-               express_samples=args.serve_express_samples, express_iterations=args.serve_express_iterations)
+               express_samples=args.serve_express_samples, express_iterations=args.serve_express_iterations,
+               isolated=args.serve_isolated)
     worst = np.argsort(slow)[-5:][::-1]
     res["worst_slowdown"] = [dict(res=f"{jobs[k]['w']}x{jobs[k]['h']}", spp=jobs[k]["spp"], depth=jobs[k]["depth"], scene=jobs[k]["scene"],
                                   latency_ms=round(float(L[k]), 1), isolated_ms=round(float(iso[k]), 1), slowdown=round(float(slow[k]), 1))
@@ -611,10 +627,13 @@ def main():
     ap.add_argument("--svgf-frames", type=int, default=64)
     ap.add_argument("--serve-jobs", type=int, default=200, help="jobs of the open-loop serving run (the full stream is 10,000)")
     ap.add_argument("--serve-burst", type=int, default=48)
+    ap.add_argument("--serve-isolated", choices=("jobs", "classes"), default="jobs",
+                    help="isolated service time of every job alone, or of each (scene, size, spp, depth) class once")
+    ap.add_argument("--serving-only", action="store_true", help="run only the serving leg (needs --launch)")
     ap.add_argument("--serve-load", type=float, default=0.8)
     ap.add_argument("--serve-slots", type=int, default=1 << 21)
     ap.add_argument("--serve-iterations", type=int, default=0, help="bounces per server step (0: the server's default)")
-    ap.add_argument("--serve-schedule", choices=("srpt", "fifo", "hrrn"), default="srpt")
+    ap.add_argument("--serve-schedule", choices=("srpt", "fifo", "hrrn", "share"), default="srpt")
     ap.add_argument("--serve-pass-samples", type=int, default=0, help="pixel samples per job pass (0: the server's default)")
     ap.add_argument("--serve-min-items", type=int, default=0, help="split smaller jobs into sample sub-streams (0: the server's default)")
     ap.add_argument("--serve-express-slots", type=int, default=0, help="path slots of the express lane (0: the server's default, -1: none)")
@@ -643,6 +662,13 @@ def main():
         log(f"calibration copy: {nbytes} bytes")
         src.free()
         dst.free()
+    if args.serving_only:
+        if not args.launch:
+            sys.exit("--serving-only needs --launch (the launch configuration the render leg would tune)")
+        field = host.Scene.field(1024, seed=1, env_width=2048).build()
+        launch = {k: int(v) for k, v in (item.split("=") for item in args.launch.split(","))}
+        print(json.dumps({"serving": bench_serving(args, gpu, host, field, launch)}), flush=True)
+        return
     scene, cam, rs, rnd = bench_render(args, gpu, host, RenderSettings)
     cornell = None if args.no_cornell else bench_cornell(args, gpu, host, not args.no_cpu)
     trav = None if args.no_traversal else bench_traversal(gpu, host, not args.no_cpu)

@@ -239,7 +239,7 @@ typedef struct ph_server_config {
     int32_t iterations_per_step; /* bounces per ph_server_step (0: 4) */
     int32_t max_jobs;            /* jobs admitted at once, at most 1024 (0: 1024) */
     int64_t pass_samples;        /* pixel samples per pass of a job (0: 4 x path_slots) */
-    int32_t schedule;            /* PH_SCHEDULE_SRPT or PH_SCHEDULE_FIFO */
+    int32_t schedule;            /* PH_SCHEDULE_SRPT, _FIFO, _HRRN or _SHARE */
     int32_t min_items;           /* a job with fewer pixels runs each pixel's samples as sub-streams so that
                                     pixels x sub-streams reaches this (0: an eighth of the main lane's slots,
                                     all of the express lane's; -1 or 1: never split) */
@@ -254,6 +254,9 @@ typedef struct ph_server_config {
 #define PH_SCHEDULE_FIFO 1 /* passes in the order they were queued (round robin over running jobs) */
 #define PH_SCHEDULE_HRRN 2 /* highest response ratio next: (waiting time + estimated remaining service) / estimated
                               remaining service, re-evaluated every step; evens out slowdown across job sizes */
+#define PH_SCHEDULE_SHARE 3 /* processor sharing: the running jobs' unstarted work items are handed out in
+                               round-robin slices, so every running job gets an equal share of the slots that
+                               free up (a job that cannot use its share leaves it to the others) */
 
 typedef struct ph_job_desc {
     int32_t scene; /* id returned by ph_server_add_scene */

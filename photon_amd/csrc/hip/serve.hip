@@ -91,7 +91,7 @@ struct ph_server {
         if (cfg.path_slots <= 0) cfg.path_slots = 1 << 21;
         if (cfg.iterations_per_step <= 0) cfg.iterations_per_step = 4;
         if (cfg.max_jobs <= 0 || cfg.max_jobs > WavefrontEngine::kMaxJobs) cfg.max_jobs = WavefrontEngine::kMaxJobs;
-        if (cfg.schedule != PH_SCHEDULE_SRPT && cfg.schedule != PH_SCHEDULE_FIFO && cfg.schedule != PH_SCHEDULE_HRRN)
+        if (cfg.schedule != PH_SCHEDULE_SRPT && cfg.schedule != PH_SCHEDULE_FIFO && cfg.schedule != PH_SCHEDULE_HRRN && cfg.schedule != PH_SCHEDULE_SHARE)
             throw std::invalid_argument("unknown schedule");
         if (cfg.express_slots == 0) cfg.express_slots = std::max(1, cfg.path_slots / 2);
         if (cfg.express_slots < 0) cfg.express_slots = 0;
@@ -109,6 +109,8 @@ struct ph_server {
             L.min_items = cfg.min_items > 0 ? cfg.min_items : cfg.min_items < 0 ? 1 : (k == 0 ? std::max(1, L.slots / 8) : L.slots);
             PH_CHECK(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
             L.engine.serve_begin(L.stream, L.slots);
+            // Processor sharing hands out a few waves' worth of items per job per round.
+            if (cfg.schedule == PH_SCHEDULE_SHARE) L.engine.serve_share(std::max(64, L.slots / 2048));
         }
     }
     ~ph_server() {
@@ -166,7 +168,7 @@ struct ph_server {
     // ratio (waiting time + estimated remaining service) / estimated remaining service.
     double priority_of(const Lane& L, const Job& j, int k0) const {
         const double remaining = double(j.items) * double(j.desc.spp / j.split - k0);
-        if (cfg.schedule == PH_SCHEDULE_FIFO) return 0.0;
+        if (cfg.schedule == PH_SCHEDULE_FIFO || cfg.schedule == PH_SCHEDULE_SHARE) return 0.0;
         if (cfg.schedule == PH_SCHEDULE_SRPT) return remaining;
         const double service = remaining / std::max(L.rate, 1e-3);
         const double waited = now_ms() - j.status.submit_ms;

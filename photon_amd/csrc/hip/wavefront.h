@@ -59,6 +59,9 @@ public:
     bool serve_enqueue(int entry, int k0, int k1, double priority);
     // New priority for the queued, not yet started work of an entry (the next serve_run lays it out again).
     void serve_reprioritize(int entry, double priority);
+    // Round-robin layout of the running passes' unstarted items, `quantum` items of each per round
+    // (processor sharing); 0 lays each pass out in one piece, in priority order.
+    void serve_share(int quantum);
     void serve_run(hipStream_t st, int iterations);
     int serve_pixels_done(int entry) const;
     void serve_release(int entry);

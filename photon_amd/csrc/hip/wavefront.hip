@@ -816,13 +816,15 @@ struct WavefrontEngine::Impl {
     std::vector<bool> entry_used;
     std::vector<int> entry_pixels;
     // A queued pass: its unstarted items are [next, pixels) of the job's pixel order. While laid out,
-    // its items occupy [first, first + pixels - next) of the item index space.
+    // its items occupy the segments (first, length) of the item index space, in pixel order: one
+    // segment for a contiguous layout, one per round for a round-robin one.
     struct Pass {
         int entry, k0, k1, pixels, next;
         double priority;
         uint64_t order;
-        long long first;
+        std::vector<std::pair<long long, long long>> segments;
     };
+    int share_quantum = 0;  // > 0: round-robin layout, this many items of each pass per round
     std::vector<Pass> passes;  // laid out in this order at the last run
     uint64_t pass_counter = 0;
     bool chunks_dirty = false;
@@ -1293,7 +1295,7 @@ bool WavefrontEngine::serve_enqueue(int entry, int k0, int k1, double priority) 
     if (entry < 0 || entry >= kMaxJobs || !m.entry_used[size_t(entry)]) throw std::invalid_argument("not an admitted job");
     if (k0 < 0 || k1 <= k0) throw std::invalid_argument("empty pass");
     if (int(m.passes.size()) >= kMaxChunks) return false;
-    m.passes.push_back(Impl::Pass{entry, k0, k1, m.entry_pixels[size_t(entry)], 0, priority, m.pass_counter++, -1});
+    m.passes.push_back(Impl::Pass{entry, k0, k1, m.entry_pixels[size_t(entry)], 0, priority, m.pass_counter++, {}});
     m.chunks_dirty = true;
     return true;
 }
@@ -1307,6 +1309,11 @@ void WavefrontEngine::serve_reprioritize(int entry, double priority) {
         }
 }
 
+void WavefrontEngine::serve_share(int quantum) {
+    impl_->share_quantum = std::max(0, quantum);
+    impl_->chunks_dirty = true;
+}
+
 void WavefrontEngine::serve_run(hipStream_t st, int iterations) {
     Impl& m = *impl_;
     if (!m.serving) throw std::runtime_error("serve_begin was not called");
@@ -1317,11 +1324,10 @@ void WavefrontEngine::serve_run(hipStream_t st, int iterations) {
     bool changed = m.chunks_dirty;
     std::vector<Impl::Pass> keep;
     for (Impl::Pass p : m.passes) {
-        if (p.first >= 0) {
-            const long long got = std::clamp<long long>(handed - p.first, 0, p.pixels - p.next);
-            if (got > 0) changed = true;
-            p.next += int(got);
-        }
+        long long got = 0;
+        for (const auto& [first, len] : p.segments) got += std::clamp<long long>(handed - first, 0, len);
+        if (got > 0) changed = true;
+        p.next += int(got);
         if (p.next < p.pixels) keep.push_back(p);
     }
     if (changed) {
@@ -1330,12 +1336,38 @@ void WavefrontEngine::serve_run(hipStream_t st, int iterations) {
         std::vector<long long> starts;
         std::vector<int4> descs;
         long long at = handed;
-        for (Impl::Pass& p : keep) {
-            p.first = at;
+        auto place = [&](Impl::Pass& p, int from, long long count) {
+            p.segments.push_back({at, count});
             starts.push_back(at);
-            descs.push_back(make_int4(p.entry, p.k0, p.k1, p.next));
-            at += p.pixels - p.next;
+            descs.push_back(make_int4(p.entry, p.k0, p.k1, from));
+            at += count;
+        };
+        std::vector<int> pos(keep.size());
+        for (size_t i = 0; i < keep.size(); i++) {
+            pos[i] = keep[i].next;
+            keep[i].segments.clear();
         }
+        if (m.share_quantum > 0) {
+            // Processor sharing: rounds of `share_quantum` items from every pass with work left, so slots
+            // freed during a run go to the running jobs in equal shares. When the chunk table would
+            // overflow, the rest of each pass follows in one piece, in priority order.
+            const size_t budget = size_t(kMaxChunks) - keep.size();
+            bool left = true;
+            while (left && starts.size() + keep.size() <= budget) {
+                left = false;
+                for (size_t i = 0; i < keep.size(); i++) {
+                    const long long rest = keep[i].pixels - pos[i];
+                    if (rest <= 0) continue;
+                    const long long q = std::min<long long>(rest, m.share_quantum);
+                    place(keep[i], pos[i], q);
+                    pos[i] += int(q);
+                    left = left || q < rest;
+                }
+            }
+        }
+        for (size_t i = 0; i < keep.size(); i++)
+            if (keep[i].pixels > pos[i]) place(keep[i], pos[i], keep[i].pixels - pos[i]);
+        if (starts.size() > size_t(kMaxChunks)) throw std::runtime_error("serving: more queued passes than the chunk table holds");
         m.total_work = at;
         if (!starts.empty()) {
             PH_CHECK(hipMemcpyAsync(m.chunk_first.ptr, starts.data(), starts.size() * sizeof(long long), hipMemcpyHostToDevice, st));

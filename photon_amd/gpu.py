@@ -61,7 +61,7 @@ class JobStatus(C.Structure):
 
 
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_RELEASED = range(4)
-SCHEDULE_SRPT, SCHEDULE_FIFO, SCHEDULE_HRRN = 0, 1, 2
+SCHEDULE_SRPT, SCHEDULE_FIFO, SCHEDULE_HRRN, SCHEDULE_SHARE = 0, 1, 2, 3
 
 
 class SvgfInput(C.Structure):

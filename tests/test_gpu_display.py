@@ -56,6 +56,7 @@ def test_display_refuses_pageable_and_short_targets():
     pinned = gpu.PinnedArray(w * h * 4)
     assert gpu.pointer_kind(pinned.ptr) == gpu.POINTER_PINNED_HOST
     r.display(rs, DISPLAY_COLOR, target=pinned.ptr)
+    r.display(rs, DISPLAY_COLOR)  # into the renderer's own buffer, which download_display reads
     assert np.array_equal(pinned.numpy(np.uint32, (h, w)), r.download_display().reshape(h, w))
     short.free()
     pinned.free()
