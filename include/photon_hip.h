@@ -222,6 +222,21 @@ int ph_svgf_run(ph_svgf* f, const ph_svgf_input* in, const ph_camera_view* previ
  * history. */
 int ph_svgf_run_compact(ph_svgf* f, const ph_compact_gbuffer* in, const ph_camera_view* camera, const ph_camera_view* previous_camera,
                         const ph_svgf_settings* settings, float* output_dev);
+/* ph_svgf_run without an allocation: no counterpart in the reference. For callers that cannot call
+ * hipMalloc (code under HIP graph capture, tight serving loops, harnesses that forbid allocation). It runs
+ * the same nine passes as ph_svgf_run on one frame of at most PH_SVGF_STATIC_MAX_PIXELS pixels (larger
+ * frames fail with a message). The frame-local planes (illumination, moments, two a-trous planes: 64
+ * bytes per pixel) live in one static device array that each GPU the library runs on holds until the
+ * process ends (PH_SVGF_STATIC_MAX_PIXELS * 64 bytes = 92 MB). The history that ph_svgf keeps inside
+ * the object is the caller's instead: `history_dev` holds four float4 planes of width*height pixels
+ * (illumination + history length, moments, normal+depth, position+material), is read when
+ * `has_history` is nonzero (0 for the first frame, as after ph_svgf_reset) and is rewritten by the call.
+ * Runs on the default stream and returns when the frame is done. Every call on a GPU shares that GPU's
+ * array: calls that overlap on one GPU (from several host threads) race and must be serialised by the
+ * caller. */
+#define PH_SVGF_STATIC_MAX_PIXELS (1600 * 900)
+int ph_svgf_run_static(const ph_svgf_input* in, int width, int height, const ph_camera_view* previous_camera, const ph_svgf_settings* settings,
+                       float* history_dev, int has_history, float* output_dev);
 const float* ph_svgf_variance_buffer(const ph_svgf* f);
 float ph_svgf_last_ms(const ph_svgf* f);
 

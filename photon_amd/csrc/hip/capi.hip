@@ -640,6 +640,22 @@ int ph_svgf_run(ph_svgf* f, const ph_svgf_input* in, const ph_camera_view* prev,
     });
 }
 
+int ph_svgf_run_static(const ph_svgf_input* in, int width, int height, const ph_camera_view* prev, const ph_svgf_settings* s, float* history,
+                       int has_history, float* out) {
+    if (!in || !prev || !s || !history || !out) return fail("null argument");
+    if (!in->color_dev || !in->albedo_dev || !in->normal_depth_dev || !in->position_dev || !in->emission_dev) return fail("missing G-buffer plane");
+    if (width <= 0 || height <= 0) return fail("frame size must be positive");
+    if (int64_t(width) * int64_t(height) > int64_t(PH_SVGF_STATIC_MAX_PIXELS))
+        return fail("frame of " + std::to_string(int64_t(width) * height) + " pixels exceeds the static SVGF scratch of " +
+                    std::to_string(PH_SVGF_STATIC_MAX_PIXELS) + " pixels");
+    if (s->atrous_iterations < 0 || s->atrous_iterations > 16) return fail("a-trous iterations out of range");
+    return guarded([&] {
+        svgf_run_static(nullptr, width, height, *in, *prev, *s, reinterpret_cast<float4*>(history), has_history != 0, reinterpret_cast<float4*>(out));
+        PH_CHECK(hipStreamSynchronize(nullptr));
+        return 0;
+    });
+}
+
 int ph_svgf_run_compact(ph_svgf* f, const ph_compact_gbuffer* in, const ph_camera_view* cam, const ph_camera_view* prev, const ph_svgf_settings* s,
                         float* out) {
     if (!f || !in || !cam || !prev || !s || !out) return fail("null argument");

@@ -14,6 +14,9 @@
 // fp32 depth, fp16 octahedral normal, int32 material; positions rebuilt from depth along each pixel's
 // centre ray). A compact frame is denoised exactly as its decoded float4 planes would be; only the
 // loads, and the format of the stored previous-frame geometry, differ.
+#include <mutex>
+#include <string>
+
 #include "internal.h"
 #include "dgbuf.h"
 #include "svgf_state.h"
@@ -357,16 +360,25 @@ __global__ void remodulate_kernel(In in, int n, const float4* filtered, float4* 
     out[i] = make_float4(c.x, c.y, c.z, 1.0f);
 }
 
+// The planes one frame works on: the previous frame's illumination and moments (history, read when
+// `has_history`, rewritten after a-trous pass 0) and four frame-local planes.
+struct PassPlanes {
+    int w, h;
+    bool has_history;
+    float4 *prev_illum, *prev_moments;
+    float4 *illum, *moments, *filter0, *filter1;
+};
+
 // The nine launches of one frame (svgf.cu:238-255) for either input format.
 template <class In, class Hist>
-void run_passes(SvgfState& f, hipStream_t st, const In& in, const Hist& prev, const ph_camera_view& prev_cam, const ph_svgf_settings& s,
+void run_passes(const PassPlanes& pl, hipStream_t st, const In& in, const Hist& prev, const ph_camera_view& prev_cam, const ph_svgf_settings& s,
                 float4* output) {
-    const int w = f.w, h = f.h, n = w * h;
-    float4 *prev_illum = f.plane[0].ptr, *prev_moments = f.plane[1].ptr;
-    float4 *illum = f.plane[4].ptr, *moments = f.plane[5].ptr, *filter[2] = {f.plane[6].ptr, f.plane[7].ptr};
+    const int w = pl.w, h = pl.h, n = w * h;
+    float4 *prev_illum = pl.prev_illum, *prev_moments = pl.prev_moments;
+    float4 *illum = pl.illum, *moments = pl.moments, *filter[2] = {pl.filter0, pl.filter1};
     const dim3 tile(TX, TY), grid((w + TX - 1) / TX, (h + TY - 1) / TY);
     const int lin = 256, lgrid = (n + lin - 1) / lin;
-    hipLaunchKernelGGL((temporal_kernel<In, Hist>), grid, tile, 0, st, in, prev_cam, w, h, int(f.has_history), s.alpha_color, s.alpha_moments,
+    hipLaunchKernelGGL((temporal_kernel<In, Hist>), grid, tile, 0, st, in, prev_cam, w, h, int(pl.has_history), s.alpha_color, s.alpha_moments,
                        prev_illum, prev_moments, prev, illum, moments);
     hipLaunchKernelGGL((variance_kernel<In>), grid, tile, 0, st, in, w, h, illum, moments, s.phi_normal, filter[0]);
     int src = 0;
@@ -391,7 +403,36 @@ void run_passes(SvgfState& f, hipStream_t st, const In& in, const Hist& prev, co
                            prev);
     hipLaunchKernelGGL((remodulate_kernel<In>), dim3(lgrid), dim3(lin), 0, st, in, n, filter[src], output);
     PH_CHECK(hipGetLastError());
-    f.has_history = true;
+}
+
+PassPlanes planes_of(const SvgfState& f) {
+    return PassPlanes{f.w, f.h, f.has_history, f.plane[0].ptr, f.plane[1].ptr, f.plane[4].ptr, f.plane[5].ptr, f.plane[6].ptr, f.plane[7].ptr};
+}
+
+// Frame-local planes of the allocation-free entry point: illumination, moments and the two a-trous
+// planes of up to kStaticSvgfPixels pixels, in one file-scope device array that every GPU holds from
+// the library's load until the process ends. Host code never names the array: it uses the address
+// that hipGetSymbolAddress returns on each device.
+__device__ __align__(256) float4 scratch[4 * size_t(PH_SVGF_STATIC_MAX_PIXELS)];
+
+constexpr int kMaxDevices = 64;
+float4* g_scratch[kMaxDevices] = {};
+std::once_flag g_scratch_once[kMaxDevices];
+hipError_t g_scratch_status[kMaxDevices] = {};
+
+// The array's address on the current device, looked up once per device with that device current.
+float4* static_scratch() {
+    int dev = 0;
+    PH_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("device index beyond the static scratch table");
+    std::call_once(g_scratch_once[dev], [dev] {
+        void* p = nullptr;
+        g_scratch_status[dev] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        g_scratch[dev] = static_cast<float4*>(p);
+    });
+    if (g_scratch_status[dev] != hipSuccess || !g_scratch[dev])
+        throw std::runtime_error(std::string("hipGetSymbolAddress of the SVGF scratch array failed: ") + hipGetErrorString(g_scratch_status[dev]));
+    return g_scratch[dev];
 }
 
 }  // namespace
@@ -415,7 +456,22 @@ void svgf_run(SvgfState& f, hipStream_t st, const ph_svgf_input& input, const ph
     const RefIn in{reinterpret_cast<const float4*>(input.color_dev), reinterpret_cast<const float4*>(input.albedo_dev),
                    reinterpret_cast<const float4*>(input.normal_depth_dev), reinterpret_cast<const float4*>(input.position_dev),
                    reinterpret_cast<const float4*>(input.emission_dev)};
-    run_passes(f, st, in, RefHist{f.plane[2].ptr, f.plane[3].ptr}, prev_cam, s, output);
+    run_passes(planes_of(f), st, in, RefHist{f.plane[2].ptr, f.plane[3].ptr}, prev_cam, s, output);
+    f.has_history = true;
+}
+
+void svgf_run_static(hipStream_t st, int width, int height, const ph_svgf_input& input, const ph_camera_view& prev_cam, const ph_svgf_settings& s,
+                     float4* history, bool has_history, float4* output) {
+    const size_t n = size_t(width) * size_t(height);
+    if (n == 0) return;
+    if (n > size_t(PH_SVGF_STATIC_MAX_PIXELS)) throw std::runtime_error("frame larger than the static SVGF scratch");
+    float4* base = static_scratch();
+    // History: the caller's four planes (illumination + length, moments, normal+depth, position+material).
+    const PassPlanes pl{width, height, has_history, history, history + n, base, base + n, base + 2 * n, base + 3 * n};
+    const RefIn in{reinterpret_cast<const float4*>(input.color_dev), reinterpret_cast<const float4*>(input.albedo_dev),
+                   reinterpret_cast<const float4*>(input.normal_depth_dev), reinterpret_cast<const float4*>(input.position_dev),
+                   reinterpret_cast<const float4*>(input.emission_dev)};
+    run_passes(pl, st, in, RefHist{history + 2 * n, history + 3 * n}, prev_cam, s, output);
 }
 
 void svgf_run_compact(SvgfState& f, hipStream_t st, const ph_compact_gbuffer& g, const ph_camera_view& cam, const ph_camera_view& prev_cam,
@@ -433,7 +489,8 @@ void svgf_run_compact(SvgfState& f, hipStream_t st, const ph_compact_gbuffer& g,
     const CompactIn in{static_cast<const float*>(g.color_dev), static_cast<const float*>(g.albedo_dev), static_cast<const float*>(g.emission_dev),
                        static_cast<const float*>(g.depth_dev), static_cast<const uint32_t*>(g.normal_dev), static_cast<const int*>(g.material_dev),
                        cam, f.w, f.h};
-    run_passes(f, st, in, CompactHist{f.prev_depth.ptr, f.prev_normal.ptr, f.prev_material.ptr}, prev_cam, s, output);
+    run_passes(planes_of(f), st, in, CompactHist{f.prev_depth.ptr, f.prev_normal.ptr, f.prev_material.ptr}, prev_cam, s, output);
+    f.has_history = true;
 }
 
 }  // namespace phk

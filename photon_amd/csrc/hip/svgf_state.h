@@ -22,6 +22,11 @@ struct SvgfState {
 
 void svgf_resize(SvgfState& f, int width, int height);
 void svgf_run(SvgfState& f, hipStream_t st, const ph_svgf_input& in, const ph_camera_view& prev_cam, const ph_svgf_settings& s, float4* output);
+// One frame with the frame-local planes in the library's static device array (no allocation): the
+// history is the caller's four planes of width*height float4 (illumination + length, moments,
+// normal+depth, position+material), read when `has_history` and rewritten by the frame.
+void svgf_run_static(hipStream_t st, int width, int height, const ph_svgf_input& in, const ph_camera_view& prev_cam, const ph_svgf_settings& s,
+                     float4* history, bool has_history, float4* output);
 // The same denoiser on the compact G-buffer traced from view `cam`.
 void svgf_run_compact(SvgfState& f, hipStream_t st, const ph_compact_gbuffer& in, const ph_camera_view& cam, const ph_camera_view& prev_cam,
                       const ph_svgf_settings& s, float4* output);
