@@ -88,6 +88,24 @@ PH_DEV f3 oct_decode(uint32_t e) {
     return mk3(x / len, y / len, z / len);
 }
 
+// The same unit vector normalised with the hardware reciprocal square root (about 1 ulp from
+// oct_decode's per component), for the denoiser's continuous edge-stopping weights, where a tap
+// decodes the normal of every neighbour it weighs.
+PH_DEV f3 oct_decode_approx(uint32_t e) {
+    // The hardware's half-to-single conversion: exact for every half (subnormals too, under the default
+    // mode that keeps half denormals; flushed, they would move the unit vector by less than 1e-4).
+    float x = float(__builtin_bit_cast(_Float16, uint16_t(e & 0xffffu))), y = float(__builtin_bit_cast(_Float16, uint16_t(e >> 16)));
+    const float z = 1.0f - fabsf(x) - fabsf(y);
+    if (z < 0.0f) {
+        const float ox = (1.0f - fabsf(y)) * (x >= 0.0f ? 1.0f : -1.0f);
+        const float oy = (1.0f - fabsf(x)) * (y >= 0.0f ? 1.0f : -1.0f);
+        x = ox;
+        y = oy;
+    }
+    const float r = __builtin_amdgcn_rsqf(x * x + y * y + z * z);
+    return mk3(x * r, y * r, z * r);
+}
+
 // Position at distance `depth` along the centre ray of pixel (x, y) of a pinhole view of `c` (the
 // camera's own origin; the lens sample and the pixel jitter of the traced ray are not stored).
 PH_DEV f3 centre_position(const ph_camera_view& c, int x, int y, int width, int height, float depth) {

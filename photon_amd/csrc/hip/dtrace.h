@@ -30,26 +30,32 @@ struct TraceCount {
 
 constexpr int kMaxStack = 64;
 
+// Per-lane overflow entries of a LaneStack<Depth> (private memory; at least one so the array exists).
+template <int Depth>
+struct SpillArea {
+    int v[Depth < kMaxStack ? kMaxStack - Depth : 1];
+};
+
+// The stack's handles only: the LDS column and the overflow array are separate objects, held through
+// pointers of their own address spaces, so the handle stays in registers and every access is a
+// ds_read/ds_write or a scratch access (never a flat-address one).
+using LdsInt = __attribute__((address_space(3))) int;
+using PrivateInt = __attribute__((address_space(5))) int;
+
 template <int Depth>
 struct LaneStack {
-    int* column;  // this lane's first LDS entry
-    int stride;   // lanes per block
-    int spill[kMaxStack - Depth];
+    LdsInt* column;    // this lane's first LDS entry
+    int stride;        // lanes per block
+    PrivateInt* spill;  // this lane's SpillArea<Depth>
+    PH_DEV LaneStack(int* lds_column, int lanes, int* spill_area)
+        : column((LdsInt*)lds_column), stride(lanes), spill((PrivateInt*)spill_area) {}
     PH_DEV void store(int sp, int v) {
-        if (sp < Depth)
+        if (Depth >= kMaxStack || sp < Depth)
             column[sp * stride] = v;
         else
             spill[sp - Depth] = v;
     }
-    PH_DEV int load(int sp) const { return sp < Depth ? column[sp * stride] : spill[sp - Depth]; }
-};
-
-template <>
-struct LaneStack<kMaxStack> {
-    int* column;
-    int stride;
-    PH_DEV void store(int sp, int v) { column[sp * stride] = v; }
-    PH_DEV int load(int sp) const { return column[sp * stride]; }
+    PH_DEV int load(int sp) const { return (Depth >= kMaxStack || sp < Depth) ? column[sp * stride] : spill[sp - Depth]; }
 };
 
 PH_DEV bool mt_hit(f3 v0, f3 e1, f3 e2, f3 o, f3 d, float tmax, float& t, float& u, float& v) {

@@ -24,7 +24,8 @@ __global__ __launch_bounds__(kPixelBlock) __attribute__((amdgpu_waves_per_eu(kSa
                                                              PathOpts opt, float4* accum, float4* albedo, float4* nd) {
     __shared__ int lds[kMegaDepth * kPixelBlock];
     const int tid = threadIdx.y * kBX + threadIdx.x;
-    LaneStack<kMegaDepth> stack{lds + tid, kPixelBlock};
+    SpillArea<kMegaDepth> spill;
+    LaneStack<kMegaDepth> stack{lds + tid, kPixelBlock, spill.v};
     const int x = blockIdx.x * kBX + threadIdx.x, y = blockIdx.y * kBY + threadIdx.y;
     if (x >= w || y >= h) return;
     const int i = y * w + x;
@@ -50,7 +51,8 @@ __global__ __launch_bounds__(kPixelBlock) __attribute__((amdgpu_waves_per_eu(kSa
                                                              float4* color, float4* albedo, float4* nd, float4* position, float4* emission) {
     __shared__ int lds[kMegaDepth * kPixelBlock];
     const int tid = threadIdx.y * kBX + threadIdx.x;
-    LaneStack<kMegaDepth> stack{lds + tid, kPixelBlock};
+    SpillArea<kMegaDepth> spill;
+    LaneStack<kMegaDepth> stack{lds + tid, kPixelBlock, spill.v};
     const int x = blockIdx.x * kBX + threadIdx.x, y = blockIdx.y * kBY + threadIdx.y;
     if (x >= w || y >= h) return;
     const int i = y * w + x;
@@ -69,7 +71,8 @@ __global__ __launch_bounds__(kPixelBlock) __attribute__((amdgpu_waves_per_eu(kSa
                                                                      uint32_t* normal, int* material) {
     __shared__ int lds[kMegaDepth * kPixelBlock];
     const int tid = threadIdx.y * kBX + threadIdx.x;
-    LaneStack<kMegaDepth> stack{lds + tid, kPixelBlock};
+    SpillArea<kMegaDepth> spill;
+    LaneStack<kMegaDepth> stack{lds + tid, kPixelBlock, spill.v};
     const int x = blockIdx.x * kBX + threadIdx.x, y = blockIdx.y * kBY + threadIdx.y;
     if (x >= w || y >= h) return;
     const int i = y * w + x;
@@ -99,7 +102,8 @@ __global__ void cdf_guide_kernel(const float* cdf, int rows, int n, size_t strid
 __global__ __launch_bounds__(kPixelBlock) void heatmap_kernel(DScene s, ph_camera_view cam, int w, int h, float* out) {
     __shared__ int lds[kMegaDepth * kPixelBlock];
     const int tid = threadIdx.y * kBX + threadIdx.x;
-    LaneStack<kMegaDepth> stack{lds + tid, kPixelBlock};
+    SpillArea<kMegaDepth> spill;
+    LaneStack<kMegaDepth> stack{lds + tid, kPixelBlock, spill.v};
     const int x = blockIdx.x * kBX + threadIdx.x, y = blockIdx.y * kBY + threadIdx.y;
     if (x >= w || y >= h) return;
     f3 o, d;
@@ -189,7 +193,8 @@ template <int Block, int Depth, bool AnyHit, bool Count, bool Persistent>
 __global__ __launch_bounds__(Block) void trace_kernel(DScene s, const ph_ray* rays, int n, ph_hit* hits, uint32_t* counts, int lds_nodes) {
     __shared__ int lds[Depth * Block];
     extern __shared__ float4 top_nodes[];
-    LaneStack<Depth> stack{lds + threadIdx.x, Block};
+    SpillArea<Depth> spill;
+    LaneStack<Depth> stack{lds + threadIdx.x, Block, spill.v};
     if (Persistent) {
         // A fixed grid; each wave refills its lanes from its share of the rays (trace_stream), with the
         // top of the tree staged in LDS.

@@ -29,20 +29,16 @@ namespace {
 
 constexpr int TX = 16, TY = 16;
 
-// Edge-stopping weights. The denoiser's tolerance is 1e-3 relative, so they use the hardware's
-// single-precision exp2/log2 (a few ulp) instead of correctly rounded exp/pow; a power-of-two normal
-// exponent (the default 128) is evaluated by exact repeated squaring.
-PH_DEV float weight_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504f); }
+// Edge-stopping weights. The denoiser's tolerance is 1e-3 relative, so a tap evaluates its weights as
+// one product in the exponent, exp2(log2 w_l + log2 w_z + log2 w_n), with the hardware's
+// single-precision exp2/log2 (a few ulp) instead of correctly rounded exp and pow.
+constexpr float kLog2e = 1.44269504f;
 
-PH_DEV float weight_pow(float x, float e) {
-    if (x <= 0.0f) return e > 0.0f ? 0.0f : 1.0f;
-    const int ie = int(e);
-    if (float(ie) == e && ie > 0 && ie <= 1024 && (ie & (ie - 1)) == 0) {
-        float r = x;
-        for (int k = ie; k > 1; k >>= 1) r *= r;
-        return r;
-    }
-    return __builtin_amdgcn_exp2f(e * __builtin_amdgcn_logf(x));
+// log2 of the normal weight max(0, d)^e: -inf where that weight is 0 (d <= 0 with e > 0), and 0 when
+// e is 0.
+PH_DEV float normal_log2w(float d, float e) {
+    if (e == 0.0f) return 0.0f;
+    return d > 0.0f ? e * __builtin_amdgcn_logf(d) : (e > 0.0f ? -__builtin_inff() : 0.0f);
 }
 
 PH_DEV f3 demod(f3 color, f3 emission, f3 albedo) {
@@ -70,6 +66,7 @@ struct RefIn {
     PH_DEV f3 albedo3(int i) const { return xyz(albedo[i]); }
     PH_DEV f3 emission3(int i) const { return xyz(emission[i]); }
     PH_DEV float4 ndz(int i) const { return nd[i]; }
+    PH_DEV float4 ndz_w(int i) const { return nd[i]; }
     PH_DEV float mat(int i) const { return pos[i].w; }
     PH_DEV f3 position(int i, int, int) const { return xyz(pos[i]); }
     PH_DEV float4 raw_color(int i) const { return color[i]; }
@@ -78,6 +75,7 @@ struct RefIn {
 struct RefHist {
     float4 *nd, *pos;
     PH_DEV float4 ndz(int j) const { return nd[j]; }
+    PH_DEV float4 ndz_w(int j) const { return nd[j]; }
     PH_DEV float mat(int j) const { return pos[j].w; }
     PH_DEV void store(const RefIn& in, int i) const {
         nd[i] = in.nd[i];
@@ -99,6 +97,12 @@ struct CompactIn {
         const f3 n = oct_decode(normal[i]);
         return make_float4(n.x, n.y, n.z, depth[i]);
     }
+    // For the edge-stopping weights of the variance and a-trous passes only: the normal normalised with
+    // the hardware reciprocal square root. Reprojection and the stored history decode exactly.
+    PH_DEV float4 ndz_w(int i) const {
+        const f3 n = oct_decode_approx(normal[i]);
+        return make_float4(n.x, n.y, n.z, depth[i]);
+    }
     PH_DEV float mat(int i) const { return float(material[i]); }
     PH_DEV f3 position(int i, int x, int y) const { return centre_position(cam, x, y, w, h, depth[i]); }
     PH_DEV float4 raw_color(int i) const {
@@ -113,6 +117,10 @@ struct CompactHist {
     int* material;
     PH_DEV float4 ndz(int j) const {
         const f3 n = oct_decode(normal[j]);
+        return make_float4(n.x, n.y, n.z, depth[j]);
+    }
+    PH_DEV float4 ndz_w(int j) const {
+        const f3 n = oct_decode_approx(normal[j]);
         return make_float4(n.x, n.y, n.z, depth[j]);
     }
     PH_DEV float mat(int j) const { return float(material[j]); }
@@ -147,9 +155,13 @@ __global__ __launch_bounds__(TX* TY) void temporal_kernel(In in, ph_camera_view 
                 if (xx < 0 || yy < 0 || xx >= w || yy >= h) continue;
                 const int j = yy * w + xx;
                 if (prev.mat(j) != mat) continue;
-                const float4 pn = prev.ndz(j);
+                const float4 pn = prev.ndz_w(j);
                 if (fabsf(pn.w - dist) > 0.05f * dist + 0.01f) continue;
-                if (dot(xyz(pn), n) < 0.9f) continue;
+                // The normal test on the exactly decoded normal wherever the fast decode's few-ulp error
+                // could decide it.
+                float nn = dot(xyz(pn), n);
+                if (fabsf(nn - 0.9f) < 1e-5f) nn = dot(xyz(prev.ndz(j)), n);
+                if (nn < 0.9f) continue;
                 const float bw = ((k & 1) ? fx : 1 - fx) * ((k >> 1) ? fy : 1 - fy);
                 const float4 pi = prev_illum[j];
                 const float4 pm = prev_moments[j];
@@ -184,129 +196,145 @@ __global__ __launch_bounds__(TX* TY) void temporal_kernel(In in, ph_camera_view 
     out_moments[i] = make_float4(m0, m1, variance, 0.0f);
 }
 
-// 7x7 bilateral variance estimate while the history is short (svgf.cu:96-132). The block stages the
-// 22x22 halo of normal+depth and position.w (material) in LDS; moments and illumination are read only
-// for accepted taps of short-history pixels.
+// The 7x7 estimate of one short-history pixel (svgf.cu:96-132): moments and illumination of the taps
+// whose material matches, weighted by normal and depth. `nd_at(dx, dy)` and `mat_at(dx, dy)` read the
+// neighbour at that offset (in range).
+template <class NdAt, class MatAt>
+PH_DEV float4 spatial_estimate(int x, int y, int w, int h, float mat, float4 ndc, NdAt nd_at, MatAt mat_at, const float4* illum, const float4* moments,
+                               float phi_normal, float4 c, float var) {
+    constexpr int R = 3;
+    const f3 n = xyz(ndc);
+    const float z = ndc.w, len = c.w;
+    float wsum = 0, m1 = 0, m2 = 0;
+    f3 acc = mk3(0);
+    // The depth weight's exponent per unit of depth difference (svgf.cu's exp(-|dz| / (0.05 z + 1e-3))).
+    const float kz = -kLog2e / (0.05f * z + 1e-3f);
+#pragma unroll
+    for (int dy = -R; dy <= R; dy++)
+#pragma unroll
+        for (int dx = -R; dx <= R; dx++) {
+            const int xx = x + dx, yy = y + dy;
+            if (xx < 0 || yy < 0 || xx >= w || yy >= h) continue;
+            if (mat_at(dx, dy) != mat) continue;
+            const float4 nd = nd_at(dx, dy);
+            const float wt = __builtin_amdgcn_exp2f(normal_log2w(fmaxf(0.0f, dot(n, xyz(nd))), phi_normal) + fabsf(nd.w - z) * kz);
+            const int j = yy * w + xx;
+            const float4 m = moments[j];
+            const float4 il = illum[j];
+            m1 = __builtin_fmaf(m.x, wt, m1);
+            m2 = __builtin_fmaf(m.y, wt, m2);
+            acc = mk3(__builtin_fmaf(il.x, wt, acc.x), __builtin_fmaf(il.y, wt, acc.y), __builtin_fmaf(il.z, wt, acc.z));
+            wsum += wt;
+        }
+    if (wsum > 0) {
+        m1 /= wsum;
+        m2 /= wsum;
+        var = fmaxf(0.0f, m2 - m1 * m1) * (4.0f / len);
+        const f3 a3 = acc / wsum;
+        c = make_float4(a3.x, a3.y, a3.z, len);
+    }
+    return make_float4(c.x, c.y, c.z, var);
+}
+
+// Variance pass. Pixels whose history has reached 4 frames keep their temporal estimate; the others take
+// the 7x7 spatial estimate. A block with no such pixel (the common case once the camera has settled)
+// only copies; any other stages the 22x22 halo of normal+depth and material in LDS once (gathering the
+// taps of a block's few short-history pixels through the caches instead measured slower).
+
 template <class In>
 __global__ __launch_bounds__(TX* TY) void variance_kernel(In in, int w, int h, const float4* illum, const float4* moments, float phi_normal,
                                                            float4* out) {
     constexpr int R = 3, SX = TX + 2 * R, SY = TY + 2 * R;
     __shared__ float4 s_nd[SY][SX];
     __shared__ float s_mat[SY][SX];
+    const int x = blockIdx.x * TX + threadIdx.x, y = blockIdx.y * TY + threadIdx.y;
+    const bool inside = x < w && y < h;
+    const int i = inside ? y * w + x : 0;
+    const float4 c = inside ? illum[i] : make_float4(0, 0, 0, 4.0f);
+    const float mat = inside ? in.mat(i) : -1.0f;
+    const bool spatial = inside && c.w < 4.0f && mat >= 0;
+    if (!__syncthreads_or(spatial)) {
+        if (inside) out[i] = make_float4(c.x, c.y, c.z, moments[i].z);
+        return;
+    }
     const int bx = blockIdx.x * TX - R, by = blockIdx.y * TY - R;
     for (int t = threadIdx.y * TX + threadIdx.x; t < SX * SY; t += TX * TY) {
         const int sy = t / SX, sx = t % SX;
         const int gx = bx + sx, gy = by + sy;
         if (gx >= 0 && gy >= 0 && gx < w && gy < h) {
             const int j = gy * w + gx;
-            s_nd[sy][sx] = in.ndz(j);
+            s_nd[sy][sx] = in.ndz_w(j);
             s_mat[sy][sx] = in.mat(j);
         }
     }
     __syncthreads();
-    const int x = blockIdx.x * TX + threadIdx.x, y = blockIdx.y * TY + threadIdx.y;
-    if (x >= w || y >= h) return;
-    const int i = y * w + x;
-    float4 c = illum[i];
-    const float len = c.w;
-    float var = moments[i].z;
-    const float mat = s_mat[threadIdx.y + R][threadIdx.x + R];
-    if (len < 4.0f && mat >= 0) {
-        const float4 ndc = s_nd[threadIdx.y + R][threadIdx.x + R];
-        const f3 n = xyz(ndc);
-        const float z = ndc.w;
-        float wsum = 0, m1 = 0, m2 = 0;
-        f3 acc = mk3(0);
-        for (int dy = -R; dy <= R; dy++)
-            for (int dx = -R; dx <= R; dx++) {
-                const int xx = x + dx, yy = y + dy;
-                if (xx < 0 || yy < 0 || xx >= w || yy >= h) continue;
-                const int ly = threadIdx.y + R + dy, lx = threadIdx.x + R + dx;
-                if (s_mat[ly][lx] != mat) continue;
-                const float4 nd = s_nd[ly][lx];
-                const float wn = weight_pow(fmaxf(0.0f, dot(n, xyz(nd))), phi_normal);
-                const float wz = weight_exp(-fabsf(nd.w - z) / (0.05f * z + 1e-3f));
-                const float wt = wn * wz;
-                const int j = yy * w + xx;
-                const float4 m = moments[j];
-                m1 += m.x * wt;
-                m2 += m.y * wt;
-                acc = acc + xyz(illum[j]) * wt;
-                wsum += wt;
-            }
-        if (wsum > 0) {
-            m1 /= wsum;
-            m2 /= wsum;
-            var = fmaxf(0.0f, m2 - m1 * m1) * (4.0f / len);
-            f3 a = acc / wsum;
-            c = make_float4(a.x, a.y, a.z, len);
-        }
-    }
-    out[i] = make_float4(c.x, c.y, c.z, var);
+    if (!inside) return;
+    const int ly = threadIdx.y + R, lx = threadIdx.x + R;
+    float4 r = make_float4(c.x, c.y, c.z, moments[i].z);
+    if (spatial)
+        r = spatial_estimate(
+            x, y, w, h, mat, s_nd[ly][lx], [&](int dx, int dy) { return s_nd[ly + dy][lx + dx]; }, [&](int dx, int dy) { return s_mat[ly + dy][lx + dx]; },
+            illum, moments, phi_normal, c, r.w);
+    out[i] = r;
 }
 
-// One a-trous pass (svgf.cu:134-186). For steps 1 and 2 the block stages src, normal+depth and the
-// material id of its (16 + 4*step + 2)^2 footprint in LDS; larger steps read through the caches.
-template <int Step, class In>
-__global__ __launch_bounds__(TX* TY) void atrous_kernel(In in, int w, int h, int step_rt, float phi_color, float phi_normal, float phi_depth,
-                                                         const float4* src, float4* dst) {
-    constexpr bool Tiled = Step > 0;
-    constexpr int R = Tiled ? 2 * Step + 1 : 1;  // taps reach 2*step; the 3x3 prefilter and gradient need 1
-    constexpr int SX = Tiled ? TX + 2 * R : 1, SY = Tiled ? TY + 2 * R : 1;
-    __shared__ float4 s_src[SY][SX];
-    __shared__ float4 s_nd[SY][SX];
-    __shared__ float s_mat[SY][SX];
-    const int step = Tiled ? Step : step_rt;
-    if (Tiled) {
-        const int bx = blockIdx.x * TX - R, by = blockIdx.y * TY - R;
-        for (int t = threadIdx.y * TX + threadIdx.x; t < SX * SY; t += TX * TY) {
-            const int sy = t / SX, sx = t % SX;
-            // Clamp to the image: the prefilter and the gradient clamp their coordinates, and the
-            // a-trous taps skip out-of-image positions before reading.
-            const int gx = min(max(blockIdx.x * TX - R + sx, 0), w - 1), gy = min(max(blockIdx.y * TY - R + sy, 0), h - 1);
-            const int j = gy * w + gx;
-            s_src[sy][sx] = src[j];
-            s_nd[sy][sx] = in.ndz(j);
-            s_mat[sy][sx] = in.mat(j);
-            (void)bx;
-            (void)by;
-        }
-        __syncthreads();
-    }
-    const int x = blockIdx.x * TX + threadIdx.x, y = blockIdx.y * TY + threadIdx.y;
-    if (x >= w || y >= h) return;
-    const int i = y * w + x;
-    const int lx = threadIdx.x + R, ly = threadIdx.y + R;
-    // Loads through LDS (tiled) or global memory; (gx, gy) are image coordinates already in range.
-    auto SRC = [&](int gx, int gy) { return Tiled ? s_src[ly + gy - y][lx + gx - x] : src[gy * w + gx]; };
-    auto ND = [&](int gx, int gy) { return Tiled ? s_nd[ly + gy - y][lx + gx - x] : in.ndz(gy * w + gx); };
-    auto MAT = [&](int gx, int gy) { return Tiled ? s_mat[ly + gy - y][lx + gx - x] : in.mat(gy * w + gx); };
-    const float4 center = SRC(x, y);
-    const float mat = MAT(x, y);
-    if (mat < 0) {
-        dst[i] = center;
-        return;
-    }
+// What a pass does with its result besides writing its output plane: the history of the next frame is
+// taken after pass 0 (svgf.cu:247-248: the filtered colour with the history length, the moments, and
+// the geometry), and the last pass remodulates by the albedo and adds the emission (svgf.cu:196-209:
+// misses pass the raw colour through). Folding these into the passes saves a read of the plane and a
+// launch each.
+template <class In, class Hist>
+struct PassTail {
+    bool keep;     // write the pass's output plane
+    bool history;  // store the next frame's history
+    const float4* illum;
+    const float4* moments;
+    float4* prev_illum;
+    float4* prev_moments;
+    Hist prev;
+    float4* remod;  // non-null: write the remodulated output here
+};
+
+template <class In>
+PH_DEV float4 remodulated(const In& in, int i, float mat, float4 filtered) {
+    if (mat < 0) return in.raw_color(i);
+    const f3 c = xyz(filtered) * vmax3(in.albedo3(i), mk3(1e-3f)) + in.emission3(i);
+    return make_float4(c.x, c.y, c.z, 1.0f);
+}
+
+// The a-trous filter of one pixel (svgf.cu:134-186) whose material is `mat` (>= 0). The accessors read
+// what the pixel's block has staged or what lies in memory, by offset from the pixel:
+//   tap_src/tap_nd/tap_mat(dx, dy): the tap at (x + dx * step, y + dy * step), in the image;
+//   pre(dx, dy): src.w at (x + dx, y + dy) clamped to the image (the 3x3 variance prefilter);
+//   depth(dx, dy): the depth at (x + dx, y + dy) clamped to the image (the depth gradient).
+template <class TapSrc, class TapNd, class TapMat, class Pre, class Depth>
+PH_DEV float4 atrous_pixel(int x, int y, int w, int h, int step, float phi_color, float phi_normal, float phi_depth, float4 center, float mat,
+                           float4 ndc, TapSrc tap_src, TapNd tap_nd, TapMat tap_mat, Pre pre, Depth depth) {
     float var = 0, vw = 0;
 #pragma unroll
     for (int dy = -1; dy <= 1; dy++)
 #pragma unroll
         for (int dx = -1; dx <= 1; dx++) {
-            const int xx = min(max(x + dx, 0), w - 1), yy = min(max(y + dy, 0), h - 1);
             const float k = (dx == 0 && dy == 0) ? 0.25f : ((dx == 0 || dy == 0) ? 0.125f : 0.0625f);
-            var += SRC(xx, yy).w * k;
+            var += pre(dx, dy) * k;
             vw += k;
         }
     var /= vw;
-    const float4 ndc = ND(x, y);
     const f3 n = xyz(ndc);
     const float z = ndc.w;
-    const float zx = ND(min(x + 1, w - 1), y).w - ND(max(x - 1, 0), y).w;
-    const float zy = ND(x, min(y + 1, h - 1)).w - ND(x, max(y - 1, 0)).w;
+    const float zx = depth(1, 0) - depth(-1, 0);
+    const float zy = depth(0, 1) - depth(0, -1);
     const float grad = 0.5f * fmaxf(fabsf(zx), fabsf(zy));
     const float lum_c = lum(xyz(center));
     const float sigma_l = phi_color * __builtin_sqrtf(fmaxf(var, 0.0f)) + 1e-4f;
     const float kernel[3] = {3.0f / 8.0f, 1.0f / 4.0f, 1.0f / 16.0f};
+    // The luminance and depth weights (svgf.cu: exp(-|dl| / sigma_l) and exp(-|dz| / (phi_depth * grad
+    // * dist + 1e-3 z + 1e-4))) as exponents per unit difference: one division per pixel for the
+    // luminance and one per tap distance (dx^2 + dy^2 in {0, 1, 2, 4, 5, 8}) instead of two per tap.
+    const float kl = -kLog2e / sigma_l;
+    const float gz = phi_depth * grad, zb = 1e-3f * z;
+    auto kz_at = [&](int d2) { return -kLog2e / (gz * (__builtin_sqrtf(float(d2)) * float(step)) + zb + 1e-4f); };
+    const float kz[9] = {kz_at(0), kz_at(1), kz_at(2), 0.0f, kz_at(4), kz_at(5), 0.0f, 0.0f, kz_at(8)};
     f3 sum = mk3(0);
     float wsum = 0, vsum = 0;
 #pragma unroll
@@ -315,25 +343,133 @@ __global__ __launch_bounds__(TX* TY) void atrous_kernel(In in, int w, int h, int
         for (int dx = -2; dx <= 2; dx++) {
             const int xx = x + dx * step, yy = y + dy * step;
             if (xx < 0 || yy < 0 || xx >= w || yy >= h) continue;
-            if (MAT(xx, yy) != mat) continue;
-            const float4 sv = SRC(xx, yy);
-            const float4 nd = ND(xx, yy);
+            if (tap_mat(dx, dy) != mat) continue;
+            const float4 sv = tap_src(dx, dy);
+            const float4 nd = tap_nd(dx, dy);
             const float k = kernel[dx < 0 ? -dx : dx] * kernel[dy < 0 ? -dy : dy];
-            const float wl = weight_exp(-fabsf(lum(xyz(sv)) - lum_c) / sigma_l);
-            const float wn = weight_pow(fmaxf(0.0f, dot(n, xyz(nd))), phi_normal);
-            const float dist = __builtin_sqrtf(float(dx * dx + dy * dy)) * float(step);
-            const float wz = weight_exp(-fabsf(nd.w - z) / (phi_depth * grad * dist + 1e-3f * z + 1e-4f));
-            const float wt = k * wl * wn * wz;
-            sum = sum + xyz(sv) * wt;
-            vsum += wt * wt * sv.w;
+            const float e = fabsf(lum(xyz(sv)) - lum_c) * kl + fabsf(nd.w - z) * kz[dx * dx + dy * dy] +
+                            normal_log2w(fmaxf(0.0f, dot(n, xyz(nd))), phi_normal);
+            const float wt = k * __builtin_amdgcn_exp2f(e);
+            sum = mk3(__builtin_fmaf(sv.x, wt, sum.x), __builtin_fmaf(sv.y, wt, sum.y), __builtin_fmaf(sv.z, wt, sum.z));
+            vsum = __builtin_fmaf(wt * wt, sv.w, vsum);
             wsum += wt;
         }
     if (wsum > 0) {
         const f3 a = sum / wsum;
-        dst[i] = make_float4(a.x, a.y, a.z, vsum / (wsum * wsum));
-    } else {
-        dst[i] = center;
+        return make_float4(a.x, a.y, a.z, vsum / (wsum * wsum));
     }
+    return center;
+}
+
+template <class In, class Hist>
+PH_DEV void finish_pass(const In& in, int i, float mat, float4 res, float4* dst, const PassTail<In, Hist>& tail) {
+    if (tail.keep) dst[i] = res;
+    if (tail.history) {
+        tail.prev_illum[i] = make_float4(res.x, res.y, res.z, tail.illum[i].w);
+        tail.prev_moments[i] = tail.moments[i];
+        tail.prev.store(in, i);
+    }
+    if (tail.remod) tail.remod[i] = remodulated(in, i, mat, res);
+}
+
+// One a-trous pass on 16x16 tiles. For steps 1 and 2 the block stages src, normal+depth and the material
+// id of its (16 + 4*step + 2)^2 footprint in LDS; Step 0 (any step) reads through the caches.
+template <int Step, class In, class Hist>
+__global__ __launch_bounds__(TX* TY) void atrous_kernel(In in, int w, int h, int step_rt, float phi_color, float phi_normal, float phi_depth,
+                                                         const float4* src, float4* dst, PassTail<In, Hist> tail) {
+    constexpr bool Tiled = Step > 0;
+    constexpr int R = Tiled ? 2 * Step + 1 : 1;  // taps reach 2*step; the 3x3 prefilter and gradient need 1
+    constexpr int SX = Tiled ? TX + 2 * R : 1, SY = Tiled ? TY + 2 * R : 1;
+    __shared__ float4 s_src[SY][SX];
+    __shared__ float4 s_nd[SY][SX];
+    __shared__ float s_mat[SY][SX];
+    const int step = Tiled ? Step : step_rt;
+    if (Tiled) {
+        for (int t = threadIdx.y * TX + threadIdx.x; t < SX * SY; t += TX * TY) {
+            const int sy = t / SX, sx = t % SX;
+            // Clamp to the image: the prefilter and the gradient clamp their coordinates, and the
+            // a-trous taps skip out-of-image positions before reading.
+            const int gx = min(max(int(blockIdx.x) * TX - R + sx, 0), w - 1), gy = min(max(int(blockIdx.y) * TY - R + sy, 0), h - 1);
+            const int j = gy * w + gx;
+            s_src[sy][sx] = src[j];
+            s_nd[sy][sx] = in.ndz_w(j);
+            s_mat[sy][sx] = in.mat(j);
+        }
+        __syncthreads();
+    }
+    const int x = blockIdx.x * TX + threadIdx.x, y = blockIdx.y * TY + threadIdx.y;
+    if (x >= w || y >= h) return;
+    const int i = y * w + x;
+    const int lx = threadIdx.x + R, ly = threadIdx.y + R;
+    const float4 center = Tiled ? s_src[ly][lx] : src[i];
+    const float mat = Tiled ? s_mat[ly][lx] : in.mat(i);
+    float4 res = center;
+    if (mat >= 0) {
+        auto cx = [&](int dx) { return min(max(x + dx, 0), w - 1); };
+        auto cy = [&](int dy) { return min(max(y + dy, 0), h - 1); };
+        if (Tiled)
+            res = atrous_pixel(
+                x, y, w, h, step, phi_color, phi_normal, phi_depth, center, mat, s_nd[ly][lx],
+                [&](int dx, int dy) { return s_src[ly + dy * Step][lx + dx * Step]; }, [&](int dx, int dy) { return s_nd[ly + dy * Step][lx + dx * Step]; },
+                [&](int dx, int dy) { return s_mat[ly + dy * Step][lx + dx * Step]; }, [&](int dx, int dy) { return s_src[ly + dy][lx + dx].w; },
+                [&](int dx, int dy) { return s_nd[ly + dy][lx + dx].w; });
+        else
+            res = atrous_pixel(
+                x, y, w, h, step, phi_color, phi_normal, phi_depth, center, mat, in.ndz_w(i),
+                [&](int dx, int dy) { return src[i + dy * step * w + dx * step]; }, [&](int dx, int dy) { return in.ndz_w(i + dy * step * w + dx * step); },
+                [&](int dx, int dy) { return in.mat(i + dy * step * w + dx * step); }, [&](int dx, int dy) { return src[cy(dy) * w + cx(dx)].w; },
+                [&](int dx, int dy) { return in.ndz_w(cy(dy) * w + cx(dx)).w; });
+    }
+    finish_pass(in, i, mat, res, dst, tail);
+}
+
+// One a-trous pass for the larger steps (4, 8, 16) on lattice rows: a block is 64 neighbouring pixels
+// of each of 4 rows spaced `Step` apart (rows y0, y0 + Step, ...), so that all its taps fall on 8 rows
+// of the same lattice. The block stages those 8 rows, 64 + 4 * Step columns wide, in LDS: coalesced
+// loads of about 2.5-4 texels per pixel instead of 25 gathers per pixel through the caches. The
+// prefilter's and gradient's neighbours in the rows above and below (not on the lattice) are read from
+// memory.
+constexpr int kLatticeX = 64, kLatticeRows = 4;
+
+template <int Step, class In, class Hist>
+__global__ __launch_bounds__(kLatticeX* kLatticeRows) void atrous_lattice_kernel(In in, int w, int h, float phi_color, float phi_normal, float phi_depth,
+                                                                                 const float4* src, float4* dst, PassTail<In, Hist> tail) {
+    constexpr int SW = kLatticeX + 4 * Step, SR = kLatticeRows + 4;
+    __shared__ float4 s_src[SR][SW];
+    __shared__ float4 s_nd[SR][SW];
+    __shared__ float s_mat[SR][SW];
+    // blockIdx.y = group * Step + residue: lattice rows residue + (4 * group + r) * Step, r = 0..3.
+    const int residue = blockIdx.y % Step, k0 = (blockIdx.y / Step) * kLatticeRows;
+    const int x0 = blockIdx.x * kLatticeX;
+    for (int t = threadIdx.y * kLatticeX + threadIdx.x; t < SR * SW; t += kLatticeX * kLatticeRows) {
+        const int r = t / SW, c = t % SW;
+        // Clamped like the tiles: taps never read out-of-image entries, the prefilter's row-y
+        // neighbours (columns x +- 1) read the clamped ones.
+        const int gx = min(max(x0 - 2 * Step + c, 0), w - 1), gy = min(max(residue + (k0 - 2 + r) * Step, 0), h - 1);
+        const int j = gy * w + gx;
+        s_src[r][c] = src[j];
+        s_nd[r][c] = in.ndz_w(j);
+        s_mat[r][c] = in.mat(j);
+    }
+    __syncthreads();
+    const int x = x0 + threadIdx.x, y = residue + (k0 + int(threadIdx.y)) * Step;
+    if (x >= w || y >= h) return;
+    const int i = y * w + x;
+    const int lr = threadIdx.y + 2, lc = threadIdx.x + 2 * Step;
+    const float4 center = s_src[lr][lc];
+    const float mat = s_mat[lr][lc];
+    float4 res = center;
+    if (mat >= 0) {
+        auto cx = [&](int dx) { return min(max(x + dx, 0), w - 1); };
+        auto cy = [&](int dy) { return min(max(y + dy, 0), h - 1); };
+        res = atrous_pixel(
+            x, y, w, h, Step, phi_color, phi_normal, phi_depth, center, mat, s_nd[lr][lc],
+            [&](int dx, int dy) { return s_src[lr + dy][lc + dx * Step]; }, [&](int dx, int dy) { return s_nd[lr + dy][lc + dx * Step]; },
+            [&](int dx, int dy) { return s_mat[lr + dy][lc + dx * Step]; },
+            [&](int dx, int dy) { return dy == 0 ? s_src[lr][lc + dx].w : src[cy(dy) * w + cx(dx)].w; },
+            [&](int dx, int dy) { return dy == 0 ? s_nd[lr][lc + dx].w : in.ndz_w(cy(dy) * w + cx(dx)).w; });
+    }
+    finish_pass(in, i, mat, res, dst, tail);
 }
 
 template <class In, class Hist>
@@ -351,13 +487,7 @@ template <class In>
 __global__ void remodulate_kernel(In in, int n, const float4* filtered, float4* out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    if (in.mat(i) < 0) {
-        out[i] = in.raw_color(i);
-        return;
-    }
-    const f3 a = in.albedo3(i);
-    const f3 c = xyz(filtered[i]) * vmax3(a, mk3(1e-3f)) + in.emission3(i);
-    out[i] = make_float4(c.x, c.y, c.z, 1.0f);
+    out[i] = remodulated(in, i, in.mat(i), filtered[i]);
 }
 
 // The planes one frame works on: the previous frame's illumination and moments (history, read when
@@ -368,6 +498,12 @@ struct PassPlanes {
     float4 *prev_illum, *prev_moments;
     float4 *illum, *moments, *filter0, *filter1;
 };
+
+// Grid of atrous_lattice_kernel: 64-pixel column groups by (groups of 4 lattice rows) x residues.
+dim3 lattice_grid(int w, int h, int step) {
+    const int rows = (h + step - 1) / step;  // lattice rows of residue 0, the most of any residue
+    return dim3((w + kLatticeX - 1) / kLatticeX, ((rows + kLatticeRows - 1) / kLatticeRows) * step);
+}
 
 // The nine launches of one frame (svgf.cu:238-255) for either input format.
 template <class In, class Hist>
@@ -382,26 +518,37 @@ void run_passes(const PassPlanes& pl, hipStream_t st, const In& in, const Hist& 
                        prev_illum, prev_moments, prev, illum, moments);
     hipLaunchKernelGGL((variance_kernel<In>), grid, tile, 0, st, in, w, h, illum, moments, s.phi_normal, filter[0]);
     int src = 0;
-    for (int it = 0; it < s.atrous_iterations; it++) {
+    const int iters = s.atrous_iterations;
+    for (int it = 0; it < iters; it++) {
         const int step = 1 << it;
+        const bool last = it == iters - 1;
+        // The last pass's plane is needed only when it is plane 6, the variance buffer a display reads.
+        PassTail<In, Hist> tail{!last || 1 - src == 0, it == 0, illum, moments, prev_illum, prev_moments, prev, last ? output : nullptr};
         if (step == 1)
-            hipLaunchKernelGGL((atrous_kernel<1, In>), grid, tile, 0, st, in, w, h, step, s.phi_color, s.phi_normal, s.phi_depth, filter[src],
-                               filter[1 - src]);
+            hipLaunchKernelGGL((atrous_kernel<1, In, Hist>), grid, tile, 0, st, in, w, h, step, s.phi_color, s.phi_normal, s.phi_depth, filter[src],
+                               filter[1 - src], tail);
         else if (step == 2)
-            hipLaunchKernelGGL((atrous_kernel<2, In>), grid, tile, 0, st, in, w, h, step, s.phi_color, s.phi_normal, s.phi_depth, filter[src],
-                               filter[1 - src]);
+            hipLaunchKernelGGL((atrous_kernel<2, In, Hist>), grid, tile, 0, st, in, w, h, step, s.phi_color, s.phi_normal, s.phi_depth, filter[src],
+                               filter[1 - src], tail);
+        else if (step == 4)
+            hipLaunchKernelGGL((atrous_lattice_kernel<4, In, Hist>), lattice_grid(w, h, 4), dim3(kLatticeX, kLatticeRows), 0, st, in, w, h, s.phi_color,
+                               s.phi_normal, s.phi_depth, filter[src], filter[1 - src], tail);
+        else if (step == 8)
+            hipLaunchKernelGGL((atrous_lattice_kernel<8, In, Hist>), lattice_grid(w, h, 8), dim3(kLatticeX, kLatticeRows), 0, st, in, w, h, s.phi_color,
+                               s.phi_normal, s.phi_depth, filter[src], filter[1 - src], tail);
+        else if (step == 16)
+            hipLaunchKernelGGL((atrous_lattice_kernel<16, In, Hist>), lattice_grid(w, h, 16), dim3(kLatticeX, kLatticeRows), 0, st, in, w, h, s.phi_color,
+                               s.phi_normal, s.phi_depth, filter[src], filter[1 - src], tail);
         else
-            hipLaunchKernelGGL((atrous_kernel<0, In>), grid, tile, 0, st, in, w, h, step, s.phi_color, s.phi_normal, s.phi_depth, filter[src],
-                               filter[1 - src]);
+            hipLaunchKernelGGL((atrous_kernel<0, In, Hist>), grid, tile, 0, st, in, w, h, step, s.phi_color, s.phi_normal, s.phi_depth, filter[src],
+                               filter[1 - src], tail);
         src = 1 - src;
-        if (it == 0)
-            hipLaunchKernelGGL((history_kernel<In, Hist>), dim3(lgrid), dim3(lin), 0, st, in, n, filter[src], illum, moments, prev_illum, prev_moments,
-                               prev);
     }
-    if (s.atrous_iterations == 0)
+    if (iters == 0) {
         hipLaunchKernelGGL((history_kernel<In, Hist>), dim3(lgrid), dim3(lin), 0, st, in, n, filter[src], illum, moments, prev_illum, prev_moments,
                            prev);
-    hipLaunchKernelGGL((remodulate_kernel<In>), dim3(lgrid), dim3(lin), 0, st, in, n, filter[src], output);
+        hipLaunchKernelGGL((remodulate_kernel<In>), dim3(lgrid), dim3(lin), 0, st, in, n, filter[src], output);
+    }
     PH_CHECK(hipGetLastError());
 }
 

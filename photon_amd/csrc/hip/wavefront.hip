@@ -450,7 +450,8 @@ template <int Block, int Depth, bool Count, bool Multi>
 __global__ __launch_bounds__(Block) void extend_kernel(Work work, State st, int* counters, int sort, int refill, int lds_nodes, TraceTotals* totals) {
     __shared__ int lds[Depth * Block];
     extern __shared__ float4 top_nodes[];
-    LaneStack<Depth> stack{lds + threadIdx.x, Block};
+    SpillArea<Depth> spill;
+    LaneStack<Depth> stack{lds + threadIdx.x, Block, spill.v};
     TraceCount cnt{0, 0, 0};
     trace_queue<false, Count, Multi>(
         work, st, st.q_ext, counters[C_EXT], counters[C_PARTS], refill, lds_nodes, top_nodes, stack, cnt,
@@ -475,7 +476,8 @@ template <int Block, int Depth, bool Count, bool Multi>
 __global__ __launch_bounds__(Block) void shadow_kernel(Work work, State st, int* counters, int refill, int lds_nodes, TraceTotals* totals) {
     __shared__ int lds[Depth * Block];
     extern __shared__ float4 top_nodes[];
-    LaneStack<Depth> stack{lds + threadIdx.x, Block};
+    SpillArea<Depth> spill;
+    LaneStack<Depth> stack{lds + threadIdx.x, Block, spill.v};
     TraceCount cnt{0, 0, 0};
     trace_queue<true, Count, Multi>(
         work, st, st.q_shadow, counters[C_SHADOW], counters[C_PARTS], refill, lds_nodes, top_nodes, stack, cnt,
