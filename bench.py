@@ -449,7 +449,8 @@ def bench_serving(args, gpu, host, field, launch):
               host.Scene.spheres(256, box=100.0, alpha_fraction=0.1, seed=1).build(), field]
     log(f"serving scenes: {[s.num_tris for s in scenes]} triangles, setup {time.time() - t:.1f} s")
     srv = gpu.Server(path_slots=args.serve_slots, iterations_per_step=args.serve_iterations,
-                     schedule={"fifo": gpu.SCHEDULE_FIFO, "srpt": gpu.SCHEDULE_SRPT, "hrrn": gpu.SCHEDULE_HRRN, "share": gpu.SCHEDULE_SHARE}[args.serve_schedule],
+                     schedule={"fifo": gpu.SCHEDULE_FIFO, "srpt": gpu.SCHEDULE_SRPT, "hrrn": gpu.SCHEDULE_HRRN, "share": gpu.SCHEDULE_SHARE,
+                               "deadline": gpu.SCHEDULE_DEADLINE}[args.serve_schedule], deadline_stretch=args.serve_stretch,
                      pass_samples=args.serve_pass_samples, min_items=args.serve_min_items,
                      express_slots=args.serve_express_slots, express_samples=args.serve_express_samples,
                      express_iterations=args.serve_express_iterations)
@@ -556,6 +557,7 @@ def bench_serving(args, gpu, host, field, launch):
                slowdown_vs_peak_rate={"p50": round(float(np.percentile(slow_rate, 50)), 2), "p99": round(float(np.percentile(slow_rate, 99)), 2)},
                isolated_ms={"p50": round(float(np.percentile(iso, 50)), 2), "p99": round(float(np.percentile(iso, 99)), 2)},
                by_job_size=by_size, path_slots=args.serve_slots, iterations_per_step=args.serve_iterations, schedule=args.serve_schedule,
+               deadline_stretch=args.serve_stretch,
                pass_samples=args.serve_pass_samples, min_items=args.serve_min_items, express_slots=args.serve_express_slots,
                express_samples=args.serve_express_samples, express_iterations=args.serve_express_iterations,
                isolated=args.serve_isolated)
@@ -633,7 +635,8 @@ def main():
     ap.add_argument("--serve-load", type=float, default=0.8)
     ap.add_argument("--serve-slots", type=int, default=1 << 21)
     ap.add_argument("--serve-iterations", type=int, default=0, help="bounces per server step (0: the server's default)")
-    ap.add_argument("--serve-schedule", choices=("srpt", "fifo", "hrrn", "share"), default="srpt")
+    ap.add_argument("--serve-schedule", choices=("srpt", "fifo", "hrrn", "share", "deadline"), default="srpt")
+    ap.add_argument("--serve-stretch", type=float, default=0.0, help="deadline schedule: deadline = submit + stretch x estimated service (0: 4)")
     ap.add_argument("--serve-pass-samples", type=int, default=0, help="pixel samples per job pass (0: the server's default)")
     ap.add_argument("--serve-min-items", type=int, default=0, help="split smaller jobs into sample sub-streams (0: the server's default)")
     ap.add_argument("--serve-express-slots", type=int, default=0, help="path slots of the express lane (0: the server's default, -1: none)")

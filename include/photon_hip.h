@@ -254,7 +254,7 @@ typedef struct ph_server_config {
     int32_t iterations_per_step; /* bounces per ph_server_step (0: 4) */
     int32_t max_jobs;            /* jobs admitted at once, at most 1024 (0: 1024) */
     int64_t pass_samples;        /* pixel samples per pass of a job (0: 4 x path_slots) */
-    int32_t schedule;            /* PH_SCHEDULE_SRPT, _FIFO, _HRRN or _SHARE */
+    int32_t schedule;            /* PH_SCHEDULE_SRPT, _FIFO, _HRRN, _SHARE or _DEADLINE */
     int32_t min_items;           /* a job with fewer pixels runs each pixel's samples as sub-streams so that
                                     pixels x sub-streams reaches this (0: an eighth of the main lane's slots,
                                     all of the express lane's; -1 or 1: never split) */
@@ -263,6 +263,8 @@ typedef struct ph_server_config {
     int32_t express_slots;       /* path slots of the express lane, a second engine on its own stream that
                                     runs concurrently with the main lane (0: path_slots / 2; -1: no lane) */
     int32_t express_iterations;  /* bounces per run of the express lane (0: iterations_per_step) */
+    double deadline_stretch;     /* PH_SCHEDULE_DEADLINE: a job's deadline is its submission time plus this
+                                    many times its estimated service time (0: 4) */
 } ph_server_config;
 
 #define PH_SCHEDULE_SRPT 0 /* unstarted work of the job with the least remaining work first */
@@ -272,6 +274,9 @@ typedef struct ph_server_config {
 #define PH_SCHEDULE_SHARE 3 /* processor sharing: the running jobs' unstarted work items are handed out in
                                round-robin slices, so every running job gets an equal share of the slots that
                                free up (a job that cannot use its share leaves it to the others) */
+#define PH_SCHEDULE_DEADLINE 4 /* earliest deadline first, with deadline = submission time + deadline_stretch x
+                                  the job's nominal service time (its samples at 1e6 samples per ms): a job
+                                  waits at most about that multiple of its own size behind newer work */
 
 typedef struct ph_job_desc {
     int32_t scene; /* id returned by ph_server_add_scene */

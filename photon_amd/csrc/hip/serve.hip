@@ -52,6 +52,7 @@ struct Job {
     int passes_queued = 0;  // passes queued so far
     int pass_k0 = 0, pass_k1 = 0;
     bool pass_pending = false;  // the next pass could not be queued yet
+    double deadline = 0.0;      // PH_SCHEDULE_DEADLINE: fixed when the job is admitted
     DevBuf<float4> accum;
 };
 
@@ -91,8 +92,11 @@ struct ph_server {
         if (cfg.path_slots <= 0) cfg.path_slots = 1 << 21;
         if (cfg.iterations_per_step <= 0) cfg.iterations_per_step = 4;
         if (cfg.max_jobs <= 0 || cfg.max_jobs > WavefrontEngine::kMaxJobs) cfg.max_jobs = WavefrontEngine::kMaxJobs;
-        if (cfg.schedule != PH_SCHEDULE_SRPT && cfg.schedule != PH_SCHEDULE_FIFO && cfg.schedule != PH_SCHEDULE_HRRN && cfg.schedule != PH_SCHEDULE_SHARE)
+        if (cfg.schedule != PH_SCHEDULE_SRPT && cfg.schedule != PH_SCHEDULE_FIFO && cfg.schedule != PH_SCHEDULE_HRRN && cfg.schedule != PH_SCHEDULE_SHARE &&
+            cfg.schedule != PH_SCHEDULE_DEADLINE)
             throw std::invalid_argument("unknown schedule");
+        if (!(cfg.deadline_stretch >= 0.0)) throw std::invalid_argument("deadline_stretch must be >= 0");
+        if (cfg.deadline_stretch == 0.0) cfg.deadline_stretch = 4.0;
         if (cfg.express_slots == 0) cfg.express_slots = std::max(1, cfg.path_slots / 2);
         if (cfg.express_slots < 0) cfg.express_slots = 0;
         if (cfg.express_samples <= 0) cfg.express_samples = int64_t(16) * cfg.express_slots;
@@ -155,6 +159,10 @@ struct ph_server {
             j.entry = entry;
             j.status.state = PH_JOB_RUNNING;
             j.status.admit_ms = now_ms();
+            // The service estimate: the job's samples at a fixed nominal rate of 1e6 samples/ms. Only the
+            // order of deadlines matters, and a fixed rate keeps the order of jobs admitted at different
+            // times consistent (the lanes' measured rates vary with the mix).
+            j.deadline = j.status.submit_ms + cfg.deadline_stretch * double(j.status.samples) * 1e-6;
             const int64_t per_pass = L.pass_samples / std::max<int64_t>(1, j.items);
             j.pass_spp = int(std::min<int64_t>(std::max<int64_t>(per_pass, 1), j.desc.spp / j.split));
             queue_next_pass(L, j);
@@ -165,11 +173,13 @@ struct ph_server {
     }
 
     // Lower runs first. SRPT: the job's remaining samples; FIFO: queue order; HRRN: minus the response
-    // ratio (waiting time + estimated remaining service) / estimated remaining service.
+    // ratio (waiting time + estimated remaining service) / estimated remaining service; DEADLINE: the
+    // job's deadline.
     double priority_of(const Lane& L, const Job& j, int k0) const {
         const double remaining = double(j.items) * double(j.desc.spp / j.split - k0);
         if (cfg.schedule == PH_SCHEDULE_FIFO || cfg.schedule == PH_SCHEDULE_SHARE) return 0.0;
         if (cfg.schedule == PH_SCHEDULE_SRPT) return remaining;
+        if (cfg.schedule == PH_SCHEDULE_DEADLINE) return j.deadline;
         const double service = remaining / std::max(L.rate, 1e-3);
         const double waited = now_ms() - j.status.submit_ms;
         return -(waited + service) / std::max(service, 1e-6);

@@ -2,12 +2,12 @@
 // (src/denoise/svgf.cu:19-255): temporal accumulation -> variance -> a-trous pass 0 -> history ->
 // a-trous passes 1..4 -> remodulation.
 //
-// MI355X layout: the reference's planes are float4 per pixel (16-byte loads). The variance estimate and
-// the a-trous passes stage their footprint through LDS: a 16x16 output tile reads a (16 + 2r)^2 halo of
-// the G-buffer planes it needs once per block instead of once per tap. The larger a-trous steps (4, 8,
-// 16) gather directly through the caches: a wave's loads for one tap cover 4 rows of 16 neighbouring
-// pixels, so they stay coalesced, and staging by residue class mod the step (so that a block's taps
-// share one lattice halo) measured slower, its staging loads being strided.
+// MI355X layout: the reference's planes are float4 per pixel (16-byte loads). The variance estimate
+// stages its footprint through LDS: a 16x16 output tile reads the 22x22 halo of the G-buffer planes it
+// needs once per block instead of once per tap. The a-trous passes run on lattice rows: a block takes
+// 64 neighbouring pixels of 4 rows spaced one step apart, whose taps all fall on 8 rows of one lattice,
+// and stages those rows whole (contiguous loads and conflict-free LDS reads, where square tiles staged
+// by residue class would load strided).
 //
 // Two input formats share the kernels through an accessor policy: the reference's five float4 planes
 // (RefIn), and the real-time workload's compact G-buffer (CompactIn: 12-byte colour, albedo and emission,
@@ -372,63 +372,35 @@ PH_DEV void finish_pass(const In& in, int i, float mat, float4 res, float4* dst,
     if (tail.remod) tail.remod[i] = remodulated(in, i, mat, res);
 }
 
-// One a-trous pass on 16x16 tiles. For steps 1 and 2 the block stages src, normal+depth and the material
-// id of its (16 + 4*step + 2)^2 footprint in LDS; Step 0 (any step) reads through the caches.
-template <int Step, class In, class Hist>
-__global__ __launch_bounds__(TX* TY) void atrous_kernel(In in, int w, int h, int step_rt, float phi_color, float phi_normal, float phi_depth,
-                                                         const float4* src, float4* dst, PassTail<In, Hist> tail) {
-    constexpr bool Tiled = Step > 0;
-    constexpr int R = Tiled ? 2 * Step + 1 : 1;  // taps reach 2*step; the 3x3 prefilter and gradient need 1
-    constexpr int SX = Tiled ? TX + 2 * R : 1, SY = Tiled ? TY + 2 * R : 1;
-    __shared__ float4 s_src[SY][SX];
-    __shared__ float4 s_nd[SY][SX];
-    __shared__ float s_mat[SY][SX];
-    const int step = Tiled ? Step : step_rt;
-    if (Tiled) {
-        for (int t = threadIdx.y * TX + threadIdx.x; t < SX * SY; t += TX * TY) {
-            const int sy = t / SX, sx = t % SX;
-            // Clamp to the image: the prefilter and the gradient clamp their coordinates, and the
-            // a-trous taps skip out-of-image positions before reading.
-            const int gx = min(max(int(blockIdx.x) * TX - R + sx, 0), w - 1), gy = min(max(int(blockIdx.y) * TY - R + sy, 0), h - 1);
-            const int j = gy * w + gx;
-            s_src[sy][sx] = src[j];
-            s_nd[sy][sx] = in.ndz_w(j);
-            s_mat[sy][sx] = in.mat(j);
-        }
-        __syncthreads();
-    }
+// One a-trous pass at any step, gathering every tap through the caches: the fallback for steps beyond
+// the lattice kernel's (a-trous iterations past 5).
+template <class In, class Hist>
+__global__ __launch_bounds__(TX* TY) void atrous_gather_kernel(In in, int w, int h, int step, float phi_color, float phi_normal, float phi_depth,
+                                                                const float4* src, float4* dst, PassTail<In, Hist> tail) {
     const int x = blockIdx.x * TX + threadIdx.x, y = blockIdx.y * TY + threadIdx.y;
     if (x >= w || y >= h) return;
     const int i = y * w + x;
-    const int lx = threadIdx.x + R, ly = threadIdx.y + R;
-    const float4 center = Tiled ? s_src[ly][lx] : src[i];
-    const float mat = Tiled ? s_mat[ly][lx] : in.mat(i);
+    const float4 center = src[i];
+    const float mat = in.mat(i);
     float4 res = center;
     if (mat >= 0) {
         auto cx = [&](int dx) { return min(max(x + dx, 0), w - 1); };
         auto cy = [&](int dy) { return min(max(y + dy, 0), h - 1); };
-        if (Tiled)
-            res = atrous_pixel(
-                x, y, w, h, step, phi_color, phi_normal, phi_depth, center, mat, s_nd[ly][lx],
-                [&](int dx, int dy) { return s_src[ly + dy * Step][lx + dx * Step]; }, [&](int dx, int dy) { return s_nd[ly + dy * Step][lx + dx * Step]; },
-                [&](int dx, int dy) { return s_mat[ly + dy * Step][lx + dx * Step]; }, [&](int dx, int dy) { return s_src[ly + dy][lx + dx].w; },
-                [&](int dx, int dy) { return s_nd[ly + dy][lx + dx].w; });
-        else
-            res = atrous_pixel(
-                x, y, w, h, step, phi_color, phi_normal, phi_depth, center, mat, in.ndz_w(i),
-                [&](int dx, int dy) { return src[i + dy * step * w + dx * step]; }, [&](int dx, int dy) { return in.ndz_w(i + dy * step * w + dx * step); },
-                [&](int dx, int dy) { return in.mat(i + dy * step * w + dx * step); }, [&](int dx, int dy) { return src[cy(dy) * w + cx(dx)].w; },
-                [&](int dx, int dy) { return in.ndz_w(cy(dy) * w + cx(dx)).w; });
+        res = atrous_pixel(
+            x, y, w, h, step, phi_color, phi_normal, phi_depth, center, mat, in.ndz_w(i),
+            [&](int dx, int dy) { return src[i + dy * step * w + dx * step]; }, [&](int dx, int dy) { return in.ndz_w(i + dy * step * w + dx * step); },
+            [&](int dx, int dy) { return in.mat(i + dy * step * w + dx * step); }, [&](int dx, int dy) { return src[cy(dy) * w + cx(dx)].w; },
+            [&](int dx, int dy) { return in.ndz_w(cy(dy) * w + cx(dx)).w; });
     }
     finish_pass(in, i, mat, res, dst, tail);
 }
 
-// One a-trous pass for the larger steps (4, 8, 16) on lattice rows: a block is 64 neighbouring pixels
-// of each of 4 rows spaced `Step` apart (rows y0, y0 + Step, ...), so that all its taps fall on 8 rows
-// of the same lattice. The block stages those 8 rows, 64 + 4 * Step columns wide, in LDS: coalesced
-// loads of about 2.5-4 texels per pixel instead of 25 gathers per pixel through the caches. The
-// prefilter's and gradient's neighbours in the rows above and below (not on the lattice) are read from
-// memory.
+// One a-trous pass on lattice rows (steps 1 to 16): a block is 64 neighbouring pixels of each of 4 rows
+// spaced `Step` apart (rows y0, y0 + Step, ...), so that all its taps fall on 8 rows of the same lattice.
+// The block stages those 8 rows, 64 + 4 * Step columns wide, in LDS: coalesced loads of about 2 to 4
+// texels per pixel instead of 25 gathers per pixel through the caches, and LDS reads in which a wave's
+// lanes take consecutive entries. The prefilter's and gradient's neighbours in the rows above and below
+// are staged rows at step 1 and are read from memory otherwise.
 constexpr int kLatticeX = 64, kLatticeRows = 4;
 
 template <int Step, class In, class Hist>
@@ -443,8 +415,8 @@ __global__ __launch_bounds__(kLatticeX* kLatticeRows) void atrous_lattice_kernel
     const int x0 = blockIdx.x * kLatticeX;
     for (int t = threadIdx.y * kLatticeX + threadIdx.x; t < SR * SW; t += kLatticeX * kLatticeRows) {
         const int r = t / SW, c = t % SW;
-        // Clamped like the tiles: taps never read out-of-image entries, the prefilter's row-y
-        // neighbours (columns x +- 1) read the clamped ones.
+        // Clamped to the image: taps never read out-of-image entries, and the prefilter's and gradient's
+        // neighbours (x +- 1, and y +- 1 at step 1) read the clamped ones, as their clamped coordinates ask.
         const int gx = min(max(x0 - 2 * Step + c, 0), w - 1), gy = min(max(residue + (k0 - 2 + r) * Step, 0), h - 1);
         const int j = gy * w + gx;
         s_src[r][c] = src[j];
@@ -466,8 +438,8 @@ __global__ __launch_bounds__(kLatticeX* kLatticeRows) void atrous_lattice_kernel
             x, y, w, h, Step, phi_color, phi_normal, phi_depth, center, mat, s_nd[lr][lc],
             [&](int dx, int dy) { return s_src[lr + dy][lc + dx * Step]; }, [&](int dx, int dy) { return s_nd[lr + dy][lc + dx * Step]; },
             [&](int dx, int dy) { return s_mat[lr + dy][lc + dx * Step]; },
-            [&](int dx, int dy) { return dy == 0 ? s_src[lr][lc + dx].w : src[cy(dy) * w + cx(dx)].w; },
-            [&](int dx, int dy) { return dy == 0 ? s_nd[lr][lc + dx].w : in.ndz_w(cy(dy) * w + cx(dx)).w; });
+            [&](int dx, int dy) { return (dy == 0 || Step == 1) ? s_src[lr + dy][lc + dx].w : src[cy(dy) * w + cx(dx)].w; },
+            [&](int dx, int dy) { return (dy == 0 || Step == 1) ? s_nd[lr + dy][lc + dx].w : in.ndz_w(cy(dy) * w + cx(dx)).w; });
     }
     finish_pass(in, i, mat, res, dst, tail);
 }
@@ -525,11 +497,11 @@ void run_passes(const PassPlanes& pl, hipStream_t st, const In& in, const Hist& 
         // The last pass's plane is needed only when it is plane 6, the variance buffer a display reads.
         PassTail<In, Hist> tail{!last || 1 - src == 0, it == 0, illum, moments, prev_illum, prev_moments, prev, last ? output : nullptr};
         if (step == 1)
-            hipLaunchKernelGGL((atrous_kernel<1, In, Hist>), grid, tile, 0, st, in, w, h, step, s.phi_color, s.phi_normal, s.phi_depth, filter[src],
-                               filter[1 - src], tail);
+            hipLaunchKernelGGL((atrous_lattice_kernel<1, In, Hist>), lattice_grid(w, h, 1), dim3(kLatticeX, kLatticeRows), 0, st, in, w, h, s.phi_color,
+                               s.phi_normal, s.phi_depth, filter[src], filter[1 - src], tail);
         else if (step == 2)
-            hipLaunchKernelGGL((atrous_kernel<2, In, Hist>), grid, tile, 0, st, in, w, h, step, s.phi_color, s.phi_normal, s.phi_depth, filter[src],
-                               filter[1 - src], tail);
+            hipLaunchKernelGGL((atrous_lattice_kernel<2, In, Hist>), lattice_grid(w, h, 2), dim3(kLatticeX, kLatticeRows), 0, st, in, w, h, s.phi_color,
+                               s.phi_normal, s.phi_depth, filter[src], filter[1 - src], tail);
         else if (step == 4)
             hipLaunchKernelGGL((atrous_lattice_kernel<4, In, Hist>), lattice_grid(w, h, 4), dim3(kLatticeX, kLatticeRows), 0, st, in, w, h, s.phi_color,
                                s.phi_normal, s.phi_depth, filter[src], filter[1 - src], tail);
@@ -540,7 +512,7 @@ void run_passes(const PassPlanes& pl, hipStream_t st, const In& in, const Hist& 
             hipLaunchKernelGGL((atrous_lattice_kernel<16, In, Hist>), lattice_grid(w, h, 16), dim3(kLatticeX, kLatticeRows), 0, st, in, w, h, s.phi_color,
                                s.phi_normal, s.phi_depth, filter[src], filter[1 - src], tail);
         else
-            hipLaunchKernelGGL((atrous_kernel<0, In, Hist>), grid, tile, 0, st, in, w, h, step, s.phi_color, s.phi_normal, s.phi_depth, filter[src],
+            hipLaunchKernelGGL((atrous_gather_kernel<In, Hist>), grid, tile, 0, st, in, w, h, step, s.phi_color, s.phi_normal, s.phi_depth, filter[src],
                                filter[1 - src], tail);
         src = 1 - src;
     }

@@ -47,7 +47,8 @@ class RenderStats(C.Structure):
 class ServerConfig(C.Structure):
     _fields_ = [("memory_budget", C.c_int64), ("path_slots", C.c_int32), ("iterations_per_step", C.c_int32),
                 ("max_jobs", C.c_int32), ("pass_samples", C.c_int64), ("schedule", C.c_int32), ("min_items", C.c_int32),
-                ("express_samples", C.c_int64), ("express_slots", C.c_int32), ("express_iterations", C.c_int32)]
+                ("express_samples", C.c_int64), ("express_slots", C.c_int32), ("express_iterations", C.c_int32),
+                ("deadline_stretch", C.c_double)]
 
 
 class JobDesc(C.Structure):
@@ -61,7 +62,7 @@ class JobStatus(C.Structure):
 
 
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_RELEASED = range(4)
-SCHEDULE_SRPT, SCHEDULE_FIFO, SCHEDULE_HRRN, SCHEDULE_SHARE = 0, 1, 2, 3
+SCHEDULE_SRPT, SCHEDULE_FIFO, SCHEDULE_HRRN, SCHEDULE_SHARE, SCHEDULE_DEADLINE = 0, 1, 2, 3, 4
 
 
 class SvgfInput(C.Structure):
@@ -472,10 +473,10 @@ class Server:
     """The serving loop (ph_server_*): resident scenes, jobs admitted by HBM footprint, continuous batching."""
 
     def __init__(self, memory_budget=0, path_slots=0, iterations_per_step=0, max_jobs=0, pass_samples=0, schedule=SCHEDULE_SRPT, min_items=0,
-                 express_samples=0, express_slots=0, express_iterations=0):
+                 express_samples=0, express_slots=0, express_iterations=0, deadline_stretch=0.0):
         self._h = C.c_void_p()
         cfg = ServerConfig(memory_budget, path_slots, iterations_per_step, max_jobs, pass_samples, schedule, min_items,
-                           express_samples, express_slots, express_iterations)
+                           express_samples, express_slots, express_iterations, deadline_stretch)
         _check(lib().ph_server_create(C.byref(cfg), C.byref(self._h)))
         self._sizes = {}
 

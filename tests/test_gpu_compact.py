@@ -113,3 +113,27 @@ def test_renderer_compact_frames_denoise():
         assert _within(a, b).mean() >= 0.999, f"frame {k}"
         prev = cam
     r.close()
+
+
+@pytest.mark.parametrize("iterations", [0, 1, 3, 6])
+def test_compact_svgf_iteration_counts_match_oracle(iterations):
+    """Other a-trous iteration counts than the default 5: none (the history and remodulation launches),
+    fewer (the last pass remodulates at steps 1 and 4), and 6 (step 32, past the lattice kernel's steps,
+    on the gathering fallback)."""
+    scene = host.Scene.builtin("cornell-c1").build()
+    w, h = 136, 80
+    frames = _frames(scene, w, h, 4, seed=11)
+    sv = SvgfSettings.make(atrous_iterations=iterations)
+    ora = O.SvgfOracle(w, h)
+    f = gpu.Svgf()
+    f.resize(w, h)
+    out = gpu.DeviceArray(w * h * 16)
+    for k, (cam, prev, c) in enumerate(frames):
+        gb = gpu.CompactGBuffer.from_numpy(c, w, h)
+        f.run_compact(gb, cam, prev, sv, out.ptr)
+        a = out.numpy(np.float32, (h, w, 4))
+        b = ora.run_compact(c, cam, prev, sv)
+        ok = _within(a, b)
+        print(f"{iterations} iterations, frame {k}: {100 * ok.mean():.3f}% within 1e-3")
+        assert np.all(np.isfinite(a))
+        assert ok.mean() >= 0.999

@@ -79,6 +79,8 @@ def test_split_jobs_match_the_renderer_and_themselves(scenes):
                                  dict(path_slots=3000, pass_samples=4000, schedule=gpu.SCHEDULE_HRRN),
                                  dict(path_slots=3000, pass_samples=5000, schedule=gpu.SCHEDULE_SHARE),
                                  dict(path_slots=1 << 14, schedule=gpu.SCHEDULE_SHARE, iterations_per_step=1),
+                                 dict(path_slots=3000, pass_samples=4000, schedule=gpu.SCHEDULE_DEADLINE),
+                                 dict(path_slots=1 << 14, schedule=gpu.SCHEDULE_DEADLINE, deadline_stretch=0.5, iterations_per_step=1),
                                  dict(path_slots=1 << 14, express_slots=-1),
                                  dict(path_slots=4096, express_slots=512, express_samples=40000, express_iterations=1),
                                  dict(path_slots=2048, express_slots=2048, express_samples=12000, iterations_per_step=1)])
