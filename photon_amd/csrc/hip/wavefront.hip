@@ -331,23 +331,13 @@ PH_DEV bool idle_takes_work(const unsigned long long* items) {
     return Multi && Kind == K_DONE && items[0] < items[1];
 }
 
-// Sum over the block of one value per thread (all threads get it).
-PH_DEV int block_sum(int v, int* red) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    const int wave = threadIdx.x >> 6;
-    __syncthreads();
-    if (lane_id() == 0) red[wave] = v;
-    __syncthreads();
-    int t = 0;
-    for (int w = 0; w < kCompactThreads / 64; w++) t += red[w];
-    return t;
-}
-
+// Per-chunk counts of every category: partial[k * nchunks + chunk].
 template <int Kind, bool Multi>
 __global__ __launch_bounds__(kCompactThreads) void compact_count_kernel(const uint8_t* stage, int n, const unsigned long long* items, int* partial,
                                                                        int nchunks) {
     constexpr int K = Compact<Kind>::cats;
-    __shared__ int red[kCompactThreads / 64];
+    constexpr int W = kCompactThreads / 64;
+    __shared__ int red[K][W];
     const bool idle = idle_takes_work<Kind, Multi>(items);
     const int first = blockIdx.x * kCompactChunk + threadIdx.x * kCompactPerThread;
     uint8_t b[16];
@@ -358,21 +348,95 @@ __global__ __launch_bounds__(kCompactThreads) void compact_count_kernel(const ui
         const unsigned m = (first + j < n) ? Compact<Kind>::bits(b[j], idle) : 0u;
         for (int k = 0; k < K; k++) cnt[k] += (m >> k) & 1u;
     }
+    const int wave = threadIdx.x >> 6;
     for (int k = 0; k < K; k++) {
-        const int t = block_sum(cnt[k], red);
-        if (threadIdx.x == 0) partial[k * nchunks + blockIdx.x] = t;
+        int v = cnt[k];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if (lane_id() == 0) red[k][wave] = v;
+    }
+    __syncthreads();
+    if (int(threadIdx.x) < K) {
+        int t = 0;
+        for (int w = 0; w < W; w++) t += red[threadIdx.x][w];
+        partial[threadIdx.x * nchunks + blockIdx.x] = t;
     }
 }
 
-// Writes the slots of each category in slot order: chunk offset (sum of the earlier chunks' counts),
-// then the exclusive scan of the per-thread counts inside the chunk. The last chunk writes the totals.
+// Each chunk's first output position in each category: the totals of the earlier categories of the same
+// list, plus the exclusive prefix of the chunk's count within its category. One block per category
+// (each also sums the earlier categories of its list); the positions go to `first_pos`, laid out like
+// `partial`. The block of a list's last category writes the list's length, and for the finished paths
+// the block of category 2 writes how many take new work.
+constexpr int kScanThreads = 1024;
+
+template <int Kind>
+__global__ __launch_bounds__(kScanThreads) void compact_scan_kernel(const int* partial, int* first_pos, int nchunks, int* counters) {
+    constexpr int K = Compact<Kind>::cats;
+    constexpr int W = kScanThreads / 64;
+    __shared__ int wsum[W];
+    const int k = blockIdx.x;
+    const int l = Compact<Kind>::list(k);
+    const int lane = lane_id(), wave = threadIdx.x >> 6;
+    // Sum over the block (all threads get it).
+    auto block_total = [&](int v) {
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if (lane == 0) wsum[wave] = v;
+        __syncthreads();
+        int t = 0;
+        for (int w = 0; w < W; w++) t += wsum[w];
+        __syncthreads();
+        return t;
+    };
+    int earlier = 0;
+    for (int k2 = 0; k2 < k; k2++)
+        if (Compact<Kind>::list(k2) == l)
+            for (int i = threadIdx.x; i < nchunks; i += kScanThreads) earlier += partial[k2 * nchunks + i];
+    const int base = block_total(earlier);
+    // Exclusive scan of this category's counts: consecutive runs per thread, then over the block.
+    const int per = (nchunks + kScanThreads - 1) / kScanThreads;
+    const int lo = min(nchunks, int(threadIdx.x) * per), hi = min(nchunks, lo + per);
+    const int* p = partial + k * nchunks;
+    int s = 0;
+    for (int i = lo; i < hi; i++) s += p[i];
+    int v = s;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int u = __shfl_up(v, off);
+        if (lane >= off) v += u;
+    }
+    if (lane == 63) wsum[wave] = v;
+    __syncthreads();
+    int before = 0, total = 0;
+    for (int w = 0; w < W; w++) {
+        const int t = wsum[w];
+        if (w < wave) before += t;
+        total += t;
+    }
+    int run = base + before + v - s;
+    for (int i = lo; i < hi; i++) {
+        first_pos[k * nchunks + i] = run;
+        run += p[i];
+    }
+    if (threadIdx.x == 0) {
+        if (k == K - 1 || Compact<Kind>::list(k + 1) != l) counters[Compact<Kind>::counter(l)] = base + total;
+        if (Kind == K_DONE && k == 2) counters[C_NEED] = total;  // finished paths taking new work
+    }
+}
+
+// Writes the slots of each category in slot order. The block first places its entries in LDS, category
+// after category (positions from a scan of the per-thread counts), then copies each category's run to
+// its place in the output list (the chunk's first position from compact_scan_kernel) with coalesced
+// stores.
 template <int Kind, bool Multi>
-__global__ __launch_bounds__(kCompactThreads) void compact_scatter_kernel(const uint8_t* stage, int n, int* counters, const unsigned long long* items,
-                                                                         const int* partial, int nchunks, QueueOut out) {
+__global__ __launch_bounds__(kCompactThreads) void compact_scatter_kernel(const uint8_t* stage, int n, const unsigned long long* items,
+                                                                         const int* first_pos, int nchunks, QueueOut out) {
     constexpr int K = Compact<Kind>::cats;
     constexpr int W = kCompactThreads / 64;
-    __shared__ int red[W];
+    // A slot is in one category, except that a finished path can also have a shadow ray pending.
+    constexpr int kMaxEntries = (Kind == K_DONE ? 2 : 1) * kCompactChunk;
     __shared__ int wave_tot[K][W];
+    __shared__ int seg[K + 1];  // the block's entries of category k are buf[seg[k] .. seg[k + 1])
+    __shared__ int dst[K];      // where they go in their list
+    __shared__ int buf[kMaxEntries];
     const bool idle = idle_takes_work<Kind, Multi>(items);
     const int c = blockIdx.x;
     const int first = c * kCompactChunk + threadIdx.x * kCompactPerThread;
@@ -386,37 +450,43 @@ __global__ __launch_bounds__(kCompactThreads) void compact_scatter_kernel(const 
         for (int k = 0; k < K; k++) cnt[k] += (m[j] >> k) & 1;
     }
     const int lane = lane_id(), wave = threadIdx.x >> 6;
-    int pos[K], cat_tot[K];
-    int offset[Compact<Kind>::lists] = {};  // entries of the list's earlier categories
+    // Inclusive scans over the wave, then over the waves of the block.
+    int incl[K];
     for (int k = 0; k < K; k++) {
-        int before = 0, all = 0;
-        for (int i = threadIdx.x; i < nchunks; i += kCompactThreads) {
-            const int v = partial[k * nchunks + i];
-            if (i < c) before += v;
-            all += v;
-        }
-        const int l = Compact<Kind>::list(k);
-        const int base = block_sum(before, red) + offset[l];
-        cat_tot[k] = block_sum(all, red);
-        offset[l] += cat_tot[k];
-        // Inclusive scan over the wave, then over the waves of the block.
         int v = cnt[k];
         for (int off = 1; off < 64; off <<= 1) {
             const int u = __shfl_up(v, off);
             if (lane >= off) v += u;
         }
+        incl[k] = v;
         if (lane == 63) wave_tot[k][wave] = v;
-        __syncthreads();
+    }
+    if (int(threadIdx.x) < K) dst[threadIdx.x] = first_pos[threadIdx.x * nchunks + c];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int k = 0; k < K; k++) {
+            seg[k] = acc;
+            for (int w = 0; w < W; w++) acc += wave_tot[k][w];
+        }
+        seg[K] = acc;
+    }
+    __syncthreads();
+    int pos[K];
+    for (int k = 0; k < K; k++) {
         int prev = 0;
         for (int w = 0; w < wave; w++) prev += wave_tot[k][w];
-        pos[k] = base + prev + v - cnt[k];
+        pos[k] = seg[k] + prev + incl[k] - cnt[k];
     }
     for (int j = 0; j < 16; j++)
         for (int k = 0; k < K; k++)
-            if ((m[j] >> k) & 1) out.q[Compact<Kind>::list(k)][pos[k]++] = first + j;
-    if (c == nchunks - 1 && threadIdx.x == 0) {
-        for (int l = 0; l < Compact<Kind>::lists; l++) counters[Compact<Kind>::counter(l)] = offset[l];
-        if (Kind == K_DONE) counters[C_NEED] = cat_tot[2];
+            if ((m[j] >> k) & 1) buf[pos[k]++] = first + j;
+    __syncthreads();
+    const int total = seg[K];
+    for (int e = threadIdx.x; e < total; e += kCompactThreads) {
+        int k = 0;
+        while (k + 1 < K && e >= seg[k + 1]) k++;
+        out.q[Compact<Kind>::list(k)][dst[k] + (e - seg[k])] = buf[e];
     }
 }
 
@@ -792,7 +862,7 @@ struct WavefrontEngine::Impl {
     ph_wavefront_config cfg{128, 24, 0, 16, 1, 1, 16, 16, 0, 1, 1, 1};
     int slots = 0;
     int cus = 0;
-    DevBuf<int> job, pixel, sample, sample_end, part, counters, q_ext, q_shadow, q_term, q_miss, q_cls[kClasses], mat_class, partial;
+    DevBuf<int> job, pixel, sample, sample_end, part, counters, q_ext, q_shadow, q_term, q_miss, q_cls[kClasses], mat_class, partial, first_pos;
     DevBuf<uint8_t> stage;
     DevBuf<int> job_done;
     DevBuf<long long> chunk_first;
@@ -873,6 +943,7 @@ struct WavefrontEngine::Impl {
         for (auto& q : q_cls) q.allocate(size_t(n));
         stage.allocate(size_t(n + 15) / 16 * 16);
         partial.allocate(size_t(kMaxCategories) * size_t(chunks_of(n)));
+        first_pos.allocate(size_t(kMaxCategories) * size_t(chunks_of(n)));
         rng.allocate(size_t(n));
         for (DevBuf<float4>* b : {&ro, &rd, &beta, &L, &hit, &so, &sd, &sc}) b->allocate(size_t(n));
         slots = n;
@@ -928,8 +999,10 @@ struct WavefrontEngine::Impl {
             out.q[1] = s.q_term;
         }
         hipLaunchKernelGGL((compact_count_kernel<Kind, Multi>), dim3(nch), dim3(kCompactThreads), 0, st, s.stage, slots, items.ptr, partial.ptr, nch);
-        hipLaunchKernelGGL((compact_scatter_kernel<Kind, Multi>), dim3(nch), dim3(kCompactThreads), 0, st, s.stage, slots, counters.ptr, items.ptr,
-                           partial.ptr, nch, out);
+        hipLaunchKernelGGL((compact_scan_kernel<Kind>), dim3(Compact<Kind>::cats), dim3(kScanThreads), 0, st, partial.ptr, first_pos.ptr, nch,
+                           counters.ptr);
+        hipLaunchKernelGGL((compact_scatter_kernel<Kind, Multi>), dim3(nch), dim3(kCompactThreads), 0, st, s.stage, slots, items.ptr, first_pos.ptr,
+                           nch, out);
     }
 
     // One bounce. Event pairs bracket the traversal launches when timing.
