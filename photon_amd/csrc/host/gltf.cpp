@@ -136,34 +136,60 @@ std::string percent_decode(const std::string& uri) {
     return out;
 }
 
-bool base64_decode(const std::string& in, std::vector<uint8_t>& out) {
-    out.reserve(out.size() + in.size() / 4 * 3 + 3);
+// Sextet of each base64 digit; 64 for the white space a data URI may carry, 65 for '=', 66 otherwise.
+struct Base64Table {
+    uint8_t v[256];
+    constexpr Base64Table() : v() {
+        for (int c = 0; c < 256; c++) v[c] = 66;
+        for (int c = 0; c < 26; c++) {
+            v['A' + c] = uint8_t(c);
+            v['a' + c] = uint8_t(26 + c);
+        }
+        for (int c = 0; c < 10; c++) v['0' + c] = uint8_t(52 + c);
+        v[int('+')] = 62;
+        v[int('/')] = 63;
+        v[int('\n')] = v[int('\r')] = v[int(' ')] = 64;
+        v[int('=')] = 65;
+    }
+};
+constexpr Base64Table kBase64;
+
+// Decodes until the first '=' (or the end), skipping white space; false on any other character.
+bool base64_decode(std::string_view in, std::vector<uint8_t>& out) {
+    const size_t start = out.size();
+    out.resize(start + in.size() / 4 * 3 + 3);
+    uint8_t* dst = out.data() + start;
+    const uint8_t* p = reinterpret_cast<const uint8_t*>(in.data());
+    const uint8_t* const end = p + in.size();
+    // Whole quads of digits four characters at a time; anything else drops to the general loop.
+    while (end - p >= 4) {
+        const uint32_t a = kBase64.v[p[0]], b = kBase64.v[p[1]], c = kBase64.v[p[2]], d = kBase64.v[p[3]];
+        if ((a | b | c | d) >= 64) break;
+        const uint32_t q = (a << 18) | (b << 12) | (c << 6) | d;
+        dst[0] = uint8_t(q >> 16);
+        dst[1] = uint8_t(q >> 8);
+        dst[2] = uint8_t(q);
+        dst += 3;
+        p += 4;
+    }
     uint32_t acc = 0;
     int bits = 0;
-    for (const char ch : in) {
-        if (ch == '=') break;
-        int v;
-        if (ch >= 'A' && ch <= 'Z')
-            v = ch - 'A';
-        else if (ch >= 'a' && ch <= 'z')
-            v = ch - 'a' + 26;
-        else if (ch >= '0' && ch <= '9')
-            v = ch - '0' + 52;
-        else if (ch == '+')
-            v = 62;
-        else if (ch == '/')
-            v = 63;
-        else if (ch == '\n' || ch == '\r' || ch == ' ')
-            continue;
-        else
+    for (; p < end; p++) {
+        const uint32_t v = kBase64.v[*p];
+        if (v == 65) break;
+        if (v == 64) continue;
+        if (v > 65) {
+            out.resize(size_t(dst - out.data()));
             return false;
-        acc = (acc << 6) | uint32_t(v);
+        }
+        acc = (acc << 6) | v;
         bits += 6;
         if (bits >= 8) {
             bits -= 8;
-            out.push_back(uint8_t(acc >> bits));
+            *dst++ = uint8_t(acc >> bits);
         }
     }
+    out.resize(size_t(dst - out.data()));
     return true;
 }
 
@@ -318,12 +344,10 @@ public:
         std::vector<uint8_t> file;
         if (!read_file(utf8_path(path_), file)) fail("cannot open " + path_);
         dir_ = utf8_path(path_).parent_path();
-        std::string text;
+        std::string glb_text;
         const bool glb = file.size() >= 12 && std::memcmp(file.data(), "glTF", 4) == 0;
-        if (glb)
-            split_glb(file, text);
-        else
-            text.assign(file.begin(), file.end());
+        if (glb) split_glb(file, glb_text);
+        const std::string_view text = glb ? std::string_view(glb_text) : std::string_view(reinterpret_cast<const char*>(file.data()), file.size());
         try {
             doc_ = json::Document::read(text);
         } catch (const json::SyntaxError& e) {
@@ -404,12 +428,12 @@ private:
                 if (!glb) fail("buffer " + std::to_string(i) + " has no data");
                 data = glb_bin_;
             } else {
-                const std::string uri(b.member("uri").text());
-                if (uri.rfind("data:", 0) == 0) {
+                const std::string_view uri = b.member("uri").text();
+                if (uri.substr(0, 5) == "data:") {
                     const size_t comma = uri.find(',');
-                    if (comma == std::string::npos || !base64_decode(uri.substr(comma + 1), data)) fail("invalid data URI");
-                } else if (!read_file(dir_ / utf8_path(percent_decode(uri)), data)) {
-                    fail("cannot read buffer " + uri);
+                    if (comma == std::string_view::npos || !base64_decode(uri.substr(comma + 1), data)) fail("invalid data URI");
+                } else if (!read_file(dir_ / utf8_path(percent_decode(std::string(uri))), data)) {
+                    fail("cannot read buffer " + std::string(uri));
                 }
             }
             buffers_.push_back(std::move(data));

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <map>
 
 namespace phh::json {
@@ -104,9 +105,13 @@ private:
         std::string& out = doc_.chars_;
         const size_t lo = out.size();
         for (;;) {
-            // Plain runs (such as base64 buffers) are copied whole.
-            const size_t stop_at = in_.find_first_of("\"\\", at_);
-            if (stop_at == std::string_view::npos) stop("string not closed");
+            // Plain runs (such as base64 buffers) are copied whole: the next quote, and the first
+            // backslash before it, found with memchr.
+            const char* base = in_.data();
+            const void* q = std::memchr(base + at_, '"', in_.size() - at_);
+            if (!q) stop("string not closed");
+            size_t stop_at = size_t(static_cast<const char*>(q) - base);
+            if (const void* bs = std::memchr(base + at_, '\\', stop_at - at_)) stop_at = size_t(static_cast<const char*>(bs) - base);
             out.append(in_.data() + at_, stop_at - at_);
             at_ = stop_at + 1;
             if (in_[stop_at] == '"') break;

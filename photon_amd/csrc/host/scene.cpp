@@ -66,8 +66,11 @@ void append_mesh(Mesh& m, const std::vector<vec3>& p, const std::vector<int>& id
     extend_stream(m.nrm, n, first, p.size());
     extend_stream(m.uv, t, first, p.size());
     m.pos.insert(m.pos.end(), p.begin(), p.end());
-    m.corner.reserve(m.corner.size() + idx.size());
-    for (const int i : idx) m.corner.push_back(int(first) + i);
+    // resize keeps the vector's geometric growth; reserving the exact new size on every call would
+    // reallocate and copy the whole corner list once per appended mesh.
+    const size_t at = m.corner.size();
+    m.corner.resize(at + idx.size());
+    for (size_t k = 0; k < idx.size(); k++) m.corner[at + k] = int(first) + idx[k];
     m.mat.insert(m.mat.end(), idx.size() / 3, material);
 }
 

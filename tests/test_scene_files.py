@@ -769,3 +769,32 @@ def test_decoders_survive_corrupted_files(tmp_path):
             host.Scene.from_gltf(tmp_path / "m.gltf")
         except RuntimeError:
             pass
+
+
+def test_gltf_data_uri_base64_forms(tmp_path):
+    """A buffer's base64 with line breaks and spaces anywhere, or without its '=' padding, decodes to the
+    same bytes; any other character is refused."""
+    pos = np.array([0, 0, 0, 1, 0, 0, 0, 1, 0], np.float32).tobytes()
+    idx = np.array([0, 1, 2], np.uint16).tobytes() + b"\0\0"  # 44 bytes: the encoding ends in '=' padding
+    body = {"bufferViews": [{"buffer": 0, "byteOffset": 0, "byteLength": 36}, {"buffer": 0, "byteOffset": 36, "byteLength": 6}],
+            "accessors": [{"bufferView": 0, "componentType": 5126, "count": 3, "type": "VEC3", "min": [0, 0, 0], "max": [1, 1, 0]},
+                          {"bufferView": 1, "componentType": 5123, "count": 3, "type": "SCALAR"}],
+            "meshes": [{"primitives": [{"attributes": {"POSITION": 0}, "indices": 1}]}],
+            "nodes": [{"mesh": 0}], "scenes": [{"nodes": [0]}], "scene": 0}
+    enc = base64.b64encode(pos + idx).decode()
+    assert enc.endswith("=")
+
+    def load(text, name):
+        doc = _doc_with_buffer(pos + idx, body)
+        doc["buffers"][0]["uri"] = "data:application/octet-stream;base64," + text
+        s = host.Scene.from_gltf(_write(tmp_path / name, doc))
+        r = s.raw()
+        return np.ctypeslib.as_array(r.positions, (r.num_vertices * 3,)).copy()
+
+    want = load(enc, "plain.gltf")
+    assert np.array_equal(want, np.frombuffer(pos, np.float32))
+    spaced = "".join(c + ("\n" if k % 7 == 6 else " " if k % 11 == 10 else "") for k, c in enumerate(enc))
+    assert np.array_equal(load(spaced, "spaced.gltf"), want)
+    assert np.array_equal(load(enc.rstrip("="), "unpadded.gltf"), want)
+    with pytest.raises(RuntimeError, match="invalid data URI"):
+        load(enc[:20] + "*" + enc[20:], "bad.gltf")
