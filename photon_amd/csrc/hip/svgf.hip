@@ -131,11 +131,15 @@ struct CompactHist {
     }
 };
 
+// Temporal accumulation (svgf.cu:37-94) on 64x4 blocks: a wave reads 64 neighbouring pixels of one row,
+// so the 12-byte colour planes of the compact G-buffer stream in whole cache lines.
+constexpr int kRowX = 64, kRowY = 4;
+
 template <class In, class Hist>
-__global__ __launch_bounds__(TX* TY) void temporal_kernel(In in, ph_camera_view prev_cam, int w, int h, int has_history, float alpha_c,
+__global__ __launch_bounds__(kRowX* kRowY) void temporal_kernel(In in, ph_camera_view prev_cam, int w, int h, int has_history, float alpha_c,
                                                            float alpha_m, const float4* prev_illum, const float4* prev_moments, Hist prev,
                                                            float4* out_illum, float4* out_moments) {
-    const int x = blockIdx.x * TX + threadIdx.x, y = blockIdx.y * TY + threadIdx.y;
+    const int x = blockIdx.x * kRowX + threadIdx.x, y = blockIdx.y * kRowY + threadIdx.y;
     if (x >= w || y >= h) return;
     const int i = y * w + x;
     const float mat = in.mat(i);
@@ -486,7 +490,7 @@ void run_passes(const PassPlanes& pl, hipStream_t st, const In& in, const Hist& 
     float4 *illum = pl.illum, *moments = pl.moments, *filter[2] = {pl.filter0, pl.filter1};
     const dim3 tile(TX, TY), grid((w + TX - 1) / TX, (h + TY - 1) / TY);
     const int lin = 256, lgrid = (n + lin - 1) / lin;
-    hipLaunchKernelGGL((temporal_kernel<In, Hist>), grid, tile, 0, st, in, prev_cam, w, h, int(pl.has_history), s.alpha_color, s.alpha_moments,
+    hipLaunchKernelGGL((temporal_kernel<In, Hist>), dim3((w + kRowX - 1) / kRowX, (h + kRowY - 1) / kRowY), dim3(kRowX, kRowY), 0, st, in, prev_cam, w, h, int(pl.has_history), s.alpha_color, s.alpha_moments,
                        prev_illum, prev_moments, prev, illum, moments);
     hipLaunchKernelGGL((variance_kernel<In>), grid, tile, 0, st, in, w, h, illum, moments, s.phi_normal, filter[0]);
     int src = 0;
