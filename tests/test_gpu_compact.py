@@ -41,6 +41,28 @@ def _within(a, b):
     return np.all((d <= 1e-3 * np.abs(b)) | (d <= 1e-6), axis=-1)
 
 
+@pytest.mark.parametrize("name,w,h", [("cornell-c1", 100, 75), ("materials", 64, 48), ("glass", 33, 17)])
+def test_gbuffer_sample_is_the_per_pixel_render(name, w, h):
+    """The real-time G-buffer sample is the per-pixel kernel's one-sample render of frame 0: the same
+    colour, albedo, normal and depth, bit for bit, also at sizes that leave partial blocks."""
+    scene = host.Scene.builtin(name).build()
+    rs = RenderSettings.make(w, h, seed=3)
+    cam = scene.camera_view(w / h)
+    planes = lambda r, bufs: [gpu.copy_device_to_numpy(r.buffer_ptr(b), w * h * 16, np.float32, (h, w, 4)) for b in bufs]
+    r4 = uploaded(scene, w, h)
+    r4.render_denoised(cam, cam, rs, SvgfSettings.make())
+    color, albedo, nd = planes(r4, (gpu.BUF_COLOR, gpu.BUF_ALBEDO, gpu.BUF_NORMAL_DEPTH))
+    r4.close()
+    r1 = uploaded(scene, w, h, engine=gpu.ENGINE_MEGAKERNEL)
+    r1.render(cam, rs, 1)
+    acc, albedo1, nd1 = planes(r1, (gpu.BUF_ACCUM, gpu.BUF_ALBEDO, gpu.BUF_NORMAL_DEPTH))
+    r1.close()
+    # The accumulation adds the sample to zero, which turns a -0 component into +0; values are compared.
+    assert np.array_equal(color[..., :3], acc[..., :3])
+    assert np.array_equal(albedo.view(np.uint32), albedo1.view(np.uint32))
+    assert np.array_equal(nd.view(np.uint32), nd1.view(np.uint32))
+
+
 @pytest.mark.parametrize("name", ["cornell-c1", "materials"])
 def test_compact_sample_is_the_encoded_float4_sample(name):
     scene = host.Scene.builtin(name).build()
