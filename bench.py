@@ -441,6 +441,72 @@ def draw_jobs(n, seed):
     return jobs
 
 
+def ideal_server(arrival_s, work_s):
+    """Response times of the same job stream on an ideal server: one resource of the saturated capacity,
+    preemptive at no cost, where a job of `work_s` seconds of work finishes after exactly that much service
+    and any share of the capacity can go to any job. Returns {"fifo", "srpt", "ps"} -> response seconds
+    per job, for first-come-first-served, shortest-remaining-work-first and processor sharing (every job
+    present gets an equal share). Arrivals must be sorted."""
+    import heapq
+    import numpy as np
+    a = np.asarray(arrival_s, np.float64)
+    w = np.asarray(work_s, np.float64)
+    n = len(a)
+    out = {}
+    done = np.zeros(n)
+    t = 0.0
+    for k in range(n):
+        t = max(t, a[k]) + w[k]
+        done[k] = t
+    out["fifo"] = done - a
+
+    done = np.zeros(n)
+    t, i, heap = 0.0, 0, []
+    while i < n or heap:
+        if not heap:
+            t = max(t, a[i])
+        while i < n and a[i] <= t:
+            heapq.heappush(heap, [w[i], i])
+            i += 1
+        rem, k = heap[0]
+        nxt = a[i] if i < n else np.inf
+        if t + rem <= nxt:
+            t += rem
+            heapq.heappop(heap)
+            done[k] = t
+        else:
+            heap[0][0] = rem - (nxt - t)
+            t = nxt
+    out["srpt"] = done - a
+
+    # Processor sharing: between events every present job receives 1/m of the capacity; the next event is
+    # an arrival or the completion of the job with the least remaining work. Remaining work is kept as
+    # attained-service offsets so that one event costs O(m log m) at most.
+    done = np.zeros(n)
+    t, i = 0.0, 0
+    attained = 0.0  # service each present job has received since the reference point
+    heap = []  # (work + attained at arrival, job): the job leaves when `attained` reaches the key
+    while i < n or heap:
+        if not heap:
+            t = max(t, a[i])
+        while i < n and a[i] <= t:
+            heapq.heappush(heap, (w[i] + attained, i))
+            i += 1
+        m = len(heap)
+        nxt = a[i] if i < n else np.inf
+        need = heap[0][0] - attained
+        if t + need * m <= nxt:
+            t += need * m
+            attained = heap[0][0]
+            while heap and heap[0][0] <= attained:
+                done[heapq.heappop(heap)[1]] = t
+        else:
+            attained += (nxt - t) / m
+            t = nxt
+    out["ps"] = done - a
+    return out
+
+
 def bench_serving(args, gpu, host, field, launch):
     """Open-loop Poisson arrivals at 80% of the saturated throughput over three resident scenes."""
     import numpy as np
@@ -561,6 +627,17 @@ def bench_serving(args, gpu, host, field, launch):
                pass_samples=args.serve_pass_samples, min_items=args.serve_min_items, express_slots=args.serve_express_slots,
                express_samples=args.serve_express_samples, express_iterations=args.serve_express_iterations,
                isolated=args.serve_isolated)
+    # The same arrivals and job sizes on an ideal server of the saturated capacity (no concurrency limits,
+    # free preemption): what the classical schedules reach on this stream, against the measured figure of
+    # the same definition (latency over the job's samples at the saturated rate).
+    arr_s = (arrival - arrival[0]) * 1e-3
+    ideal = {}
+    for name, resp in ideal_server(arr_s, samples / peak).items():
+        s = resp / (samples / peak)
+        p50, p99 = float(np.percentile(s, 50)), float(np.percentile(s, 99))
+        ideal[name] = {"p50": round(p50, 2), "p99": round(p99, 2), "p99_over_p50": round(p99 / p50, 2)}
+    res["offered_load"] = round(float(samples.sum() / peak / (arr_s[-1] + gaps[0])), 3)
+    res["ideal_server_slowdown_vs_peak_rate"] = ideal
     worst = np.argsort(slow)[-5:][::-1]
     res["worst_slowdown"] = [dict(res=f"{jobs[k]['w']}x{jobs[k]['h']}", spp=jobs[k]["spp"], depth=jobs[k]["depth"], scene=jobs[k]["scene"],
                                   latency_ms=round(float(L[k]), 1), isolated_ms=round(float(iso[k]), 1), slowdown=round(float(slow[k]), 1))
