@@ -83,10 +83,12 @@ def sample_bytes(cs, pixels):
     """Algorithmic bytes of one render call from its instrumented counts: every traced ray (extension and
     shadow, with their node, triangle and alpha tests), a shading vertex per extension ray (an upper
     bound: a miss reads one environment texel instead) and a light sample per shadow ray, plus the
-    accumulation."""
+    accumulation. The rays of the paths finished one per thread at the end of the render are not
+    instrumented; they are counted at the average bytes of the others."""
     rays = traversal_bytes(cs.rays_extend, cs.node_visits, cs.tri_tests, cs.alpha_tests)
     rays += traversal_bytes(cs.rays_shadow, cs.shadow_node_visits, cs.shadow_tri_tests, cs.shadow_alpha_tests)
-    return rays + cs.rays_extend * B_SHADE + cs.rays_shadow * B_NEE + pixels * B_ACC
+    b = rays + cs.rays_extend * B_SHADE + cs.rays_shadow * B_NEE
+    return b * (1 + cs.rays_finish / max(1, cs.rays_extend + cs.rays_shadow)) + pixels * B_ACC
 
 
 def traversal_bytes(rays, nodes, tris, alpha):
@@ -121,14 +123,14 @@ def bench_render(args, gpu, host, RenderSettings):
         r.render(cam, rs, SPP)
         log(f"warmup {i}: {r.stats().ms_total:.1f} ms")
     times = []
-    rays_ext = rays_sh = 0
+    rays_ext = rays_sh = rays_fin = finish_paths = 0
     for i in range(args.steps):
         r.reset()
         t0 = time.perf_counter()
         r.render(cam, rs, SPP)
         times.append(time.perf_counter() - t0)
         s = r.stats()
-        rays_ext, rays_sh = s.rays_extend, s.rays_shadow
+        rays_ext, rays_sh, rays_fin, finish_paths = s.rays_extend, s.rays_shadow, s.rays_finish, s.finish_paths
         log(f"step {i}: {times[-1] * 1e3:.1f} ms, {s.iterations} iterations")
     total = sum(times)
     samples = W * H * SPP
@@ -157,8 +159,8 @@ def bench_render(args, gpu, host, RenderSettings):
     b_sample = sample_bytes(cs, W * H) / samples
     out = dict(value=value, ms_per_step=total / args.steps * 1e3, samples=samples, config=cfg.as_dict() if cfg else r.wavefront_config().as_dict(),
                bytes_per_sample=b_sample, sample_achieved_gbs=b_sample * value * 1e6 / 1e9,
-               rays_extend=rays_ext, rays_shadow=rays_sh, rays_per_sample=(rays_ext + rays_sh) / samples,
-               mrays_s=(rays_ext + rays_sh) * args.steps / total / 1e6,
+               rays_extend=rays_ext, rays_shadow=rays_sh, rays_finish=rays_fin, finish_paths=finish_paths,
+               rays_per_sample=(rays_ext + rays_sh + rays_fin) / samples, mrays_s=(rays_ext + rays_sh + rays_fin) * args.steps / total / 1e6,
                extend_ms_total=ext_ms, shadow_ms_total=shadow_ms,
                extend_avg_launch_ms=avg_launch_ms, extend_bytes_per_launch=bytes_per_launch, achieved_gbs=achieved,
                node_visits_per_ray=cs.node_visits / max(1, cs.rays_extend), tri_tests_per_ray=cs.tri_tests / max(1, cs.rays_extend),
@@ -292,7 +294,7 @@ def bench_cornell(args, gpu, host, cpu):
     r.set_counting(False)
     r.close()
     b = sample_bytes(cs, W * H) / (W * H * SPP)
-    res.update(bytes_per_sample=round(b, 1), rays_per_sample=round((cs.rays_extend + cs.rays_shadow) / (W * H * SPP), 3),
+    res.update(bytes_per_sample=round(b, 1), rays_per_sample=round((cs.rays_extend + cs.rays_shadow + cs.rays_finish) / (W * H * SPP), 3),
                achieved_gbs=round(b * res["wavefront_msamples_s"] * 1e6 / 1e9, 1))
     res["roofline_frac"] = round(res["achieved_gbs"] / HBM_PEAK_GBS, 4)
     if cpu:
@@ -780,6 +782,7 @@ def main():
                             "note": "rays, a shading vertex per extension ray (upper bound) and a light sample per shadow ray, plus accumulation"},
         "mrays_s": round(rnd["mrays_s"], 1),
         "rays_per_sample": round(rnd["rays_per_sample"], 3),
+        "finish_paths": int(rnd["finish_paths"]),
         "node_visits_per_ray": round(rnd["node_visits_per_ray"], 2),
         "tri_tests_per_ray": round(rnd["tri_tests_per_ray"], 2),
         "display": rnd.get("display"),

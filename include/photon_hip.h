@@ -68,6 +68,10 @@ typedef struct ph_wavefront_config {
                               within rounding of the in-order sum, and free of the tail of pixels with long paths.
                               With S = spp every partial sum is one sample and the accumulation is the in-order one,
                               bit for bit */
+    int32_t finish_percent;/* 0-100. Once every work item of a render is handed out and at most this percentage of the
+                              path slots still holds a path, one kernel runs each remaining path to the end of its work
+                              item in its own thread (the per-pixel kernel's order, so the image is unchanged) instead of
+                              draining them over many sparse bounces. 0: off. Not used when serving */
 } ph_wavefront_config;
 
 /* Statistics of the last render call. After render_denoised, ms_shade is the one-sample G-buffer trace
@@ -85,6 +89,9 @@ typedef struct ph_render_stats {
     /* Filled only when counting is enabled: closest-hit (extension) and any-hit (shadow) traversal work. */
     int64_t node_visits, tri_tests, alpha_tests;
     int64_t shadow_node_visits, shadow_tri_tests, shadow_alpha_tests;
+    /* Wavefront engine: paths finished one per thread at the end of the render (finish_percent), and the
+     * extension and shadow rays they traced (not in rays_extend / rays_shadow, nor in the counts above). */
+    int64_t finish_paths, rays_finish;
 } ph_render_stats;
 
 /* The five G-buffer planes the reference denoiser consumes (SvgfInput, svgf.h:24-30), as device

@@ -23,14 +23,34 @@ struct PrimaryOut {
     f3 emission;
 };
 
+// A path between two vertices: what it has gathered, its throughput, the previous vertex's BSDF pdf and
+// whether that vertex skipped light sampling, and the depth of the next vertex. A new path starts at
+// depth 0 with nothing gathered, unit throughput and prev_delta set.
+struct PathState {
+    f3 L, beta;
+    float prev_pdf;
+    bool prev_delta;
+    int depth;
+};
+
+// Rays a path traced (extension and shadow), when the caller counts them.
+struct RayTally {
+    unsigned ext, shadow;
+};
+
+// The path from its next extension ray (ro, rd) on, with the generator in the state it has at that
+// point: the remaining vertices exactly as a path that started here would run them. trace_path is the
+// whole path; the wavefront engine resumes its last paths in flight with this.
 template <class Stack, bool Count>
-PH_DEV f3 trace_path(const DScene& s, f3 ro, f3 rd, Pcg& rng, const PathOpts& opt, PrimaryOut* primary, Stack& stack, TraceCount* cnt) {
-    f3 L = mk3(0), beta = mk3(1);
-    float prev_pdf = 0;
-    bool prev_delta = true;
+PH_DEV f3 trace_path_from(const DScene& s, f3 ro, f3 rd, Pcg& rng, const PathOpts& opt, PathState ps, PrimaryOut* primary, Stack& stack,
+                          TraceCount* cnt, RayTally* tally) {
+    f3 L = ps.L, beta = ps.beta;
+    float prev_pdf = ps.prev_pdf;
+    bool prev_delta = ps.prev_delta;
     const bool has_lights = s.p_env + s.p_emissive + s.p_punctual > 0;
-    for (int depth = 0;; depth++) {
+    for (int depth = ps.depth;; depth++) {
         HitRec hit{kInf, -1, 0.0f, 0.0f};
+        if (tally) tally->ext++;
         if (!traverse<false, Count>(s, ro, rd, kInf, hit, stack, cnt)) {
             f3 le = sky_radiance(s, rd);
             float w = (prev_delta || s.p_env <= 0) ? 1.0f : mis_power(prev_pdf, sky_pdf(s, rd));
@@ -78,6 +98,7 @@ PH_DEV f3 trace_path(const DScene& s, f3 ro, f3 rd, Pcg& rng, const PathOpts& op
                     f3 so = nudge(it.p, dot(ls.wi, it.ng) > 0 ? it.ng : -it.ng);
                     float stmax = ls.dist == kInf ? kInf : ls.dist * (1.0f - 1e-4f);
                     HitRec sh{kInf, -1, 0.0f, 0.0f};
+                    if (tally) tally->shadow++;
                     if (!traverse<true, Count>(s, so, ls.wi, stmax, sh, stack, cnt)) {
                         float w = ls.delta ? 1.0f : mis_power(ls.pdf, pdf_b);
                         L = L + clamp_contribution(beta * f * fabsf(wi.z) * ls.radiance * (w / ls.pdf), depth, opt.clamp);
@@ -108,18 +129,24 @@ PH_DEV f3 trace_path(const DScene& s, f3 ro, f3 rd, Pcg& rng, const PathOpts& op
     return L;
 }
 
+template <class Stack, bool Count>
+PH_DEV f3 trace_path(const DScene& s, f3 ro, f3 rd, Pcg& rng, const PathOpts& opt, PrimaryOut* primary, Stack& stack, TraceCount* cnt,
+                     RayTally* tally = nullptr) {
+    return trace_path_from<Stack, Count>(s, ro, rd, rng, opt, PathState{mk3(0), mk3(1), 0.0f, true, 0}, primary, stack, cnt, tally);
+}
+
 // render_sample: seeds the pixel's generator, draws jitter and lens, traces the path, and zeroes a
 // non-finite or negative result.
 template <class Stack, bool Count>
 PH_DEV f3 pixel_sample(const DScene& s, const ph_camera_view& cam, int x, int y, int width, int height, uint32_t frame, uint32_t seed,
-                       const PathOpts& opt, PrimaryOut* primary, Stack& stack, TraceCount* cnt) {
+                       const PathOpts& opt, PrimaryOut* primary, Stack& stack, TraceCount* cnt, RayTally* tally = nullptr) {
     uint32_t pixel = uint32_t(y) * uint32_t(width) + uint32_t(x);
     Pcg rng = pixel_rng(pixel, frame, seed);
     f2 jitter = pcg_uniform2(rng);
     f2 lens = pcg_uniform2(rng);
     f3 o, d;
     eye_ray(cam, float(x) + jitter.x, float(y) + jitter.y, width, height, lens, o, d);
-    f3 L = trace_path<Stack, Count>(s, o, d, rng, opt, primary, stack, cnt);
+    f3 L = trace_path<Stack, Count>(s, o, d, rng, opt, primary, stack, cnt, tally);
     if (!finite3(L) || L.x < 0 || L.y < 0 || L.z < 0) L = mk3(0);
     return L;
 }

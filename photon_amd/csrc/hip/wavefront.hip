@@ -26,6 +26,8 @@
 //   shadow    any-hit traversal of the NEE queue; unoccluded contributions are added to L
 //   regen     finished paths: add L to the pixel, start the next sample or the next work item
 //   compact   -> extension queue of the next iteration
+//   finish    (a single render, once every item is handed out and few paths are left) each remaining
+//             path and the rest of its work item in one thread, as the per-pixel kernel runs them
 // Traversal, shading and regeneration kernels are persistent: a fixed grid whose waves take the
 // queue's 64-entry batches in turn (no atomics: atomics on one address serialize). Traversal lanes are
 // refilled one ray at a time from the wave's batches, so a wave is not held by its longest ray.
@@ -763,6 +765,77 @@ __global__ void advance_items_kernel(unsigned long long* items, const int* count
     if (threadIdx.x == 0) items[0] += (unsigned long long)counters[C_NEED];
 }
 
+// ---- the end of a single render ------------------------------------------------------------------------
+// Once every work item is handed out, the paths still in flight drain over many bounces, each with few
+// paths and the fixed cost of every launch. When the extension queue holds at most `limit` paths and no
+// item is left, finish_kernel instead runs each of them to the end of its work item in one thread, as
+// the per-pixel kernel runs a pixel: the path from its pending extension ray (trace_path_from, same
+// vertices, same draws), then the item's remaining samples, accumulated as regen_kernel accumulates
+// them. The queue is then cleared (finish_clear_kernel) and the rest of the iteration finds no work.
+constexpr int kFinishBlock = 128, kFinishDepth = 32;
+
+PH_DEV bool finish_now(const int* counters, const unsigned long long* items, int limit) {
+    const int n = counters[C_EXT];
+    return n > 0 && n <= limit && items[0] >= items[1];
+}
+
+// tally: [0] extension rays, [1] shadow rays, [2] paths finished.
+__global__ __launch_bounds__(kFinishBlock) __attribute__((amdgpu_waves_per_eu(4))) void finish_kernel(Work work, State st, const int* counters, int limit,
+                                                                                                        unsigned long long* rays) {
+    if (!finish_now(counters, work.items, limit)) return;
+    __shared__ int lds[kFinishDepth * kFinishBlock];
+    SpillArea<kFinishDepth> spill;
+    LaneStack<kFinishDepth> stack{lds + threadIdx.x, kFinishBlock, spill.v};
+    const int n = counters[C_EXT];
+    const JobParams jp = single_job<false>(work);
+    const bool parts = jp.split > 1;
+    RayTally tally{0, 0};
+    for (int i = int(blockIdx.x * kFinishBlock + threadIdx.x); i < n; i += int(gridDim.x) * kFinishBlock) {
+        const int slot = st.q_ext[i];
+        const int pixel = st.pixel[slot];
+        const int x = pixel % jp.width, y = pixel / jp.width;
+        int k = st.sample[slot];
+        const int end = parts ? st.sample_end[slot] : jp.spp;
+        float4* acc = jp.accum + (parts ? st.part[slot] * jp.pixels : 0) + pixel;
+        float4 a = *acc;
+        const float4 o4 = st.ro[slot], d4 = st.rd[slot];
+        const int flags = __float_as_int(d4.w);
+        const PathState ps{xyz(st.L[slot]), xyz(st.beta[slot]), o4.w, ((flags >> 16) & 1) != 0, flags & 0xFFFF};
+        const bool first = ps.depth == 0 && k == 0 && jp.albedo;
+        PrimaryOut prim{mk3(0), 0.0f, mk3(0), -1, mk3(1), mk3(0)};
+        Pcg rng = slot_rng(st, slot, pixel);
+        f3 L = trace_path_from<LaneStack<kFinishDepth>, false>(jp.scene, xyz(o4), xyz(d4), rng, jp.opt, ps, first ? &prim : nullptr, stack, nullptr,
+                                                                &tally);
+        if (first) {
+            jp.albedo[pixel] = make_float4(prim.albedo.x, prim.albedo.y, prim.albedo.z, 1.0f);
+            jp.nd[pixel] = make_float4(prim.normal.x, prim.normal.y, prim.normal.z, prim.depth);
+        }
+        for (;;) {
+            if (!finite3(L) || L.x < 0 || L.y < 0 || L.z < 0) L = mk3(0);
+            a.x += L.x;
+            a.y += L.y;
+            a.z += L.z;
+            const float l = lum(L);
+            a.w += l * l;
+            if (++k >= end) break;
+            L = pixel_sample<LaneStack<kFinishDepth>, false>(jp.scene, jp.camera, x, y, jp.width, jp.height, jp.frame0 + uint32_t(k), jp.seed, jp.opt,
+                                                             nullptr, stack, nullptr, &tally);
+        }
+        *acc = a;
+        st.stage[slot] = S_IDLE;
+    }
+    const unsigned long long e = wave_sum(tally.ext), sh = wave_sum(tally.shadow);
+    if (lane_id() == 0) {
+        atomicAdd(rays, e);
+        atomicAdd(rays + 1, sh);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(rays + 2, (unsigned long long)n);
+}
+
+__global__ void finish_clear_kernel(int* counters, const unsigned long long* items, int limit) {
+    if (threadIdx.x == 0 && finish_now(counters, items, limit)) counters[C_EXT] = 0;
+}
+
 // The end of a single render in sub-streams: adds the partial sums to the accumulation, sub-stream 0
 // first.
 __global__ void add_parts_kernel(const float4* parts, int pixels, int count, float4* accum) {
@@ -859,7 +932,7 @@ std::vector<int> material_classes(const DScene& scene, hipStream_t st) {
 }
 
 struct WavefrontEngine::Impl {
-    ph_wavefront_config cfg{128, 24, 0, 16, 1, 1, 16, 16, 0, 1, 1, 1};
+    ph_wavefront_config cfg{128, 24, 0, 16, 1, 1, 16, 16, 0, 1, 1, 1, 10};
     int slots = 0;
     int cus = 0;
     DevBuf<int> job, pixel, sample, sample_end, part, counters, q_ext, q_shadow, q_term, q_miss, q_cls[kClasses], mat_class, partial, first_pos;
@@ -873,7 +946,7 @@ struct WavefrontEngine::Impl {
     DevBuf<float4> part_sums;  // partial sums of a single render in sub-streams
     DevBuf<JobParams> jobs;
     DevBuf<TraceTotals> totals;             // [0] extension rays, [1] shadow rays
-    DevBuf<unsigned long long> ray_totals;  // rays traced: [0] extension, [1] shadow
+    DevBuf<unsigned long long> ray_totals;  // rays traced: [0] extension, [1] shadow; finish_kernel: [2] extension, [3] shadow, [4] paths
     int* host_counters = nullptr;
     int* host_done = nullptr;
     int* host_sizes = nullptr;
@@ -916,7 +989,7 @@ struct WavefrontEngine::Impl {
         chunk_pass.allocate(kMaxChunks);
         job_done.allocate(kMaxJobs);
         totals.allocate(2);
-        ray_totals.allocate(2);
+        ray_totals.allocate(5);
         cus = device_cus();
         entry_used.assign(kMaxJobs, false);
         entry_pixels.assign(kMaxJobs, 0);
@@ -1009,9 +1082,9 @@ struct WavefrontEngine::Impl {
     template <bool Multi>
     void iteration(hipStream_t st, const ShapeFns& fns, bool count, hipEvent_t* ev) {
         const int tgrid = cus * std::max(1, cfg.blocks_per_cu);
-        // Shading: 4 blocks of 256 threads per CU, which is what the shading kernels' registers allow (116
-        // VGPRs, 4 waves per SIMD). Compiling them for 6 or 8 waves (with a grid to match) spills and was
-        // slower: 940 and 897 against 995 Msamples/s.
+        // Shading: 4 blocks of 256 threads per CU. The shading kernels use 232 VGPRs (2 waves per SIMD), so
+        // half of the blocks start as the others finish their share. Compiling them for 6 or 8 waves (with a
+        // grid to match) spills and was slower: 940 and 897 against 995 Msamples/s.
         const int sgrid = cus * 4;
         const State s = state();
         const Work w = work();
@@ -1035,6 +1108,11 @@ struct WavefrontEngine::Impl {
         hipLaunchKernelGGL(regen_kernel<Multi>, dim3(sgrid), dim3(256), 0, st, w, s, cnt);
         hipLaunchKernelGGL(advance_items_kernel, dim3(1), dim3(64), 0, st, items.ptr, cnt);
         compact<K_EXT, Multi>(st, s);
+        const int finish_limit = int(int64_t(slots) * std::clamp(cfg.finish_percent, 0, 100) / 100);
+        if (!Multi && finish_limit > 0) {
+            hipLaunchKernelGGL(finish_kernel, dim3(cus * 8), dim3(kFinishBlock), 0, st, w, s, cnt, finish_limit, ray_totals.ptr + 2);
+            hipLaunchKernelGGL(finish_clear_kernel, dim3(1), dim3(64), 0, st, cnt, items.ptr, finish_limit);
+        }
         PH_CHECK(hipGetLastError());
     }
 
@@ -1103,7 +1181,7 @@ struct WavefrontEngine::Impl {
         PH_CHECK(hipMemcpyAsync(counters.ptr + C_PARTS, host_sizes + 1, sizeof(int), hipMemcpyHostToDevice, st));
         PH_CHECK(hipMemsetAsync(items.ptr, 0, 2 * sizeof(unsigned long long), st));
         PH_CHECK(hipMemsetAsync(totals.ptr, 0, 2 * sizeof(TraceTotals), st));
-        PH_CHECK(hipMemsetAsync(ray_totals.ptr, 0, 2 * sizeof(unsigned long long), st));
+        PH_CHECK(hipMemsetAsync(ray_totals.ptr, 0, 5 * sizeof(unsigned long long), st));
     }
 
     JobParams params_of(const WavefrontJob& job, const int* cls) const {
@@ -1139,6 +1217,7 @@ void WavefrontEngine::set_config(const ph_wavefront_config& c) {
     if (c.ray_bins != 0 && c.ray_bins != 1) throw std::invalid_argument("ray_bins must be 0 or 1");
     if (c.sample_split < 1 || c.sample_split > 256) throw std::invalid_argument("sample_split must be in [1, 256]");
     if (c.xcd_parts != 1 && c.xcd_parts != 8) throw std::invalid_argument("xcd_parts must be 1 or 8");
+    if (c.finish_percent < 0 || c.finish_percent > 100) throw std::invalid_argument("finish_percent must be in [0, 100]");
     if (c.lds_nodes < 0 || c.lds_nodes > DeviceScene::kTopNodes || size_t(c.block) * size_t(c.stack_depth) * 4 + size_t(c.lds_nodes) * 64 > 65536)
         throw std::invalid_argument("lds_nodes does not fit the block's LDS with its stacks");
     impl_->cfg = c;
@@ -1206,10 +1285,12 @@ void WavefrontEngine::render(hipStream_t st, const WavefrontJob& job, bool count
     }
     TraceTotals tot[2] = {};
     if (counting) PH_CHECK(hipMemcpy(tot, m.totals.ptr, sizeof(tot), hipMemcpyDeviceToHost));
-    unsigned long long rays[2] = {0, 0};
+    unsigned long long rays[5] = {0, 0, 0, 0, 0};
     PH_CHECK(hipMemcpy(rays, m.ray_totals.ptr, sizeof(rays), hipMemcpyDeviceToHost));
     stats.rays_extend = int64_t(rays[0]);
     stats.rays_shadow = int64_t(rays[1]) + m.host_counters[C_SHADOW];  // the last iteration's queue
+    stats.rays_finish = int64_t(rays[2] + rays[3]);
+    stats.finish_paths = int64_t(rays[4]);
     stats.ms_extend = ms_extend;
     stats.ms_shadow = ms_shadow;
     stats.iterations = iterations;

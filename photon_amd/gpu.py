@@ -26,7 +26,7 @@ class WavefrontConfig(C.Structure):
     _fields_ = [("block", C.c_int32), ("stack_depth", C.c_int32), ("path_slots", C.c_int32), ("blocks_per_cu", C.c_int32),
                 ("sort_materials", C.c_int32), ("use_graph", C.c_int32), ("graph_iterations", C.c_int32),
                 ("refill_lanes", C.c_int32), ("lds_nodes", C.c_int32), ("ray_bins", C.c_int32), ("xcd_parts", C.c_int32),
-                ("sample_split", C.c_int32)]
+                ("sample_split", C.c_int32), ("finish_percent", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -38,7 +38,7 @@ class RenderStats(C.Structure):
                 ("shade_launches", C.c_int64), ("iterations", C.c_int64), ("rays_extend", C.c_int64),
                 ("rays_shadow", C.c_int64), ("samples", C.c_int64), ("node_visits", C.c_int64), ("tri_tests", C.c_int64),
                 ("alpha_tests", C.c_int64), ("shadow_node_visits", C.c_int64), ("shadow_tri_tests", C.c_int64),
-                ("shadow_alpha_tests", C.c_int64)]
+                ("shadow_alpha_tests", C.c_int64), ("finish_paths", C.c_int64), ("rays_finish", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

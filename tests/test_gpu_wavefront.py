@@ -128,3 +128,32 @@ def test_autotune_keeps_the_image():
     acc = gpu.copy_device_to_numpy(r.buffer_ptr(gpu.BUF_ACCUM), w * h * 16, np.float32, (h, w, 4))
     _, ref, _, _ = _render(scene, w, h, 4, gpu.ENGINE_MEGAKERNEL)
     assert acc.tobytes() == ref.tobytes()
+
+
+@pytest.mark.parametrize("name,w,h,spp,cfg", [
+    ("materials", 96, 54, 8, dict(finish_percent=10)),
+    ("materials", 96, 54, 8, dict(finish_percent=100)),
+    ("cornell-c1", 64, 64, 8, dict(finish_percent=100, path_slots=1000)),
+    ("glass", 80, 60, 8, dict(finish_percent=40, path_slots=3000, use_graph=0)),
+    ("cornell-c1", 64, 64, 16, dict(finish_percent=30, sample_split=4, path_slots=2048)),
+    ("materials", 64, 48, 8, dict(finish_percent=20, sample_split=8, path_slots=5000)),
+])
+def test_finishing_the_last_paths_one_per_thread_keeps_the_image(name, w, h, spp, cfg):
+    """The paths left once every work item is handed out, finished one per thread (finish_percent),
+    give the same accumulation and G-buffer as the engine without it, bit for bit, and so the per-pixel
+    kernel's image where the split keeps the in-order sum. With 100 they are finished after the first
+    bounce, with new items' first samples (sample 0, depth 0) among them when the slots are fewer than
+    the pixels."""
+    scene = host.Scene.builtin(name).build()
+    _, off_acc, off_g, off_st = _render(scene, w, h, spp, gpu.ENGINE_WAVEFRONT, cfg=dict(cfg, finish_percent=0))
+    _, on_acc, on_g, on_st = _render(scene, w, h, spp, gpu.ENGINE_WAVEFRONT, cfg=cfg)
+    print(f"{name} {cfg}: finished one per thread {on_st.finish_paths} paths, {on_st.rays_finish} rays; "
+          f"iterations {off_st.iterations} -> {on_st.iterations}")
+    assert off_st.finish_paths == 0 and off_st.rays_finish == 0
+    assert on_st.finish_paths > 0 and on_st.rays_finish > 0
+    assert on_acc.tobytes() == off_acc.tobytes()
+    assert on_g[0].tobytes() == off_g[0].tobytes() and on_g[1].tobytes() == off_g[1].tobytes()
+    if cfg.get("sample_split", 1) in (1, spp):
+        _, ref_acc, ref_g, _ = _render(scene, w, h, spp, gpu.ENGINE_MEGAKERNEL)
+        assert on_acc.tobytes() == ref_acc.tobytes()
+        assert on_g[0].tobytes() == ref_g[0].tobytes() and on_g[1].tobytes() == ref_g[1].tobytes()
