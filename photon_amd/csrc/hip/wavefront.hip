@@ -59,6 +59,7 @@ enum : int {
     C_CHUNKS = 10,  // entries of the chunk table
     C_PARTS = 11,   // 1, or 8 for per-XCD parts of the queues (wave_share)
     C_NEED = 12,    // the last C_NEED entries of the finished-path queue take a new work item
+    C_FINISH = 13,  // finish_kernel: extension-queue entries handed out so far
     C_COUNT = 16
 };
 
@@ -779,8 +780,10 @@ PH_DEV bool finish_now(const int* counters, const unsigned long long* items, int
     return n > 0 && n <= limit && items[0] >= items[1];
 }
 
-// tally: [0] extension rays, [1] shadow rays, [2] paths finished.
-__global__ __launch_bounds__(kFinishBlock) __attribute__((amdgpu_waves_per_eu(4))) void finish_kernel(Work work, State st, const int* counters, int limit,
+// tally: [0] extension rays, [1] shadow rays, [2] paths finished. Lanes take queue entries as they become
+// free (one atomic per wave and refill, on C_FINISH), and run one sample per loop turn, so a lane whose
+// item ends takes the next entry instead of idling while its wave finishes longer paths.
+__global__ __launch_bounds__(kFinishBlock) __attribute__((amdgpu_waves_per_eu(4))) void finish_kernel(Work work, State st, int* counters, int limit,
                                                                                                         unsigned long long* rays) {
     if (!finish_now(counters, work.items, limit)) return;
     __shared__ int lds[kFinishDepth * kFinishBlock];
@@ -790,39 +793,63 @@ __global__ __launch_bounds__(kFinishBlock) __attribute__((amdgpu_waves_per_eu(4)
     const JobParams jp = single_job<false>(work);
     const bool parts = jp.split > 1;
     RayTally tally{0, 0};
-    for (int i = int(blockIdx.x * kFinishBlock + threadIdx.x); i < n; i += int(gridDim.x) * kFinishBlock) {
-        const int slot = st.q_ext[i];
-        const int pixel = st.pixel[slot];
-        const int x = pixel % jp.width, y = pixel / jp.width;
-        int k = st.sample[slot];
-        const int end = parts ? st.sample_end[slot] : jp.spp;
-        float4* acc = jp.accum + (parts ? st.part[slot] * jp.pixels : 0) + pixel;
-        float4 a = *acc;
-        const float4 o4 = st.ro[slot], d4 = st.rd[slot];
-        const int flags = __float_as_int(d4.w);
-        const PathState ps{xyz(st.L[slot]), xyz(st.beta[slot]), o4.w, ((flags >> 16) & 1) != 0, flags & 0xFFFF};
-        const bool first = ps.depth == 0 && k == 0 && jp.albedo;
-        PrimaryOut prim{mk3(0), 0.0f, mk3(0), -1, mk3(1), mk3(0)};
-        Pcg rng = slot_rng(st, slot, pixel);
-        f3 L = trace_path_from<LaneStack<kFinishDepth>, false>(jp.scene, xyz(o4), xyz(d4), rng, jp.opt, ps, first ? &prim : nullptr, stack, nullptr,
-                                                                &tally);
-        if (first) {
-            jp.albedo[pixel] = make_float4(prim.albedo.x, prim.albedo.y, prim.albedo.z, 1.0f);
-            jp.nd[pixel] = make_float4(prim.normal.x, prim.normal.y, prim.normal.z, prim.depth);
+    bool live = false, resumed = false;
+    int slot = 0, pixel = 0, k = 0, end = 0;
+    float4* acc = nullptr;
+    float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (;;) {
+        const uint64_t idle = __ballot(!live);
+        if (idle) {
+            const int leader = __ffsll(static_cast<unsigned long long>(idle)) - 1;
+            int base = 0;
+            if (lane_id() == leader) base = atomicAdd(counters + C_FINISH, __popcll(idle));
+            base = __shfl(base, leader);
+            const int i = base + __popcll(idle & ((1ull << lane_id()) - 1ull));
+            if (!live && i < n) {
+                slot = st.q_ext[i];
+                pixel = st.pixel[slot];
+                k = st.sample[slot];
+                end = parts ? st.sample_end[slot] : jp.spp;
+                acc = jp.accum + (parts ? st.part[slot] * jp.pixels : 0) + pixel;
+                a = *acc;
+                live = true;
+                resumed = false;
+            }
         }
-        for (;;) {
+        if (__ballot(live) == 0) break;
+        if (live) {
+            f3 L;
+            if (!resumed) {
+                // The path in flight, from its pending extension ray and its generator's current state.
+                const float4 o4 = st.ro[slot], d4 = st.rd[slot];
+                const int flags = __float_as_int(d4.w);
+                const PathState ps{xyz(st.L[slot]), xyz(st.beta[slot]), o4.w, ((flags >> 16) & 1) != 0, flags & 0xFFFF};
+                const bool first = ps.depth == 0 && k == 0 && jp.albedo;
+                PrimaryOut prim{mk3(0), 0.0f, mk3(0), -1, mk3(1), mk3(0)};
+                Pcg rng = slot_rng(st, slot, pixel);
+                L = trace_path_from<LaneStack<kFinishDepth>, false>(jp.scene, xyz(o4), xyz(d4), rng, jp.opt, ps, first ? &prim : nullptr, stack,
+                                                                    nullptr, &tally);
+                if (first) {
+                    jp.albedo[pixel] = make_float4(prim.albedo.x, prim.albedo.y, prim.albedo.z, 1.0f);
+                    jp.nd[pixel] = make_float4(prim.normal.x, prim.normal.y, prim.normal.z, prim.depth);
+                }
+                resumed = true;
+            } else {
+                L = pixel_sample<LaneStack<kFinishDepth>, false>(jp.scene, jp.camera, pixel % jp.width, pixel / jp.width, jp.width, jp.height,
+                                                                 jp.frame0 + uint32_t(k), jp.seed, jp.opt, nullptr, stack, nullptr, &tally);
+            }
             if (!finite3(L) || L.x < 0 || L.y < 0 || L.z < 0) L = mk3(0);
             a.x += L.x;
             a.y += L.y;
             a.z += L.z;
             const float l = lum(L);
             a.w += l * l;
-            if (++k >= end) break;
-            L = pixel_sample<LaneStack<kFinishDepth>, false>(jp.scene, jp.camera, x, y, jp.width, jp.height, jp.frame0 + uint32_t(k), jp.seed, jp.opt,
-                                                             nullptr, stack, nullptr, &tally);
+            if (++k >= end) {
+                *acc = a;
+                st.stage[slot] = S_IDLE;
+                live = false;
+            }
         }
-        *acc = a;
-        st.stage[slot] = S_IDLE;
     }
     const unsigned long long e = wave_sum(tally.ext), sh = wave_sum(tally.shadow);
     if (lane_id() == 0) {
