@@ -55,3 +55,42 @@ def test_device_bvh_build_refuses_inputs_above_its_limit():
     assert limit == 262144
     with pytest.raises(RuntimeError, match="limited to PH_BVH_BUILD_MAX_TRIS"):
         gpu.bvh_build(np.zeros((3, 3), np.float32), np.zeros((limit + 1, 3), np.int32))
+
+
+LAYOUTS = [("gpu", "GpuInfo", "ph_gpu_info"), ("gpu", "WavefrontConfig", "ph_wavefront_config"), ("gpu", "RenderStats", "ph_render_stats"),
+           ("gpu", "ServerConfig", "ph_server_config"), ("gpu", "JobDesc", "ph_job_desc"), ("gpu", "JobStatus", "ph_job_status"),
+           ("gpu", "SvgfInput", "ph_svgf_input"), ("gpu", "BvhBuildStats", "ph_bvh_build_stats"), ("gpu", "CompactGBufferDesc", "ph_compact_gbuffer"),
+           ("types", "BvhNode", "ph_bvh_node"), ("types", "TriGeom", "ph_tri_geom"), ("types", "TextureInfo", "ph_texture_info"),
+           ("types", "Material", "ph_material"), ("types", "EmissiveTri", "ph_emissive_tri"), ("types", "AliasEntry", "ph_alias_entry"),
+           ("types", "PunctualLight", "ph_punctual_light"), ("types", "CameraView", "ph_camera_view"), ("types", "Ray", "ph_ray"),
+           ("types", "Hit", "ph_hit"), ("types", "PathOptions", "ph_path_options"), ("types", "RenderSettings", "ph_render_settings"),
+           ("types", "SvgfSettings", "ph_svgf_settings"), ("types", "SceneDesc", "ph_scene_desc"), ("types", "RawScene", "phs_raw_scene"),
+           ("types", "BvhStats", "phs_bvh_stats")]
+
+
+def test_ctypes_layouts_match_the_headers():
+    """Every ctypes mirror of a C ABI struct has the header's size and, field by field, its offsets (a
+    field added on one side only, or in another place, fails here)."""
+    import importlib
+    import shutil
+    import subprocess
+    import tempfile
+    if not shutil.which("gcc"):
+        pytest.skip("no host compiler")
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "photon_hip.h"', '#include "photon_host.h"', "int main(void) {"]
+    expect = []
+    for mod, cls, cname in LAYOUTS:
+        S = getattr(importlib.import_module("photon_amd." + mod), cls)
+        lines.append(f'    printf("%zu\\n", sizeof({cname}));')
+        expect.append(C.sizeof(S))
+        for name, _ in S._fields_:
+            lines.append(f'    printf("%zu\\n", offsetof({cname}, {name}));')
+            expect.append(getattr(S, name).offset)
+    lines.append("    return 0;\n}")
+    with tempfile.TemporaryDirectory() as d:
+        src, exe = os.path.join(d, "t.c"), os.path.join(d, "t")
+        open(src, "w").write("\n".join(lines))
+        r = subprocess.run(["gcc", "-std=c11", "-I" + os.path.join(ROOT, "include"), src, "-o", exe], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+        got = [int(x) for x in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
+    assert got == expect
