@@ -75,6 +75,15 @@ PH_DEV uint8_t ext_stage(int bin) { return uint8_t(S_EXT | (bin << 3)); }
 PH_DEV int ray_bin(f3 d, int bins) { return bins ? 1 + (d.x < 0 ? 1 : 0) + (d.y < 0 ? 2 : 0) + (d.z < 0 ? 4 : 0) : 1; }
 
 constexpr int kClasses = 4;
+
+// Path state is read and written once per stage: streaming (non-temporal) accesses, so that it does not
+// push the BVH and triangle lines out of the caches.
+using v4f = float __attribute__((ext_vector_type(4)));
+PH_DEV float4 ld_stream(const float4* p) {
+    const v4f v = __builtin_nontemporal_load(reinterpret_cast<const v4f*>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+PH_DEV void st_stream(float4* p, float4 a) { __builtin_nontemporal_store(v4f{a.x, a.y, a.z, a.w}, reinterpret_cast<v4f*>(p)); }
 constexpr int kMaxChunks = 1 << 16;
 // Ordered compaction: a block of 256 threads covers 4096 slots, 16 consecutive slots per thread.
 constexpr int kCompactThreads = 256, kCompactPerThread = 16, kCompactChunk = kCompactThreads * kCompactPerThread;
@@ -229,10 +238,10 @@ PH_DEV void start_sample(const JobParams& jp, const State& st, int slot, int pix
     st.pixel[slot] = pixel;
     st.sample[slot] = k;
     st.rng[slot] = rng.state;
-    st.ro[slot] = make_float4(o.x, o.y, o.z, 0.0f);
-    st.rd[slot] = make_float4(d.x, d.y, d.z, __int_as_float(1 << 16));
-    st.beta[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-    st.L[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    st_stream(st.ro + slot, make_float4(o.x, o.y, o.z, 0.0f));
+    st_stream(st.rd + slot, make_float4(d.x, d.y, d.z, __int_as_float(1 << 16)));
+    st_stream(st.beta + slot, make_float4(1.0f, 1.0f, 1.0f, 0.0f));
+    st_stream(st.L + slot, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
 }
 
 // Starts work item `w` on `slot`: finds its job and pixel, records them, seeds the first sample.
@@ -529,12 +538,12 @@ __global__ __launch_bounds__(Block) void extend_kernel(Work work, State st, int*
     trace_queue<false, Count, Multi>(
         work, st, st.q_ext, counters[C_EXT], counters[C_PARTS], refill, lds_nodes, top_nodes, stack, cnt,
         [&](int slot, Walker& w, HitRec& hit) {
-            const float4 o4 = st.ro[slot], d4 = st.rd[slot];
+            const float4 o4 = ld_stream(st.ro + slot), d4 = ld_stream(st.rd + slot);
             w = walker(xyz(o4), xyz(d4), kInf);
             hit = HitRec{kInf, -1, 0.0f, 0.0f};
         },
         [&](int slot, int entry, const TraceScene& ts, const Walker&, const HitRec& hit) {
-            st.hit[slot] = make_float4(hit.t, __int_as_float(hit.tri), hit.u, hit.v);
+            st_stream(st.hit + slot, make_float4(hit.t, __int_as_float(hit.tri), hit.u, hit.v));
             uint8_t next = S_MISS;
             if (hit.tri >= 0) {
                 const int* cls = work.jobs[entry].mat_class;
@@ -555,18 +564,18 @@ __global__ __launch_bounds__(Block) void shadow_kernel(Work work, State st, int*
     trace_queue<true, Count, Multi>(
         work, st, st.q_shadow, counters[C_SHADOW], counters[C_PARTS], refill, lds_nodes, top_nodes, stack, cnt,
         [&](int slot, Walker& w, HitRec& hit) {
-            const float4 o4 = st.so[slot], d4 = st.sd[slot];
+            const float4 o4 = ld_stream(st.so + slot), d4 = ld_stream(st.sd + slot);
             w = walker(xyz(o4), xyz(d4), o4.w);
             hit = HitRec{kInf, -1, 0.0f, 0.0f};
         },
         [&](int slot, int, const TraceScene&, const Walker& w, const HitRec&) {
             if (w.found) return;
-            const float4 c = st.sc[slot];
-            float4 L = st.L[slot];
+            const float4 c = ld_stream(st.sc + slot);
+            float4 L = ld_stream(st.L + slot);
             L.x += c.x;
             L.y += c.y;
             L.z += c.z;
-            st.L[slot] = L;
+            st_stream(st.L + slot, L);
         });
     if (Count) flush_counts(cnt, totals);
 }
@@ -582,17 +591,17 @@ __global__ __launch_bounds__(256) void miss_kernel(Work work, State st, int* cou
             const int slot = st.q_miss[i];
             const JobParams& jp = job_of<Multi>(work, st, slot, single);
             const DScene& s = jp.scene;
-            const float4 o4 = st.ro[slot], d4 = st.rd[slot];
+            const float4 o4 = ld_stream(st.ro + slot), d4 = ld_stream(st.rd + slot);
             const f3 rd = xyz(d4);
             const int flags = __float_as_int(d4.w);
             const int depth = flags & 0xFFFF;
             const bool prev_delta = (flags >> 16) & 1;
-            const float4 b4 = st.beta[slot];
-            const float4 L4 = st.L[slot];
+            const float4 b4 = ld_stream(st.beta + slot);
+            const float4 L4 = ld_stream(st.L + slot);
             const f3 le = sky_radiance(s, rd);
             const float w = (prev_delta || s.p_env <= 0) ? 1.0f : mis_power(o4.w, sky_pdf(s, rd));
             const f3 L = xyz(L4) + clamp_contribution(xyz(b4) * le * w, depth, jp.opt.clamp);
-            st.L[slot] = make_float4(L.x, L.y, L.z, 0.0f);
+            st_stream(st.L + slot, make_float4(L.x, L.y, L.z, 0.0f));
             if (depth == 0 && st.sample[slot] == 0 && jp.albedo) {
                 const int p = st.pixel[slot];
                 jp.albedo[p] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
@@ -621,14 +630,14 @@ __global__ __launch_bounds__(256) void shade_kernel(Work work, State st, int* co
             const PathOpts opt = jp.opt;
             const bool has_lights = s.p_env + s.p_emissive + s.p_punctual > 0;
             const int pixel = st.pixel[slot];
-            const float4 o4 = st.ro[slot], d4 = st.rd[slot], h4 = st.hit[slot];
+            const float4 o4 = ld_stream(st.ro + slot), d4 = ld_stream(st.rd + slot), h4 = ld_stream(st.hit + slot);
             const f3 rd = xyz(d4);
             const int flags = __float_as_int(d4.w);
             const int depth = flags & 0xFFFF;
             const bool prev_delta = (flags >> 16) & 1;
             const float prev_pdf = o4.w;
-            f3 beta = xyz(st.beta[slot]);
-            f3 L = xyz(st.L[slot]);
+            f3 beta = xyz(ld_stream(st.beta + slot));
+            f3 L = xyz(ld_stream(st.L + slot));
             const int tri = __float_as_int(h4.y);
             const Surface it = shade_point(s, xyz(o4), rd, h4.x, tri, h4.z, h4.w);
             const f3 wo_world = -rd;
@@ -661,9 +670,9 @@ __global__ __launch_bounds__(256) void shade_kernel(Work work, State st, int* co
                             const float stmax = ls.dist == kInf ? kInf : ls.dist * (1.0f - 1e-4f);
                             const float w = ls.delta ? 1.0f : mis_power(ls.pdf, pdf_b);
                             const f3 c = clamp_contribution(beta * f * fabsf(wi.z) * ls.radiance * (w / ls.pdf), depth, opt.clamp);
-                            st.so[slot] = make_float4(so.x, so.y, so.z, stmax);
-                            st.sd[slot] = make_float4(ls.wi.x, ls.wi.y, ls.wi.z, 0.0f);
-                            st.sc[slot] = make_float4(c.x, c.y, c.z, 0.0f);
+                            st_stream(st.so + slot, make_float4(so.x, so.y, so.z, stmax));
+                            st_stream(st.sd + slot, make_float4(ls.wi.x, ls.wi.y, ls.wi.z, 0.0f));
+                            st_stream(st.sc + slot, make_float4(c.x, c.y, c.z, 0.0f));
                             shadow = true;
                         }
                     }
@@ -690,16 +699,16 @@ __global__ __launch_bounds__(256) void shade_kernel(Work work, State st, int* co
                             }
                             if (cont) {
                                 bin = ray_bin(nd, bins);
-                                st.ro[slot] = make_float4(no.x, no.y, no.z, bs.pdf);
-                                st.rd[slot] = make_float4(nd.x, nd.y, nd.z, __int_as_float((depth + 1) | ((nee ? 0 : 1) << 16)));
-                                st.beta[slot] = make_float4(beta.x, beta.y, beta.z, 0.0f);
+                                st_stream(st.ro + slot, make_float4(no.x, no.y, no.z, bs.pdf));
+                                st_stream(st.rd + slot, make_float4(nd.x, nd.y, nd.z, __int_as_float((depth + 1) | ((nee ? 0 : 1) << 16))));
+                                st_stream(st.beta + slot, make_float4(beta.x, beta.y, beta.z, 0.0f));
                             }
                         }
                     }
                 }
                 st.rng[slot] = rng.state;
             }
-            st.L[slot] = make_float4(L.x, L.y, L.z, 0.0f);
+            st_stream(st.L + slot, make_float4(L.x, L.y, L.z, 0.0f));
             st.stage[slot] = uint8_t((cont ? ext_stage(bin) : term_stage<Multi>(jp, st, slot)) | (shadow ? S_SHADOW : 0));
         }
     }
@@ -729,7 +738,7 @@ __global__ __launch_bounds__(256) void regen_kernel(Work work, State st, int* co
             const JobParams& jp = job_of<Multi>(work, st, slot, single);
             const int pixel = st.pixel[slot];
             int k = st.sample[slot];
-            f3 L = xyz(st.L[slot]);
+            f3 L = xyz(ld_stream(st.L + slot));
             if (!finite3(L) || L.x < 0 || L.y < 0 || L.z < 0) L = mk3(0);
             const bool parts = Multi || jp.split > 1;
             float4* acc = jp.accum + (parts ? st.part[slot] * jp.pixels : 0) + pixel;
@@ -821,9 +830,9 @@ __global__ __launch_bounds__(kFinishBlock) __attribute__((amdgpu_waves_per_eu(4)
             f3 L;
             if (!resumed) {
                 // The path in flight, from its pending extension ray and its generator's current state.
-                const float4 o4 = st.ro[slot], d4 = st.rd[slot];
+                const float4 o4 = ld_stream(st.ro + slot), d4 = ld_stream(st.rd + slot);
                 const int flags = __float_as_int(d4.w);
-                const PathState ps{xyz(st.L[slot]), xyz(st.beta[slot]), o4.w, ((flags >> 16) & 1) != 0, flags & 0xFFFF};
+                const PathState ps{xyz(ld_stream(st.L + slot)), xyz(ld_stream(st.beta + slot)), o4.w, ((flags >> 16) & 1) != 0, flags & 0xFFFF};
                 const bool first = ps.depth == 0 && k == 0 && jp.albedo;
                 PrimaryOut prim{mk3(0), 0.0f, mk3(0), -1, mk3(1), mk3(0)};
                 Pcg rng = slot_rng(st, slot, pixel);
