@@ -1427,11 +1427,18 @@ ph_wavefront_config WavefrontEngine::autotune(hipStream_t st, const WavefrontJob
         consider(c);
     }
     if (best.use_graph)
-        for (int gi : {4, 8, 16}) {
+        for (int gi : {1, 4, 8, 16}) {
             ph_wavefront_config c = best;
             c.graph_iterations = gi;
             consider(c);
         }
+    // Path completion threshold: short renders (few bounces per path, few samples) gain from finishing more of
+    // their paths one per thread.
+    for (int fp : {0, 30, 60}) {
+        ph_wavefront_config c = best;
+        c.finish_percent = fp;
+        consider(c);
+    }
     set_config(best);
     m.tuned[key] = best;
     PH_CHECK(hipMemsetAsync(job.accum, 0, accum_bytes, st));
