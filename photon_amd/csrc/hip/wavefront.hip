@@ -1379,7 +1379,8 @@ ph_wavefront_config WavefrontEngine::autotune(hipStream_t st, const WavefrontJob
         c.sample_split = split;
         consider(c);
     }
-    for (int64_t slots : {int64_t(pixels) / 4, int64_t(pixels) / 2, int64_t(0), int64_t(pixels) * 2, int64_t(pixels) * 4, int64_t(pixels) * 8}) {
+    for (int64_t slots : {int64_t(pixels) / 4, int64_t(pixels) / 2, int64_t(0), int64_t(pixels) * 2, int64_t(pixels) * 4, int64_t(pixels) * 8,
+                           int64_t(pixels) * 12, int64_t(pixels) * 15}) {
         if (slots > 0 && slots < 65536) continue;
         if (slots > pixels && (best.sample_split < 2 || slots > int64_t(pixels) * best.sample_split || slots > (int64_t(1) << 25))) continue;
         ph_wavefront_config c = best;
