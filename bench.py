@@ -53,7 +53,7 @@ B_SHADE, B_NEE, B_ACC = 48 + 12 + 36 + 24 + 80, 16 + 8 + 48 + 4, 32
 # MI355X. Neighbouring path-slot counts lie within the tuner's 2 % acceptance margin of each other, so a
 # tuning run may stop at one of them; the default run uses this one so that its figures and the committed
 # profiles describe the same launch. --autotune runs the tuner instead.
-TUNED_LAUNCH = ("block=128,stack_depth=8,path_slots=31104000,blocks_per_cu=16,sort_materials=1,use_graph=1,graph_iterations=16,"
+TUNED_LAUNCH = ("block=128,stack_depth=8,path_slots=31104000,blocks_per_cu=16,sort_materials=1,use_graph=1,graph_iterations=5,"
                 "refill_lanes=16,lds_nodes=0,ray_bins=1,xcd_parts=1,sample_split=64,finish_percent=10")
 
 

@@ -1428,7 +1428,7 @@ ph_wavefront_config WavefrontEngine::autotune(hipStream_t st, const WavefrontJob
         consider(c);
     }
     if (best.use_graph)
-        for (int gi : {1, 4, 8, 16}) {
+        for (int gi : {1, 2, 4, 5, 8, 10, 16}) {
             ph_wavefront_config c = best;
             c.graph_iterations = gi;
             consider(c);
